@@ -1,0 +1,139 @@
+"""Benchmark: path samples/s of Snelly's spectral SDF path tracer on one MI355X.
+
+Workload (one step): the Menger-sponge metal scene -- 4 iterations, half-size 1,
+analytic complex IOR n = 0.2 + 0.8 t, k = 3.5 - 1.5 t, Beckmann roughness 0.05,
+spectral (hero wavelength), 8 bounces -- rendered at 1024 x 1024 with 64 path
+samples per pixel from a freshly seeded accumulation state, i.e. 67,108,864
+path samples.  Scene data are synthetic (an analytic SDF); nothing is read from
+disk.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--cpu-seconds 15]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "path samples/s (plain vs batched/served) at RMSE≤1%; p50/p99 job latency"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def unbiased(rad):
+    rad = rad.reshape(-1, 4)
+    return rad[:, :3] * (rad[:, 3:4] / (rad[:, 3:4] - 1.0))
+
+
+def cpu_baseline(scene, width, height, spp, budget_s):
+    """The CPU oracle (a C++ restatement of the reference shaders, OpenMP) on a
+    bounded, strided sample of the same workload's pixels."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    threads = min(threads, 16)
+    o = oracle_py.Oracle(scene, width, height)
+    npix = width * height
+    # calibrate on a sparse sample, then size the stride to the time budget
+    stride = 4096
+    rad, rng = o.new_state(99)
+    t = time.perf_counter()
+    o.render(rad, rng, spp, stride, 0, threads)
+    dt = max(time.perf_counter() - t, 1e-3)
+    per_pixel = dt / math.ceil(npix / stride)
+    stride = max(1, int(math.ceil(npix * per_pixel / budget_s)))
+    rad, rng = o.new_state(99)
+    t = time.perf_counter()
+    o.render(rad, rng, spp, stride, 0, threads)
+    dt = time.perf_counter() - t
+    pixels = math.ceil(npix / stride)
+    return {"value": pixels * spp / dt, "unit": "path samples/s", "cores": threads, "kind": "port",
+            "sample": f"every {stride}th pixel of the {width}x{height} image ({pixels} pixels) x {spp} spp, "
+                      f"{dt:.1f} s on {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--width", type=int, default=1024)
+    ap.add_argument("--height", type=int, default=1024)
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the path is per-image data parallel "
+              f"and is run on a single MI355X here", file=sys.stderr)
+        sys.exit(2)
+
+    from snelly_amd import host, native, scenes
+
+    W, H, spp = args.width, args.height, args.spp
+    scene = scenes.menger_metal(W, H, spp)
+    ctx = native.Context(0)
+    sess = host.Session(scene, W, H, ctx)
+    sess.prog.profiling(True)
+    for w in range(args.warmup):
+        sess.state.seed(1000 + w)
+        sess.render(spp)
+    ctx.sync()
+    images = []
+    trace_ms = 0.0
+    trace_steps = 0
+    launches = 0
+    step_ms = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ts = time.perf_counter()
+        sess.state.seed(1234 + k)
+        sess.render(spp)
+        st = sess.prog.stats()
+        step_ms.append((time.perf_counter() - ts) * 1e3)
+        trace_ms += st.ms_trace
+        trace_steps += st.trace_steps
+        launches += st.trace_launches
+        if k < 2:
+            images.append(unbiased(sess.radiance()))
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    samples = W * H * spp * args.steps
+    value = samples / elapsed
+
+    flops = trace_steps * scene.flops_per_step
+    achieved = flops / (trace_ms * 1e-3) / 1e12
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "kernel": "snl_trace", "avg_launch_ms": round(trace_ms / launches, 4),
+                "flops_per_launch": flops / launches, "flops_per_step": scene.flops_per_step,
+                "steps_per_launch": trace_steps / launches}
+    rr = None
+    if len(images) == 2:
+        a, b = images
+        rr = float(np.linalg.norm(a - b) / math.sqrt(2.0) / np.linalg.norm(0.5 * (a + b)))
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "path samples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(elapsed * 1e3 / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (analytic SDF scene, PCG-seeded per-pixel RNG)",
+        "config": {"workload": "menger4_metal_spectral", "width": W, "height": H, "spp": spp, "bounces": 8,
+                   "mode": "plain", "path_samples_per_step": W * H * spp},
+        "roofline": roofline,
+        "rrmse_per_step": None if rr is None else round(rr, 5),
+        "spp_for_rrmse_1pct": None if rr is None else int(math.ceil(spp * (rr / 0.01) ** 2)),
+    }
+    sess.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(scenes.menger_metal(W, H, spp), W, H, spp, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,128 @@
+/* snelly.h -- C ABI of libsnelly_hip.so, the MI355X (gfx950) implementation of
+ * Snelly's path-tracing hot path.
+ *
+ * Every entry point replaces one WebGL2 operation that the reference's JS host
+ * performs for its shader programs (the "GLU" layer, js/gl.js, driven by
+ * js/renderer.js and js/snelly.js).  Handles are opaque; every call returns 0
+ * on success or a negative SNL_E_* code, with a message from snl_last_error().
+ * All calls on one context must come from one host thread (the reference's JS
+ * is single-threaded too).  Nothing in this header uses torch or HIP types.
+ */
+#ifndef SNELLY_H
+#define SNELLY_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SNL_OK 0
+#define SNL_E_ARG -1      /* bad argument */
+#define SNL_E_HIP -2      /* HIP runtime error */
+#define SNL_E_COMPILE -3  /* scene code failed to translate or compile */
+#define SNL_E_NOMEM -4    /* device allocation failed */
+#define SNL_E_UNSUPPORTED -5
+
+/* uniform / value type codes (GLSL types) */
+#define SNL_T_FLOAT 1
+#define SNL_T_VEC2 2
+#define SNL_T_VEC3 3
+#define SNL_T_VEC4 4
+#define SNL_T_INT 5
+#define SNL_T_MAT2 6
+#define SNL_T_MAT3 7
+#define SNL_T_MAT4 8
+#define SNL_T_SAMPLER2D 9
+#define SNL_T_SAMPLER3D 10
+
+/* program build flags */
+#define SNL_BUILD_PRECISE 1  /* no FMA contraction (-ffp-contract=off) */
+
+typedef struct snl_ctx_s* snl_ctx;
+typedef struct snl_prog_s* snl_prog;
+typedef struct snl_tex_s* snl_tex;
+typedef struct snl_state_s* snl_state;
+
+/* Context on one GPU: its stream and last-error string.
+ * Replaces the WebGL2 context set-up of GLU (js/gl.js:16-29). */
+int snl_ctx_create(int device, snl_ctx* out);
+int snl_ctx_destroy(snl_ctx ctx);
+const char* snl_last_error(snl_ctx ctx);   /* never NULL */
+const char* snl_version(void);
+
+/* Compile a program from what Renderer.compileShaders hands GLU.Shader
+ * (js/renderer.js:217-359 -> js/gl.js:179-229): the scene snippet with its
+ * default hooks (__SHADER__), the IOR code of the loaded materials
+ * (__IOR_FUNC__), the #define block (__DEFINES__) and the integer tokens
+ * __MAX_MARCH_STEPS__, __MAX_BOUNCES__, __MAX_ATMOSPHERE_SCATTERS__,
+ * __MAX_SAMPLES_PER_FRAME__.  One program holds the pathtracer, normals, pick
+ * and tonemapper kernels.  hipRTC output is cached on disk by source hash.
+ * On a compile error the translated source's diagnostics are in log. */
+int snl_program_build(snl_ctx ctx, const char* shader_glsl, const char* ior_glsl, const char* defines,
+                      int max_march_steps, int max_bounces, int max_atmosphere_scatters,
+                      int max_samples_per_frame, unsigned flags, snl_prog* out, char* log, int loglen);
+/* Compile and cache only (no GPU needed); same inputs as snl_program_build. */
+int snl_program_precompile(const char* shader_glsl, const char* ior_glsl, const char* defines,
+                           int max_march_steps, int max_bounces, int max_atmosphere_scatters,
+                           int max_samples_per_frame, unsigned flags, char* log, int loglen);
+int snl_program_destroy(snl_prog prog);
+
+/* GLU.Shader.uniformIndex (js/gl.js:255-260): -1 when the program has no such
+ * uniform, so setters on unknown names are silent no-ops as in WebGL. */
+int snl_uniform_index(snl_prog prog, const char* name);
+/* GLU.Shader.uniformF/I/2F/3F/1Fv/2Fv/3Fv/4Fv/Matrix4fv (js/gl.js:262-406):
+ * `data` holds `count` values of `type`; idx < 0 is ignored (returns 0). */
+int snl_uniform_set(snl_prog prog, int idx, int type, const void* data, int count);
+/* GLU.Shader.uniformTexture (js/gl.js:262-267) */
+int snl_uniform_texture(snl_prog prog, int idx, snl_tex tex);
+
+/* GLU.Texture(width, height, channels, isFloat, isLinear, isClamped, texels)
+ * (js/gl.js:485-523); depth > 1 makes a 3-D texture.  texels may be NULL. */
+int snl_texture_create(snl_ctx ctx, int width, int height, int depth, int channels, int linear, int clamp,
+                       const float* texels, snl_tex* out);
+/* GLU.Texture.copy (js/gl.js:503-507) */
+int snl_texture_upload(snl_tex tex, const float* texels);
+int snl_texture_destroy(snl_tex tex);
+
+/* PathtracerState pair (js/renderer.js:692-745): per-pixel radiance
+ * (X, Y, Z, N) and RNG state, resident in HBM. */
+int snl_state_create(snl_ctx ctx, int width, int height, snl_state* out);
+int snl_state_destroy(snl_state st);
+/* Radiance cleared to (0,0,0,1) as PathtracerState.clear does; RNG seeded per
+ * pixel from PCG32(pixel_index ^ seed) (integer states, exactly reproducible). */
+int snl_state_seed(snl_state st, unsigned seed);
+int snl_state_clear(snl_state st);
+/* which: 0 = radiance RGBA32F, 1 = RNG RGBA32F; 4*width*height floats, row 0 = bottom */
+int snl_state_read(snl_state st, int which, float* out);
+int snl_state_write(snl_state st, int which, const float* in);
+
+/* Renderer.render's pathtracer draw (js/renderer.js:462-644) repeated for
+ * `frames` frames: one path sample per pixel per frame per sample-per-frame. */
+int snl_render(snl_prog prog, snl_state st, int frames);
+/* the 'normals' render mode (normals-fragment-shader.glsl) */
+int snl_render_normals(snl_prog prog, snl_state st, int frames);
+/* Renderer.pick (js/renderer.js:398-460) for n pixel positions (gl_FragCoord
+ * units): out[2k] = hit distance (maxLengthScale if none), out[2k+1] = material */
+int snl_pick(snl_prog prog, const float* xy, int n, float* out);
+/* tonemapper pass (js/renderer.js:622-642); rgb (3 floats/px) and/or rgba8 may be NULL */
+int snl_tonemap(snl_prog prog, snl_state st, float* rgb, unsigned char* rgba8);
+int snl_sync(snl_ctx ctx);
+
+/* Timing and counters of the last snl_render call. */
+typedef struct snl_stats {
+  double ms_total;         /* wall time of the render call on the GPU (hipEvents) */
+  double ms_trace;         /* summed time of the trace kernel launches */
+  long long trace_steps;   /* SDF march steps executed (all rays) */
+  long long paths;         /* camera paths started */
+  int trace_launches;
+} snl_stats;
+int snl_get_stats(snl_prog prog, snl_stats* out);
+/* launch-shape knobs: trace grid (blocks), jobs fetched per wave, shade grid;
+ * 0 keeps the current value.  Also set by the autotuner. */
+int snl_set_tuning(snl_prog prog, int trace_blocks, int trace_chunk, int shade_blocks);
+/* 1 = time every trace launch with events (adds a sync per launch) */
+int snl_set_profiling(snl_prog prog, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1143 @@
+// oracle.cpp -- CPU restatement of Snelly's shader programs, used only as the
+// checker by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline.
+// It is test infrastructure, never the product path.
+//
+// Structure follows the reference megakernel line by line (one pixel per
+// loop iteration, cameraPath with inline traceRay and shadow rays):
+//   shaders/pathtracer-fragment-shader.glsl  (rand 119-130, BSDFs 379-852,
+//     traceRay 881-920, normal 922-940, lights 1051-1212,
+//     directSurfaceLighting 1221-1295, randomwalk_SSS 1468-1593,
+//     constructPrimaryRay 1602-1622, cameraPath 1625-1926, pathtrace 1934-1989)
+//   shaders/normals-fragment-shader.glsl     (traceDistance 70-113, main 293-345)
+//   shaders/pick-fragment-shader.glsl        (main 133-152)
+//   shaders/tonemapper-fragment-shader.glsl  (main 36-97)
+// Scenes are hand-written C++ SDFs (no GLSL translation), so the oracle is
+// independent of the product's scene compiler.  The homogeneous atmosphere
+// (HAS_ATMOSPHERE) is not restated here.
+//
+// Parity status of each part is recorded in the tests that pin it; see
+// tests/test_oracle.py (closed forms, invariants, brute force).
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "vec.h"
+
+using namespace orc;
+
+namespace {
+
+const float DENOM_TOLERANCE = 1.0e-7f, PDF_EPSILON = 1.0e-6f, THROUGHPUT_EPSILON = 1.0e-5f,
+            RADIANCE_EPSILON = 1.0e-6f;
+const float PI_F = 3.141592653589793f;
+enum { MAT_INVAL = -1, MAT_DIELE = 0, MAT_METAL = 1, MAT_SURFA = 2 };
+
+// ---------------------------------------------------------------- parameters (the uniforms)
+struct Params {
+  float resolution[2];
+  float camPos[3], camDir[3], camX[3], camY[3];
+  float camFovy, camAspect, camAperture, camFocalDistance;
+  float filterRadius, radianceClamp, skipProbability, shadowStrength;
+  float maxStepsIsMiss, wavelengthSamples;
+  float lengthScale, minLengthScale, maxLengthScale;
+  float haveEnvMap, envMapVisible, envMapPhiRotation, envMapThetaRotation, envMapTransitionAngle, envMapRotation;
+  float skyPower, skyTintUp[3], skyTintDown[3];
+  float sunPower, sunAngularSize, sunLatitude, sunLongitude, sunColor[3], sunDir[3], sunVisibleDirectly;
+  float sphereLightPosition[3], sphereLightRadius, sphereLightPower, sphereLightColor[3];
+  float metalRoughness, metalSpecAlbedoRGB[3];
+  float dieleRoughness, dieleAbsorptionRGB[3], dieleSpecAlbedoRGB[3];
+  float surfaceDiffuseAlbedoRGB[3], surfaceSpecAlbedoRGB[3], surfaceRoughness, surfaceIor, subsurface;
+  float subsurfaceAlbedoRGB[3], subsurfaceMFP, subsurfaceAnisotropy, subsurfaceDiffuseWeight, maxSSSSteps;
+  float exposure, invGamma, contrast, saturation, hueShift;
+};
+Params P;
+
+struct Named { const char* name; float* p; int n; };
+std::vector<Named> table() {
+  return {
+      {"resolution", P.resolution, 2}, {"camPos", P.camPos, 3}, {"camDir", P.camDir, 3}, {"camX", P.camX, 3},
+      {"camY", P.camY, 3}, {"camFovy", &P.camFovy, 1}, {"camAspect", &P.camAspect, 1},
+      {"camAperture", &P.camAperture, 1}, {"camFocalDistance", &P.camFocalDistance, 1},
+      {"filterRadius", &P.filterRadius, 1}, {"radianceClamp", &P.radianceClamp, 1},
+      {"skipProbability", &P.skipProbability, 1}, {"shadowStrength", &P.shadowStrength, 1},
+      {"maxStepsIsMiss", &P.maxStepsIsMiss, 1}, {"wavelengthSamples", &P.wavelengthSamples, 1},
+      {"lengthScale", &P.lengthScale, 1}, {"minLengthScale", &P.minLengthScale, 1},
+      {"maxLengthScale", &P.maxLengthScale, 1}, {"haveEnvMap", &P.haveEnvMap, 1},
+      {"envMapVisible", &P.envMapVisible, 1}, {"envMapPhiRotation", &P.envMapPhiRotation, 1},
+      {"envMapThetaRotation", &P.envMapThetaRotation, 1}, {"envMapTransitionAngle", &P.envMapTransitionAngle, 1},
+      {"envMapRotation", &P.envMapRotation, 1}, {"skyPower", &P.skyPower, 1}, {"skyTintUp", P.skyTintUp, 3},
+      {"skyTintDown", P.skyTintDown, 3}, {"sunPower", &P.sunPower, 1}, {"sunAngularSize", &P.sunAngularSize, 1},
+      {"sunLatitude", &P.sunLatitude, 1}, {"sunLongitude", &P.sunLongitude, 1}, {"sunColor", P.sunColor, 3},
+      {"sunDir", P.sunDir, 3}, {"sunVisibleDirectly", &P.sunVisibleDirectly, 1},
+      {"sphereLightPosition", P.sphereLightPosition, 3}, {"sphereLightRadius", &P.sphereLightRadius, 1},
+      {"sphereLightPower", &P.sphereLightPower, 1}, {"sphereLightColor", P.sphereLightColor, 3},
+      {"metalRoughness", &P.metalRoughness, 1}, {"metalSpecAlbedoRGB", P.metalSpecAlbedoRGB, 3},
+      {"dieleRoughness", &P.dieleRoughness, 1}, {"dieleAbsorptionRGB", P.dieleAbsorptionRGB, 3},
+      {"dieleSpecAlbedoRGB", P.dieleSpecAlbedoRGB, 3}, {"surfaceDiffuseAlbedoRGB", P.surfaceDiffuseAlbedoRGB, 3},
+      {"surfaceSpecAlbedoRGB", P.surfaceSpecAlbedoRGB, 3}, {"surfaceRoughness", &P.surfaceRoughness, 1},
+      {"surfaceIor", &P.surfaceIor, 1}, {"subsurface", &P.subsurface, 1},
+      {"subsurfaceAlbedoRGB", P.subsurfaceAlbedoRGB, 3}, {"subsurfaceMFP", &P.subsurfaceMFP, 1},
+      {"subsurfaceAnisotropy", &P.subsurfaceAnisotropy, 1},
+      {"subsurfaceDiffuseWeight", &P.subsurfaceDiffuseWeight, 1}, {"maxSSSSteps", &P.maxSSSSteps, 1},
+      {"exposure", &P.exposure, 1}, {"invGamma", &P.invGamma, 1}, {"contrast", &P.contrast, 1},
+      {"saturation", &P.saturation, 1}, {"hueShift", &P.hueShift, 1}};
+}
+inline V3 A3(const float* a) { return v3(a[0], a[1], a[2]); }
+
+// ---------------------------------------------------------------- build-time switches (the #defines)
+struct Config {
+  int dispersive = 0, interactive = 0, maxBounces = 3, maxMarch = 256, spf = 1, sphereLight = 0;
+} C;
+
+// ---------------------------------------------------------------- scene (hand-written SDFs)
+enum { SDF_NONE, SDF_SPHERE, SDF_SPHERE_PLANE, SDF_MENGER, SDF_MANDELBOX, SDF_TORUS };
+struct SdfSpec { int kind = SDF_NONE; float a = 0, b = 0, c = 0, d = 0; };
+SdfSpec sdfs[3];        // indexed by material id (DIELE, METAL, SURFA)
+int albedoHook = 0;     // 1: sphere/plane albedo (0.8 above y=-1+1e-3, else 0.5)
+
+// SDFs are templates on the scalar type: float for rendering, Counted for the
+// algorithmic FLOP count of one evaluation (+ - * / and sqrt count 1 each;
+// min/max/abs/floor/clamp are counted separately as non-FLOP VALU operations).
+struct Counted {
+  float v;
+  static long long flops, ops;
+};
+long long Counted::flops = 0, Counted::ops = 0;
+inline Counted cf(float v) { return Counted{v}; }
+inline Counted operator+(Counted a, Counted b) { ++Counted::flops; return {a.v + b.v}; }
+inline Counted operator-(Counted a, Counted b) { ++Counted::flops; return {a.v - b.v}; }
+inline Counted operator*(Counted a, Counted b) { ++Counted::flops; return {a.v * b.v}; }
+inline Counted operator/(Counted a, Counted b) { ++Counted::flops; return {a.v / b.v}; }
+inline Counted operator-(Counted a) { return {-a.v}; }
+inline bool operator>(Counted a, Counted b) { ++Counted::ops; return a.v > b.v; }
+inline Counted sqrt_(Counted a) { ++Counted::flops; return {std::sqrt(a.v)}; }
+inline Counted floor_(Counted a) { ++Counted::ops; return {std::floor(a.v)}; }
+inline Counted abs_(Counted a) { return {std::fabs(a.v)}; }  // a source modifier on gfx9
+inline Counted min_(Counted a, Counted b) { ++Counted::ops; return {fmin_(a.v, b.v)}; }
+inline Counted max_(Counted a, Counted b) { ++Counted::ops; return {fmax_(a.v, b.v)}; }
+inline Counted pow_(Counted a, Counted b) { ++Counted::flops; return {std::pow(a.v, b.v)}; }
+inline float sqrt_(float a) { return std::sqrt(a); }
+inline float floor_(float a) { return std::floor(a); }
+inline float abs_(float a) { return std::fabs(a); }
+inline float min_(float a, float b) { return fmin_(a, b); }
+inline float max_(float a, float b) { return fmax_(a, b); }
+inline float pow_(float a, float b) { return std::pow(a, b); }
+template <class T> inline T lit(float x) { return T{x}; }
+template <> inline float lit<float>(float x) { return x; }
+template <class T> inline T clamp_(T x, float lo, float hi) { return min_(max_(x, lit<T>(lo)), lit<T>(hi)); }
+
+template <class T>
+T sdBoxT(const T X[3], float size) {
+  T d[3];
+  for (int k = 0; k < 3; ++k) d[k] = abs_(X[k]) - lit<T>(size);
+  T m = min_(max_(d[0], max_(d[1], d[2])), lit<T>(0.0f));
+  T q[3];
+  for (int k = 0; k < 3; ++k) q[k] = max_(d[k], lit<T>(0.0f));
+  return m + sqrt_(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+}
+// exampleScenes/fractals/menger.html:116-134 (without the rotation), size a, scale b, iterations c
+template <class T>
+T mengerT(const T X[3], float size, float scale, int iter) {
+  T sd = sdBoxT(X, size);
+  T s = lit<T>(scale);
+  for (int i = 0; i < iter; i++) {
+    T a[3], r[3];
+    for (int k = 0; k < 3; ++k) {
+      T xs = X[k] * s;
+      a[k] = (xs - lit<T>(2.0f) * floor_(xs / lit<T>(2.0f))) - lit<T>(1.0f);
+    }
+    s = s * lit<T>(3.0f);
+    for (int k = 0; k < 3; ++k) r[k] = abs_(lit<T>(1.0f) - lit<T>(3.0f) * abs_(a[k]));
+    T da = max_(r[0], r[1]), db = max_(r[1], r[2]), dc = max_(r[2], r[0]);
+    T c = (min_(da, min_(db, dc)) - lit<T>(1.0f)) / s;
+    sd = max_(sd, c);
+  }
+  return sd;
+}
+// exampleScenes/fractals/mandelbox.html:122-135 (without the rotation); the
+// per-scene constants are INIT() work, outside the per-evaluation count
+template <class T>
+T mandelboxT(const T pos[3], float Scale, int iters, float bailout, float MinRad2) {
+  float sc[4] = {Scale / MinRad2, Scale / MinRad2, Scale / MinRad2, std::fabs(Scale) / MinRad2};
+  float absScalem1 = std::fabs(Scale - 1.0f);
+  float AbsScaleRaisedTo1mIters = std::pow(std::fabs(Scale), float(1 - iters));
+  T p[4] = {pos[0], pos[1], pos[2], lit<T>(1.0f)}, p0[4] = {pos[0], pos[1], pos[2], lit<T>(1.0f)};
+  for (int i = 0; i < iters; i++) {
+    for (int k = 0; k < 3; ++k) p[k] = clamp_(p[k], -1.0f, 1.0f) * lit<T>(2.0f) - p[k];
+    T r2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+    T f = clamp_(max_(lit<T>(MinRad2) / r2, lit<T>(MinRad2)), 0.0f, 1.0f);
+    for (int k = 0; k < 4; ++k) p[k] = p[k] * f;
+    for (int k = 0; k < 4; ++k) p[k] = p[k] * lit<T>(sc[k]) + p0[k];
+    if (r2 > lit<T>(bailout)) break;
+  }
+  return (sqrt_(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) - lit<T>(absScalem1)) / p[3] - lit<T>(AbsScaleRaisedTo1mIters);
+}
+template <class T>
+T evalSdfT(const SdfSpec& s, const T X[3]) {
+  switch (s.kind) {
+    case SDF_SPHERE: return sqrt_(X[0] * X[0] + X[1] * X[1] + X[2] * X[2]) - lit<T>(s.a);
+    case SDF_SPHERE_PLANE: return min_(sqrt_(X[0] * X[0] + X[1] * X[1] + X[2] * X[2]) - lit<T>(1.0f), X[1] + lit<T>(1.0f));
+    case SDF_MENGER: return mengerT(X, s.a, s.b, (int)s.c);
+    case SDF_MANDELBOX: return mandelboxT(X, s.a, (int)s.b, s.c, s.d);
+    case SDF_TORUS: {
+      T qx = sqrt_(X[0] * X[0] + X[2] * X[2]) - lit<T>(s.a);
+      return sqrt_(qx * qx + X[1] * X[1]) - lit<T>(s.b);
+    }
+  }
+  return lit<T>(1.0e20f);
+}
+float evalSdf(const SdfSpec& s, V3 X) {
+  float x[3] = {X.x, X.y, X.z};
+  return evalSdfT<float>(s, x);
+}
+bool hasMat(int m) { return sdfs[m].kind != SDF_NONE; }
+V3 surfaceDiffuseHook(V3 Cc, V3 X) {
+  if (albedoHook == 1) return X.y > -1.0f + 1.0e-3f ? v3(0.8f) : v3(0.5f);
+  return Cc;
+}
+
+// ---------------------------------------------------------------- materials' IOR code
+int dieleModel = 1;  // 0 Abbe, 1 Sellmeier (squared poles), 2 Sellmeier2, 3 Polyanskiy, 4 Cauchy
+std::vector<float> dieleC;
+int metalModel = 0;  // 0 tabulated, 1 analytic n = 0.2 + 0.8t, k = 3.5 - 1.5t
+std::vector<float> metalN(64, 1.0f), metalK(64, 0.0f);
+
+float IOR_DIELE(float wl) {
+  const std::vector<float>& c = dieleC;
+  auto Cn = [&](int i) { return i - 1 < (int)c.size() ? c[i - 1] : 0.0f; };
+  float um = 1.0e-3f * wl;
+  switch (dieleModel) {
+    case 0: {  // js/materials.js:846-861
+      float iorVal = Cn(1), abbe = Cn(2);
+      if (abbe <= 0.0f) return iorVal;
+      float nD2 = 0.5893f; nD2 *= nD2;
+      float nF2 = 0.4861f; nF2 *= nF2;
+      float nC2 = 0.6563f; nC2 *= nC2;
+      float cauchyC = (iorVal - 1.0f) / abbe * (nC2 * nF2) / (nC2 - nF2);
+      float cauchyB = iorVal - cauchyC / nD2;
+      return cauchyB + cauchyC / (um * um);
+    }
+    case 1: case 2: {  // js/materials.js:905-930, 956-981
+      float l2 = um * um;
+      float n2 = 1.0f + Cn(1);
+      int terms = ((int)c.size() - 1) / 2;
+      for (int t = 1; t <= terms; ++t) {
+        float pole = dieleModel == 1 ? Cn(2 * t + 1) * Cn(2 * t + 1) : Cn(2 * t + 1);
+        n2 = n2 + Cn(2 * t) * l2 / (l2 - pole);
+      }
+      return fmax_(std::sqrt(std::fabs(n2)), 1.0e-3f);
+    }
+    case 3: {  // js/materials.js:1014-1034
+      float n2 = Cn(1) + Cn(2) * std::pow(um, Cn(3)) / (um * um - std::pow(Cn(4), Cn(5)));
+      return fmax_(std::sqrt(std::fabs(n2)), 1.0e-3f);
+    }
+    case 4: {  // js/materials.js:1064-1089
+      float n = Cn(1);
+      int terms = ((int)c.size() - 1) / 2;
+      for (int t = 1; t <= terms; ++t) n = n + Cn(2 * t) * std::pow(um, Cn(2 * t + 1));
+      return fmax_(n, 1.0e-3f);
+    }
+  }
+  return 1.5f;
+}
+// GL_LINEAR, CLAMP_TO_EDGE lookup of a 1-D float table at texel centres (i+0.5)/N
+float lerpTable(const float* t, int n, float u) {
+  float f = u * float(n) - 0.5f;
+  float fl = std::floor(f);
+  float a = f - fl;
+  int i0 = (int)fl, i1 = i0 + 1;
+  i0 = i0 < 0 ? 0 : (i0 >= n ? n - 1 : i0);
+  i1 = i1 < 0 ? 0 : (i1 >= n ? n - 1 : i1);
+  return t[i0] * (1.0f - a) + t[i1] * a;
+}
+float IOR_METAL(float wl) {
+  if (metalModel == 1) { float t = (wl - 390.0f) / 360.0f; return 0.2f + 0.8f * t; }
+  return lerpTable(metalN.data(), 64, (wl - 390.0f) * float(1.0 / (750.0 - 390.0)));
+}
+float K_METAL(float wl) {
+  if (metalModel == 1) { float t = (wl - 390.0f) / 360.0f; return 3.5f - 1.5f * t; }
+  return lerpTable(metalK.data(), 64, (wl - 390.0f) * float(1.0 / (750.0 - 390.0)));
+}
+
+std::vector<float> lutXYZ(4096, 0.0f), lutICDF(4096, 0.0f);
+V3 lookupXYZ(float w) {
+  float f = w * 1024.0f - 0.5f, fl = std::floor(f), a = f - fl;
+  int i0 = (int)fl, i1 = i0 + 1;
+  i0 = i0 < 0 ? 0 : (i0 > 1023 ? 1023 : i0);
+  i1 = i1 < 0 ? 0 : (i1 > 1023 ? 1023 : i1);
+  const float* t0 = &lutXYZ[4 * i0];
+  const float* t1 = &lutXYZ[4 * i1];
+  return v3(t0[0] * (1.0f - a) + t1[0] * a, t0[1] * (1.0f - a) + t1[1] * a, t0[2] * (1.0f - a) + t1[2] * a);
+}
+
+// ---------------------------------------------------------------- RNG (pathtracer-fragment-shader.glsl:119-130)
+inline float randf(float r[4]) {
+  static const float q[4] = {1225.0f, 1585.0f, 2457.0f, 2098.0f};
+  static const float rr[4] = {1112.0f, 367.0f, 92.0f, 265.0f};
+  static const float a[4] = {3423.0f, 2646.0f, 1707.0f, 1999.0f};
+  static const float m[4] = {4194287.0f, 4194277.0f, 4194191.0f, 4194167.0f};
+  float s[4];
+  for (int k = 0; k < 4; ++k) {
+    float beta = std::floor(r[k] / q[k]);
+    float p = a[k] * (r[k] - beta * q[k]) - beta * rr[k];
+    beta = (1.0f - fsign(p)) * 0.5f * m[k];
+    r[k] = p + beta;
+    s[k] = r[k] / m[k];
+  }
+  float d = s[0] * 1.0f + s[1] * -1.0f + s[2] * 1.0f + s[3] * -1.0f;
+  return d - std::floor(d);
+}
+
+// ---------------------------------------------------------------- radiance type helpers
+inline float rt(float x, float) { return x; }
+inline V3 rt(float x, V3) { return v3(x); }
+inline float maxC(float f) { return f; }
+inline float maxC(V3 v) { return fmax_(fmax_(v.x, v.y), v.z); }
+inline float avgC(float f) { return f; }
+inline float avgC(V3 v) { return (v.x + v.y + v.z) / 3.0f; }
+inline float minR(float a, float b) { return fmin_(a, b); }
+inline V3 minR(V3 a, V3 b) { return vmin(a, b); }
+inline float expR(float a) { return std::exp(a); }
+inline V3 expR(V3 a) { return vexp(a); }
+inline float absR(float a) { return std::fabs(a); }
+inline V3 absR(V3 a) { return vabs(a); }
+inline float mixR(float x, float y, float a) { return x * (1.0f - a) + y * a; }
+inline V3 mixR(V3 x, V3 y, float a) { return x * (1.0f - a) + y * a; }
+inline float albedo(V3 RGB, V3 rgb, float) { return dot(RGB, rgb); }  // dispersive rgbToAlbedo
+inline V3 albedo(V3 RGB, V3, V3) { return RGB; }
+
+// ---------------------------------------------------------------- basis, colour (152-293)
+struct Basis { V3 nW, tW, bW; };
+V3 safe_normalize(V3 N) { float l = length(N); return N / fmax_(l, DENOM_TOLERANCE); }
+Basis makeBasis(V3 nW, bool safe = true) {
+  Basis b;
+  b.nW = nW;
+  if (std::fabs(nW.z) < std::fabs(nW.x)) b.tW = v3(nW.z, 0.0f, -nW.x);
+  else b.tW = v3(0.0f, nW.z, -nW.y);
+  b.tW = safe ? safe_normalize(b.tW) : normalize(b.tW);
+  b.bW = cross(nW, b.tW);
+  return b;
+}
+V3 worldToLocal(V3 v, const Basis& b) { return v3(dot(v, b.tW), dot(v, b.bW), dot(v, b.nW)); }
+V3 localToWorld(V3 v, const Basis& b) { return b.tW * v.x + b.bW * v.y + b.nW * v.z; }
+float cosTheta2(V3 n) { return n.z * n.z; }
+float tanTheta2(V3 n) { float ct2 = cosTheta2(n); return fmax_(0.0f, 1.0f - ct2) / fmax_(ct2, 1.0e-7f); }
+float tanTheta(V3 n) { return std::sqrt(fmax_(0.0f, tanTheta2(n))); }
+V3 xyzToRgb(V3 XYZ) {
+  V3 RGB;
+  RGB.x = 3.2404542f * XYZ.x - 1.5371385f * XYZ.y - 0.4985314f * XYZ.z;
+  RGB.y = -0.9692660f * XYZ.x + 1.8760108f * XYZ.y + 0.0415560f * XYZ.z;
+  RGB.z = 0.0556434f * XYZ.x - 0.2040259f * XYZ.y + 1.0572252f * XYZ.z;
+  float delta = -fmin_(0.0f, fmin_(fmin_(RGB.x, RGB.y), RGB.z));
+  RGB.x += delta; RGB.y += delta; RGB.z += delta;
+  float sum = RGB.x + RGB.y + RGB.z;
+  return RGB / sum;
+}
+V3 rgbToXyz(V3 RGB) {
+  return v3(0.4124564f * RGB.x + 0.3575761f * RGB.y + 0.1804375f * RGB.z,
+            0.2126729f * RGB.x + 0.7151522f * RGB.y + 0.0721750f * RGB.z,
+            0.0193339f * RGB.x + 0.1191920f * RGB.y + 0.9503041f * RGB.z);
+}
+
+// ---------------------------------------------------------------- sampling (300-372)
+V3 sampleHemisphereUniformly(float* rnd) {
+  float xi = randf(rnd);
+  float r = std::sqrt(1.0f - xi * xi);
+  float phi = 2.0f * PI_F * randf(rnd);
+  return v3(r * std::cos(phi), r * std::sin(phi), xi);
+}
+V3 sampleHemisphereCosineWeighted(float* rnd, float& pdf) {
+  float r = std::sqrt(randf(rnd));
+  float theta = 2.0f * PI_F * randf(rnd);
+  float x = r * std::cos(theta), y = r * std::sin(theta);
+  float z = std::sqrt(fmax_(0.0f, 1.0f - x * x - y * y));
+  pdf = std::fabs(z) / PI_F;
+  return v3(x, y, z);
+}
+float pdfHemisphereCosineWeighted(V3 w) { return w.z < 0.0f ? 0.0f : w.z / PI_F; }
+float powerHeuristic(float a, float b) { return a / (a + b); }
+V3 samplePhaseFunction(V3 rayDir, float g, float* rnd) {
+  float costheta;
+  if (std::fabs(g) < 1.0e-3f) costheta = 1.0f - 2.0f * randf(rnd);
+  else costheta = 1.0f / (2.0f * g) * (1.0f + g * g - ((1.0f - g * g) * (1.0f - g + 2.0f * g * randf(rnd))));
+  float sintheta = std::sqrt(fmax_(0.0f, 1.0f - costheta * costheta));
+  float phi = 2.0f * PI_F * randf(rnd);
+  Basis b = makeBasis(rayDir);
+  return costheta * rayDir + sintheta * (std::cos(phi) * b.tW + std::sin(phi) * b.bW);
+}
+
+// ---------------------------------------------------------------- microfacet + Fresnel (379-441)
+float microfacetEval(V3 m, float roughness) {
+  float t2 = tanTheta2(m), c2 = cosTheta2(m);
+  float rs = roughness * roughness;
+  float exponent = t2 / fmax_(rs, 1.0e-9f);
+  return std::exp(-exponent) / fmax_(PI_F * rs * c2 * c2, 1.0e-9f);
+}
+V3 microfacetSample(float* rnd, float roughness) {
+  float phiM = (2.0f * PI_F) * randf(rnd);
+  float cosPhiM = std::cos(phiM), sinPhiM = std::sin(phiM);
+  float tanThetaMSqr = -roughness * roughness * std::log(fmax_(1.0e-9f, randf(rnd)));
+  float cosThetaM = 1.0f / std::sqrt(1.0f + tanThetaMSqr);
+  float sinThetaM = std::sqrt(fmax_(0.0f, 1.0f - cosThetaM * cosThetaM));
+  return safe_normalize(v3(sinThetaM * cosPhiM, sinThetaM * sinPhiM, cosThetaM));
+}
+float microfacetPDF(V3 m, float roughness) { return microfacetEval(m, roughness) * std::fabs(m.z); }
+float smithG1(V3 v, V3 m, float roughness) {
+  float tanThetaAbs = std::fabs(tanTheta(v));
+  if (tanThetaAbs < 1.0e-6f) return 1.0f;
+  if (dot(v, m) * v.z <= 0.0f) return 0.0f;
+  float a = 1.0f / (fmax_(roughness, 1.0e-6f) * tanThetaAbs);
+  if (a >= 1.6f) return 1.0f;
+  float aSqr = a * a;
+  return (3.535f * a + 2.181f * aSqr) / (1.0f + 2.276f * a + 2.577f * aSqr);
+}
+float smithG2(V3 wo, V3 wi, V3 m, float r) { return smithG1(wo, m, r) * smithG1(wi, m, r); }
+float fresnelDielectricReflectance(float c, float eta_ti) {
+  float g = eta_ti * eta_ti - 1.0f + c * c;
+  if (g > 0.0f) {
+    g = std::sqrt(g);
+    float A = (g - c) / (g + c);
+    float B = (c * (g + c) - 1.0f) / (c * (g - c) + 1.0f);
+    return 0.5f * A * A * (1.0f + B * B);
+  }
+  return 1.0f;
+}
+float fresnelMetalReflectance(float cosi, float ior, float k) {
+  float cosip = std::fabs(cosi), cosi2 = cosip * cosip;
+  float iikk = ior * ior + k * k, iikkc = iikk * cosi2, twoEtaCosi = 2.0f * ior * cosip;
+  float Rparl2 = (iikkc - twoEtaCosi + 1.0f) / (iikkc + twoEtaCosi + 1.0f);
+  float Rperp2 = (iikk - twoEtaCosi + cosi2) / (iikk + twoEtaCosi + cosi2);
+  return 0.5f * (Rparl2 + Rperp2);
+}
+bool refractTransmitted(V3 n, float eta, V3 wt, V3& wi) {
+  float wtn = dot(wt, n);
+  float disc = 1.0f - eta * eta * (1.0f - wtn * wtn);
+  if (disc < 0.0f) return false;
+  wi = eta * wt - n * fsign(wtn) * (eta * std::fabs(wtn) - std::sqrt(disc));
+  return true;
+}
+
+// ---------------------------------------------------------------- the integrator, templated on RadianceType
+template <class R>
+struct Tracer {
+  V3 rgb;
+  float wl;
+
+  R A(V3 RGB) const { return albedo(RGB, rgb, R()); }
+  R one() const { return rt(1.0f, R()); }
+  R zero() const { return rt(0.0f, R()); }
+
+  bool traceRay(V3 start, V3 dir, V3& hit, int& material, float maxMarchDist) const {
+    material = MAT_INVAL;
+    float minMarch = P.minLengthScale;
+    float sdf = 1.0e20f;
+    for (int m : {MAT_SURFA, MAT_METAL, MAT_DIELE}) if (hasMat(m)) sdf = fmin_(sdf, std::fabs(evalSdf(sdfs[m], start)));
+    float InitialSign = fsign(sdf);
+    float t = InitialSign * sdf;
+    if (std::fabs(t) >= maxMarchDist) return false;
+    for (int n = 0; n < C.maxMarch; n++) {
+      V3 pW = start + t * dir;
+      sdf = 1.0e20f;
+      for (int m : {MAT_SURFA, MAT_METAL, MAT_DIELE}) {
+        if (!hasMat(m)) continue;
+        float s = std::fabs(evalSdf(sdfs[m], pW));
+        if (s < minMarch) { material = m; hit = start + t * dir; return true; }
+        sdf = fmin_(sdf, s);
+      }
+      t += InitialSign * sdf;
+      if (std::fabs(t) >= maxMarchDist) return false;
+    }
+    hit = start + t * dir;
+    return P.maxStepsIsMiss == 0.0f;
+  }
+  V3 normal(V3 pW, int m, bool safe = true) const {
+    float e = 2.0f * P.minLengthScale;
+    const SdfSpec& s = sdfs[m];
+    V3 N = v3(evalSdf(s, pW + v3(e, 0, 0)) - evalSdf(s, pW - v3(e, 0, 0)),
+              evalSdf(s, pW + v3(0, e, 0)) - evalSdf(s, pW - v3(0, e, 0)),
+              evalSdf(s, pW + v3(0, 0, e)) - evalSdf(s, pW - v3(0, 0, e)));
+    return safe ? safe_normalize(N) : normalize(N);
+  }
+
+  // Dielectric (477-633)
+  R evaluateDielectric(const Basis& basis, V3 wi, V3 wo) const {
+    float ior = IOR_DIELE(wl);
+    bool reflected = wo.z * wi.z > 0.0f;
+    R alb = A(A3(P.dieleSpecAlbedoRGB));
+    V3 bI = -wo, bO = wi;  // fromCamera
+    bool entering = bI.z < 0.0f;
+    V3 h;
+    if (reflected) h = safe_normalize(wo + wi);
+    else if (entering) h = safe_normalize(bI - ior * bO);
+    else h = safe_normalize(ior * bI - bO);
+    if (h.z < 0.0f) h = h * -1.0f;
+    float roughness = P.dieleRoughness;
+    float D = microfacetEval(h, roughness);
+    float G = smithG2(-bI, bO, h, roughness);
+    float eta = entering ? ior : 1.0f / ior;
+    R Fr = alb * fresnelDielectricReflectance(std::fabs(wi.z), eta);
+    if (reflected) return alb * Fr * D * G / fmax_(4.0f * std::fabs(wo.z) * std::fabs(wi.z), DENOM_TOLERANCE);
+    float im = dot(-bI, h), om = dot(bO, h);
+    float sd = im + eta * om;
+    float dwh = eta * eta * std::fabs(om) / fmax_(sd * sd, DENOM_TOLERANCE);
+    return (one() - Fr) * G * D * std::fabs(im) * dwh / fmax_(std::fabs(wo.z) * std::fabs(wi.z), DENOM_TOLERANCE);
+  }
+  float pdfDielectric(const Basis&, V3 wi, V3 wo) const {
+    float ior = IOR_DIELE(wl);
+    bool reflected = wo.z * wi.z > 0.0f;
+    R alb = A(A3(P.dieleSpecAlbedoRGB));
+    V3 bI = -wo, bO = wi;
+    bool entering = bI.z < 0.0f;
+    float eta = entering ? ior : 1.0f / ior;
+    R Fr = alb * fresnelDielectricReflectance(std::fabs(wi.z), eta);
+    V3 h;
+    float dwh, pdf;
+    if (reflected) {
+      h = safe_normalize(wo + wi);
+      dwh = 1.0f / fmax_(4.0f * std::fabs(dot(wi, h)), DENOM_TOLERANCE);
+      pdf = avgC(Fr);
+    } else {
+      if (entering) h = safe_normalize(bI - ior * bO);
+      else h = safe_normalize(ior * bI - bO);
+      if (h.z < 0.0f) h = h * -1.0f;
+      float im = dot(-bI, h), om = dot(bO, h);
+      float sd = im + eta * om;
+      dwh = eta * eta * std::fabs(om) / fmax_(sd * sd, DENOM_TOLERANCE);
+      pdf = 1.0f - avgC(Fr);
+    }
+    pdf *= microfacetPDF(h, P.dieleRoughness);
+    return std::fabs(pdf * dwh);
+  }
+  R sampleDielectric(const Basis&, V3 wi, V3& wo, float& pdfOut, float* rnd) const {
+    float ior = IOR_DIELE(wl);
+    R alb = A(A3(P.dieleSpecAlbedoRGB));
+    float etaRefl = (wi.z >= 0.0f) ? ior : 1.0f / ior;
+    R Fr = alb * fresnelDielectricReflectance(std::fabs(wi.z), etaRefl);
+    float roughness = P.dieleRoughness;
+    V3 m = microfacetSample(rnd, roughness);
+    float microPDF = microfacetPDF(m, roughness);
+    float reflectProb = avgC(Fr);
+    if (randf(rnd) < reflectProb) {
+      wo = -wi + 2.0f * dot(wi, m) * m;
+      if (wo.z * wi.z < 0.0f) wo = wo * -1.0f;
+      float D = microfacetEval(m, roughness), G = smithG2(wi, wo, m, roughness);
+      R f = Fr * D * G / fmax_(4.0f * std::fabs(wo.z) * std::fabs(wi.z), DENOM_TOLERANCE);
+      float dwh = 1.0f / fmax_(std::fabs(4.0f * dot(wi, m)), DENOM_TOLERANCE);
+      pdfOut = microPDF * reflectProb * dwh;
+      return f;
+    }
+    V3 bI = -wo, bO = wi;  // fromCamera: beamIncident from (uninitialised) woutputL, then overwritten
+    bool entering = bI.z < 0.0f;
+    float eta = entering ? ior : 1.0f / ior;
+    if (!refractTransmitted(m, eta, bO, bI)) return zero();
+    wo = -bI;
+    float im = dot(-bI, m);
+    R Frm = rt(fresnelDielectricReflectance(std::fabs(im), eta), R());
+    R Trm = one() - alb * Frm;
+    float D = microfacetEval(m, roughness), G = smithG2(-bI, bO, m, roughness);
+    float om = dot(bO, m);
+    float sd = im + eta * om;
+    float dwh = eta * eta * std::fabs(om) / fmax_(sd * sd, DENOM_TOLERANCE);
+    R f = (std::fabs(im) * dwh * Trm * G * D) / fmax_(std::fabs(wo.z) * std::fabs(wi.z), DENOM_TOLERANCE);
+    pdfOut = (1.0f - reflectProb) * microPDF * std::fabs(dwh);
+    return f;
+  }
+
+  // Metal (643-708)
+  R evaluateMetal(V3 wi, V3 wo) const {
+    if (wi.z < 0.0f) return zero();
+    float Fr = fresnelMetalReflectance(wi.z, IOR_METAL(wl), K_METAL(wl));
+    V3 h = normalize(wo + wi);
+    float r = P.metalRoughness;
+    float D = microfacetEval(h, r), G = smithG2(wi, wo, h, r);
+    R spec = A(A3(P.metalSpecAlbedoRGB));
+    return spec * Fr * D * G / fmax_(4.0f * std::fabs(wo.z) * std::fabs(wi.z), DENOM_TOLERANCE);
+  }
+  float pdfMetal(V3 wi, V3 wo) const {
+    if (wi.z < 0.0f) return PDF_EPSILON;
+    V3 h = safe_normalize(wo + wi);
+    float dwh = 1.0f / fmax_(std::fabs(4.0f * dot(wi, h)), DENOM_TOLERANCE);
+    return microfacetPDF(h, P.metalRoughness) * dwh;
+  }
+  R sampleMetal(V3 wi, V3& wo, float& pdfOut, float* rnd) const {
+    if (wi.z < 0.0f) return zero();
+    float Fr = fresnelMetalReflectance(wi.z, IOR_METAL(wl), K_METAL(wl));
+    float r = P.metalRoughness;
+    V3 m = microfacetSample(rnd, r);
+    wo = -wi + 2.0f * dot(wi, m) * m;
+    if (wo.z < DENOM_TOLERANCE) wo.z *= -1.0f;
+    float D = microfacetEval(m, r), G = smithG2(wi, wo, m, r);
+    R spec = A(A3(P.metalSpecAlbedoRGB));
+    R f = spec * Fr * D * G / fmax_(4.0f * std::fabs(wo.z) * std::fabs(wi.z), DENOM_TOLERANCE);
+    float dwh = 1.0f / fmax_(std::fabs(4.0f * dot(wi, m)), DENOM_TOLERANCE);
+    pdfOut = microfacetPDF(m, r) * dwh;
+    return f;
+  }
+
+  // Surface (716-807)
+  R diffuseRefl(V3 X) const { return A(surfaceDiffuseHook(A3(P.surfaceDiffuseAlbedoRGB), X)); }
+  R evaluateSurface(V3 X, V3 wi, V3 wo) const {
+    if (wi.z < 0.0f) return zero();
+    R diffuseAlbedo = (1.0f - P.subsurface) * diffuseRefl(X);
+    R specAlbedo = A(A3(P.surfaceSpecAlbedoRGB));
+    float ior = P.surfaceIor, r = P.surfaceRoughness;
+    float Fr = fresnelDielectricReflectance(wo.z, ior);
+    V3 h = normalize(wo + wi);
+    float D = microfacetEval(h, r), G = smithG2(wi, wo, h, r);
+    R f = Fr * specAlbedo * D * G / fmax_(4.0f * std::fabs(wo.z) * std::fabs(wi.z), DENOM_TOLERANCE);
+    float E = fresnelDielectricReflectance(wi.z, ior);
+    f = f + (1.0f - E) * diffuseAlbedo / PI_F;
+    return f;
+  }
+  float pdfSurface(V3 X, V3 wi, V3 wo) const {
+    if (wi.z < 0.0f) return PDF_EPSILON;
+    R diffuseAlbedo = (1.0f - P.subsurface) * diffuseRefl(X);
+    R specAlbedo = A(A3(P.surfaceSpecAlbedoRGB));
+    float E = fresnelDielectricReflectance(std::fabs(wi.z), P.surfaceIor);
+    float specW = E * avgC(specAlbedo), diffW = (1.0f - E) * avgC(diffuseAlbedo);
+    float specProb = specW / fmax_(specW + diffW, DENOM_TOLERANCE);
+    float diffusePdf = pdfHemisphereCosineWeighted(wo);
+    V3 h = safe_normalize(wo + wi);
+    float dwh = 1.0f / fmax_(std::fabs(4.0f * dot(wi, h)), DENOM_TOLERANCE);
+    float specularPdf = microfacetPDF(h, P.surfaceRoughness) * dwh;
+    return specProb * specularPdf + (1.0f - specProb) * diffusePdf;
+  }
+  R sampleSurface(V3 X, V3 wi, V3& wo, float& pdfOut, float* rnd) const {
+    if (wi.z < 0.0f) return zero();
+    R diffuseAlbedo = (1.0f - P.subsurface) * diffuseRefl(X);
+    R specAlbedo = A(A3(P.surfaceSpecAlbedoRGB));
+    float ior = P.surfaceIor, r = P.surfaceRoughness;
+    float E = fresnelDielectricReflectance(std::fabs(wi.z), ior);
+    float specW = E * avgC(specAlbedo), diffW = (1.0f - E) * avgC(diffuseAlbedo);
+    float specProb = specW / fmax_(specW + diffW, DENOM_TOLERANCE);
+    if (randf(rnd) >= specProb) {
+      wo = sampleHemisphereCosineWeighted(rnd, pdfOut);
+      pdfOut *= (1.0f - specProb);
+      return (1.0f - E) * diffuseAlbedo / PI_F;
+    }
+    V3 m = microfacetSample(rnd, r);
+    wo = -wi + 2.0f * dot(wi, m) * m;
+    pdfOut = specProb;
+    if (wo.z < DENOM_TOLERANCE) return zero();
+    float D = microfacetEval(m, r), G = smithG2(wi, wo, m, r);
+    float Fr = fresnelDielectricReflectance(std::fabs(wi.z), ior);
+    R f = Fr * specAlbedo * D * G / fmax_(4.0f * std::fabs(wo.z) * std::fabs(wi.z), DENOM_TOLERANCE);
+    float dwh = 1.0f / fmax_(std::fabs(4.0f * dot(wi, m)), DENOM_TOLERANCE);
+    pdfOut *= microfacetPDF(m, r) * dwh;
+    return f;
+  }
+
+  R evaluateBsdf(V3 X, const Basis& b, V3 wi, V3 wo, int m) const {
+    if (m == MAT_SURFA) return evaluateSurface(X, wi, wo);
+    if (m == MAT_METAL) return evaluateMetal(wi, wo);
+    if (m == MAT_DIELE) return evaluateDielectric(b, wi, wo);
+    return zero();
+  }
+  float pdfBsdf(V3 X, const Basis& b, V3 wi, V3 wo, int m) const {
+    if (m == MAT_SURFA) return pdfSurface(X, wi, wo);
+    if (m == MAT_METAL) return pdfMetal(wi, wo);
+    if (m == MAT_DIELE) return pdfDielectric(b, wi, wo);
+    return PDF_EPSILON;
+  }
+  R sampleBsdf(V3 X, const Basis& b, V3 wi, int m, V3& wo, float& pdf, float* rnd) const {
+    if (m == MAT_SURFA) return sampleSurface(X, wi, wo, pdf, rnd);
+    if (m == MAT_METAL) return sampleMetal(wi, wo, pdf, rnd);
+    if (m == MAT_DIELE) return sampleDielectric(b, wi, wo, pdf, rnd);
+    pdf = 1.0f;
+    return zero();
+  }
+
+  // lights (1051-1212)
+  R environmentRadiance(V3 dirW) const {
+    float rot_theta = PI_F * P.envMapThetaRotation / 180.0f;
+    V3 pole = v3(std::sin(rot_theta), std::cos(rot_theta), 0.0f);
+    float t = dot(dirW, pole);
+    float tt = P.envMapTransitionAngle / 180.0f;
+    float e0 = -tt, e1 = tt;
+    float s = fclamp((t - e0) / (e1 - e0), 0.0f, 1.0f);
+    s = s * s * (3.0f - 2.0f * s);
+    V3 tint = A3(P.skyTintDown) * (1.0f - s) + A3(P.skyTintUp) * s;
+    V3 RGB = v3(1.0f) * (P.skyPower * tint);
+    return A(RGB);
+  }
+  V3 sampleSunDir(float* rnd, float& pdfDir) const {
+    float theta_max = P.sunAngularSize * PI_F / 180.0f;
+    float theta = theta_max * std::sqrt(randf(rnd));
+    float ct = std::cos(theta), st = std::sqrt(fmax_(0.0f, 1.0f - ct * ct));
+    float phi = 2.0f * PI_F * randf(rnd);
+    float solid_angle = 2.0f * PI_F * (1.0f - std::cos(theta_max));
+    pdfDir = 1.0f / solid_angle;
+    return localToWorld(v3(st * std::cos(phi), st * std::sin(phi), ct), makeBasis(A3(P.sunDir)));
+  }
+  R sunRadiance(V3 dir) const {
+    float theta_max = P.sunAngularSize * PI_F / 180.0f;
+    if (dot(dir, A3(P.sunDir)) < std::cos(theta_max)) return zero();
+    return A(P.sunPower * A3(P.sunColor));
+  }
+  R sphereLightRadiance(V3 pW, V3 dir, V3& pHit) const {
+    V3 m = pW - A3(P.sphereLightPosition);
+    float b = dot(m, dir);
+    float c = dot(m, m) - P.sphereLightRadius * P.sphereLightRadius;
+    if (c > 0.0f && b > 0.0f) return zero();
+    float discr = b * b - c;
+    if (discr < 0.0f) return zero();
+    float t = fmax_(-b - std::sqrt(discr), 0.0f);
+    pHit = pW + t * dir;
+    return A(P.sphereLightPower * A3(P.sphereLightColor));
+  }
+  R sampleSphereLight(float* rnd, V3 pW, V3& pHit, float& pdfDir) const {
+    if (P.sphereLightPower < RADIANCE_EPSILON) return zero();
+    V3 cp = normalize(pW - A3(P.sphereLightPosition));
+    Basis hb = makeBasis(cp);
+    V3 vL = sampleHemisphereUniformly(rnd);
+    V3 vW = localToWorld(vL, hb);
+    float pdfArea = 1.0f / (2.0f * PI_F * P.sphereLightRadius * P.sphereLightRadius);
+    pHit = A3(P.sphereLightPosition) + P.sphereLightRadius * vW;
+    V3 dHit = pHit - pW;
+    float dHit2 = dot(dHit, dHit);
+    dHit = normalize(dHit);
+    V3 nHit = normalize(pHit - A3(P.sphereLightPosition));
+    float costheta = std::fabs(dot(nHit, dHit));
+    pdfDir = pdfArea * dHit2 / fmax_(DENOM_TOLERANCE, costheta);
+    return A(P.sphereLightPower * A3(P.sphereLightColor));
+  }
+  R shadowFactor(V3 o, V3 d, float len) const {
+    V3 h; int m;
+    R Tr = traceRay(o, d, h, m, len) ? zero() : one();
+    return absR(one() - P.shadowStrength * (one() - Tr));
+  }
+
+  // directSurfaceLighting (1221-1295)
+  R directSurfaceLighting(V3 pW, const Basis& b, V3 winW, int m, float* rnd, float& skyPdf, float& sunPdf, float& sphPdf) const {
+    V3 wi = worldToLocal(winW, b);
+    R Ld = zero();
+    if (P.skyPower > RADIANCE_EPSILON) {
+      R Li = zero();
+      V3 wL, wW;
+      if (!(P.skyPower < RADIANCE_EPSILON)) {
+        wL = sampleHemisphereCosineWeighted(rnd, skyPdf);
+        wW = localToWorld(wL, b);
+        Li = environmentRadiance(wW);
+      }
+      if (avgC(Li) > RADIANCE_EPSILON) {
+        Li = Li * shadowFactor(pW, wW, P.maxLengthScale);
+        if (avgC(Li) > RADIANCE_EPSILON) {
+          float bp = fmax_(PDF_EPSILON, pdfBsdf(pW, b, wi, wL, m));
+          R f = evaluateBsdf(pW, b, wi, wL, m);
+          float w = powerHeuristic(skyPdf, bp);
+          Ld = Ld + f * Li / fmax_(PDF_EPSILON, skyPdf) * std::fabs(dot(wW, b.nW)) * w;
+        }
+      }
+    }
+    if (P.sunPower > RADIANCE_EPSILON) {
+      R Li = zero();
+      V3 wL, wW;
+      wW = sampleSunDir(rnd, sunPdf);
+      wL = worldToLocal(wW, b);
+      if (!(wL.z < 0.0f)) Li = sunRadiance(wW);
+      if (avgC(Li) > RADIANCE_EPSILON) {
+        Li = Li * shadowFactor(pW, wW, P.maxLengthScale);
+        if (avgC(Li) > RADIANCE_EPSILON) {
+          float bp = fmax_(PDF_EPSILON, pdfBsdf(pW, b, wi, wL, m));
+          R f = evaluateBsdf(pW, b, wi, wL, m);
+          float w = powerHeuristic(sunPdf, bp);
+          Ld = Ld + f * Li / fmax_(PDF_EPSILON, sunPdf) * std::fabs(dot(wW, b.nW)) * w;
+        }
+      }
+    }
+    if (C.sphereLight && (P.sphereLightPower > RADIANCE_EPSILON) &&
+        (dot(pW - A3(P.sphereLightPosition), b.nW) <= P.sphereLightRadius)) {
+      V3 pHit;
+      R Li = sampleSphereLight(rnd, pW, pHit, sphPdf);
+      if (avgC(Li) > RADIANCE_EPSILON) {
+        float hitDist = fmax_(0.0f, length(pHit - pW) - 3.0f * P.minLengthScale);
+        V3 wW = normalize(pHit - pW);
+        V3 wL = worldToLocal(wW, b);
+        Li = Li * shadowFactor(pW, wW, hitDist);
+        if (avgC(Li) > RADIANCE_EPSILON) {
+          float bp = fmax_(PDF_EPSILON, pdfBsdf(pW, b, wi, wL, m));
+          R f = evaluateBsdf(pW, b, wi, wL, m);
+          float w = powerHeuristic(sphPdf, bp);
+          Ld = Ld + f * Li / fmax_(PDF_EPSILON, sphPdf) * std::fabs(dot(wW, b.nW)) * w;
+        }
+      }
+    }
+    return minR(rt(P.radianceClamp, R()), Ld);
+  }
+
+  // cameraPath (1625-1926) without the atmosphere and sub-surface walk
+  R cameraPath(V3 start, V3 dir, float* rnd) const {
+    R L = zero();
+    float misSky = 1.0f, misSun = 1.0f, misSph = 1.0f;
+    V3 pW = start, rayDir = dir;
+    bool inDielectric = hasMat(MAT_DIELE) && evalSdf(sdfs[MAT_DIELE], start) < 0.0f;
+    R throughput = one();
+    for (int vertex = 0; vertex <= C.maxBounces; ++vertex) {
+      if (maxC(throughput) < THROUGHPUT_EPSILON) break;
+      V3 pNext;
+      int hm;
+      bool hit = traceRay(pW, rayDir, pNext, hm, P.maxLengthScale);
+      float rayLength = P.maxLengthScale;
+      if (hit) rayLength = length(pNext - pW);
+      else {
+        bool lit = false;
+        if (C.sphereLight) {
+          V3 pl;
+          R LiSph = (P.sphereLightPower < RADIANCE_EPSILON) ? zero() : sphereLightRadiance(pW, rayDir, pl);
+          if (avgC(LiSph) > RADIANCE_EPSILON) {
+            lit = true;
+            L = L + throughput * one() * misSph * LiSph / (1.0f - 0.0f);
+          }
+        }
+        if (!lit) {
+          R Tr = one();
+          if (avgC(Tr) > RADIANCE_EPSILON) {
+            if (!(vertex == 0 && P.envMapVisible == 0.0f) && misSky > 0.0f)
+              L = L + throughput * Tr * misSky * environmentRadiance(rayDir) / (1.0f - 0.0f);
+            if (!(vertex == 0 && P.sunVisibleDirectly == 0.0f) && misSun > 0.0f)
+              L = L + throughput * Tr * misSun * sunRadiance(rayDir) / (1.0f - 0.0f);
+          }
+        }
+        break;
+      }
+      if (hm == MAT_INVAL) break;  // max-step "hit" without a material: undefined in the reference; ends the path
+      if (C.sphereLight) {
+        V3 pl;
+        R LiSph = (P.sphereLightPower < RADIANCE_EPSILON) ? zero() : sphereLightRadiance(pW, rayDir, pl);
+        if (avgC(LiSph) > RADIANCE_EPSILON) {
+          float lsl = length(pl - pW);
+          if (lsl < rayLength) L = L + throughput * one() * misSph * LiSph;
+        }
+      }
+      if (hasMat(MAT_DIELE) && inDielectric) {
+        R absorption = A(A3(P.dieleAbsorptionRGB));
+        throughput = throughput * expR(-rayLength * absorption);
+      }
+      pW = pNext;
+      V3 nW = normal(pW, hm);
+      V3 ngW = nW;
+      Basis b = makeBasis(nW);
+      V3 winW = -rayDir;
+      V3 wi = worldToLocal(winW, b);
+      // WebGL zero-initialises uninitialised locals (woutputL, bsdfPdf)
+      V3 wo = v3(0.0f);
+      float bsdfPdf = 0.0f;
+      R f = sampleBsdf(pW, b, wi, hm, wo, bsdfPdf, rnd);
+      V3 woW = localToWorld(wo, b);
+      if (hm == MAT_DIELE && dot(winW, ngW) * dot(woW, ngW) < 0.0f) inDielectric = !inDielectric;
+      rayDir = woW;
+      pW = pW + ngW * fsign(dot(rayDir, ngW)) * 3.0f * P.minLengthScale;
+      float skyPdf = 0.0f, sunPdf = 0.0f, sphPdf = 0.0f;
+      if (!(hasMat(MAT_DIELE) && inDielectric))
+        L = L + throughput * directSurfaceLighting(pW, b, winW, hm, rnd, skyPdf, sunPdf, sphPdf);
+      R fOverPdf = minR(rt(P.radianceClamp, R()), f / fmax_(PDF_EPSILON, bsdfPdf));
+      R st = fOverPdf * std::fabs(dot(woW, nW)) / fmax_(PDF_EPSILON, 1.0f - 0.0f);
+      throughput = throughput * st;
+      misSky = powerHeuristic(bsdfPdf, skyPdf);
+      misSun = powerHeuristic(bsdfPdf, sunPdf);
+      misSph = powerHeuristic(bsdfPdf, sphPdf);
+      if (maxC(throughput) < THROUGHPUT_EPSILON) break;
+    }
+    return L;
+  }
+};
+
+inline V3 toXYZ(float L, V3 xyz) { return xyz * L; }  // dispersive: xyz * L
+inline V3 toXYZ(V3 L, V3) { return rgbToXyz(L); }
+
+float sample_jitter(float xi) { return xi < 0.5f ? std::sqrt(2.0f * xi) - 1.0f : 1.0f - std::sqrt(2.0f - 2.0f * xi); }
+
+void constructPrimaryRay(float px, float py, float* rnd, V3& start, V3& dir, bool safe) {
+  float ndcx = -1.0f + 2.0f * (px / P.resolution[0]);
+  float ndcy = -1.0f + 2.0f * (py / P.resolution[1]);
+  float fh = std::tan(0.5f * (P.camFovy * 0.017453292519943295f));
+  float fw = P.camAspect * fh;
+  V3 s = -fw * ndcx * A3(P.camX) + fh * ndcy * A3(P.camY);
+  dir = safe ? safe_normalize(A3(P.camDir) + s) : normalize(A3(P.camDir) + s);
+  if (P.camAperture <= 0.0f) { start = A3(P.camPos); return; }
+  V3 focal = A3(P.camPos) + P.camFocalDistance * dir / dot(dir, A3(P.camDir));
+  float lensRadial = P.camAperture * std::sqrt(randf(rnd));
+  float theta = 2.0f * PI_F * randf(rnd);
+  V3 lensPos = A3(P.camPos) + lensRadial * (-A3(P.camX) * std::cos(theta) + A3(P.camY) * std::sin(theta));
+  start = lensPos;
+  dir = safe ? safe_normalize(focal - lensPos) : normalize(focal - lensPos);
+}
+
+// pathtrace + main (1934-2009) for one pixel
+template <class R>
+void pathtracePixel(int ix, int iy, float* rad, float* rnd) {
+  if (C.interactive) {
+    if (randf(rnd) < P.skipProbability) return;  // radiance unchanged, rng advanced
+  }
+  Tracer<R> T;
+  V3 xyz = v3(0.0f);
+  if (C.dispersive) {
+    float ws = P.wavelengthSamples;
+    float xi = 0.1f + 0.8f * (0.5f + std::floor(ws * randf(rnd))) / ws;
+    float w = lerpTable(lutICDF.data(), 4096, xi);
+    T.wl = 390.0f + (750.0f - 390.0f) * w;
+    xyz = lookupXYZ(w);
+    T.rgb = xyzToRgb(xyz);
+  } else {
+    T.wl = 390.0f + (750.0f - 390.0f) * 0.5f;
+    T.rgb = v3(0.0f);
+  }
+  R L = T.zero();
+  float px = ix + 0.5f, py = iy + 0.5f;
+  for (int n = 0; n < C.spf; ++n) {
+    float jx = 0.5f * P.filterRadius * sample_jitter(randf(rnd));
+    float jy = 0.5f * P.filterRadius * sample_jitter(randf(rnd));
+    V3 s, d;
+    constructPrimaryRay(px + jx, py + jy, rnd, s, d, true);
+    L = L + T.cameraPath(s, d, rnd);
+  }
+  L = L / float(C.spf);
+  V3 c = toXYZ(L, xyz);
+  float oldN = rad[3], newN = oldN + 1.0f;
+  V3 newL = (oldN * v3(rad[0], rad[1], rad[2]) + c) / newN;
+  rad[0] = newL.x; rad[1] = newL.y; rad[2] = newL.z; rad[3] = newN;
+}
+
+// normals program main (normals-fragment-shader.glsl:293-345)
+void normalsPixel(int ix, int iy, float* rad, float* rnd) {
+  Tracer<V3> T;
+  V3 colorXYZ = v3(0.0f);
+  for (int n = 0; n < C.spf; ++n) {
+    float r1 = randf(rnd);
+    float r2 = randf(rnd);
+    float px = ix + 0.5f + (-0.5f + r1), py = iy + 0.5f + (-0.5f + r2);
+    V3 s, d;
+    constructPrimaryRay(px, py, rnd, s, d, false);
+    // traceDistance (70-113)
+    float sdf = 1.0e20f;
+    for (int m : {MAT_SURFA, MAT_METAL, MAT_DIELE}) if (hasMat(m)) sdf = fmin_(sdf, std::fabs(evalSdf(sdfs[m], s)));
+    float InitialSign = fsign(sdf), t = 0.0f;
+    int iters = 0, material = MAT_INVAL;
+    for (int k = 0; k < C.maxMarch; k++) {
+      t += InitialSign * sdf;
+      if (t >= P.maxLengthScale) break;
+      V3 pW = s + t * d;
+      sdf = 1.0e20f;
+      bool h = false;
+      for (int m : {MAT_SURFA, MAT_METAL, MAT_DIELE}) {
+        if (!hasMat(m)) continue;
+        float v = std::fabs(evalSdf(sdfs[m], pW));
+        if (v < P.minLengthScale) { material = m; h = true; break; }
+        sdf = fmin_(sdf, v);
+      }
+      if (h) break;
+      iters++;
+    }
+    V3 hit = s + t * d;
+    bool isHit = !(t >= P.maxLengthScale) && !(P.maxStepsIsMiss != 0.0f && iters >= C.maxMarch);
+    if (isHit) {
+      V3 nW = material == MAT_INVAL ? v3(0.0f) : T.normal(hit, material, false);
+      colorXYZ = colorXYZ + rgbToXyz(0.5f * (nW + v3(1.0f)));
+    } else if (P.envMapVisible != 0.0f) {
+      colorXYZ = colorXYZ + P.skyPower * v3(1.0f);
+    }
+  }
+  colorXYZ = colorXYZ / float(C.spf);
+  float oldN = rad[3], newN = oldN + 1.0f;
+  V3 newL = (oldN * v3(rad[0], rad[1], rad[2]) + colorXYZ) / newN;
+  rad[0] = newL.x; rad[1] = newL.y; rad[2] = newL.z; rad[3] = newN;
+}
+
+uint32_t pcg_next(uint64_t& state, uint64_t inc) {
+  uint64_t old = state;
+  state = old * 6364136223846793005ull + inc;
+  uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+  uint32_t rot = (uint32_t)(old >> 59u);
+  return (xs >> rot) | (xs << ((-rot) & 31u));
+}
+
+}  // namespace
+
+extern "C" {
+
+void oracle_reset() {
+  memset(&P, 0, sizeof P);
+  C = Config();
+  for (auto& s : sdfs) s = SdfSpec();
+  albedoHook = 0;
+  dieleModel = 1;
+  dieleC.clear();
+  metalModel = 0;
+}
+
+int oracle_set(const char* name, const float* v, int n) {
+  for (auto& e : table()) {
+    if (!strcmp(e.name, name)) {
+      for (int i = 0; i < e.n && i < n; ++i) e.p[i] = v[i];
+      return 0;
+    }
+  }
+  return -1;
+}
+
+void oracle_config(int dispersive, int interactive, int maxBounces, int maxMarch, int spf, int sphereLight) {
+  C.dispersive = dispersive; C.interactive = interactive; C.maxBounces = maxBounces;
+  C.maxMarch = maxMarch; C.spf = spf; C.sphereLight = sphereLight;
+}
+
+// material id: 0 dielectric, 1 metal, 2 surface
+void oracle_sdf(int material, int kind, float a, float b, float c, float d) {
+  sdfs[material].kind = kind; sdfs[material].a = a; sdfs[material].b = b; sdfs[material].c = c; sdfs[material].d = d;
+}
+void oracle_albedo_hook(int kind) { albedoHook = kind; }
+void oracle_dielectric(int model, const float* coeffs, int n) { dieleModel = model; dieleC.assign(coeffs, coeffs + n); }
+void oracle_metal(int model, const float* n64, const float* k64) {
+  metalModel = model;
+  if (n64) metalN.assign(n64, n64 + 64);
+  if (k64) metalK.assign(k64, k64 + 64);
+}
+void oracle_luts(const float* xyz4096, const float* icdf4096) {
+  lutXYZ.assign(xyz4096, xyz4096 + 4096);
+  lutICDF.assign(icdf4096, icdf4096 + 4096);
+}
+
+float oracle_rand(float* state4) { return randf(state4); }
+float oracle_ior_diele(float wl) { return IOR_DIELE(wl); }
+float oracle_ior_metal(float wl) { return IOR_METAL(wl); }
+float oracle_k_metal(float wl) { return K_METAL(wl); }
+float oracle_fresnel_dielectric(float c, float eta) { return fresnelDielectricReflectance(c, eta); }
+float oracle_fresnel_metal(float c, float n, float k) { return fresnelMetalReflectance(c, n, k); }
+float oracle_microfacet_eval(const float* m, float r) { return microfacetEval(v3(m[0], m[1], m[2]), r); }
+float oracle_smith_g1(const float* v, const float* m, float r) { return smithG1(v3(v[0], v[1], v[2]), v3(m[0], m[1], m[2]), r); }
+float oracle_sdf_eval(int material, const float* p) { return evalSdf(sdfs[material], v3(p[0], p[1], p[2])); }
+// algorithmic FLOPs (and other VALU ops) of one SDF evaluation at p
+void oracle_sdf_count(int material, const float* p, long long* flops, long long* ops) {
+  Counted::flops = 0;
+  Counted::ops = 0;
+  Counted x[3] = {cf(p[0]), cf(p[1]), cf(p[2])};
+  evalSdfT<Counted>(sdfs[material], x);
+  *flops = Counted::flops;
+  *ops = Counted::ops;
+}
+
+// bsdf evaluate / pdf / sample in the local frame at wavelength wl (dispersive rgb from the LUT w)
+void oracle_bsdf(int material, float wl, const float* rgb, const float* wi, const float* wo, float* f3, float* pdf) {
+  Tracer<V3> T;
+  T.wl = wl;
+  T.rgb = v3(rgb[0], rgb[1], rgb[2]);
+  Basis b = makeBasis(v3(0, 0, 1));
+  V3 f = T.evaluateBsdf(v3(0, 0, 0), b, v3(wi[0], wi[1], wi[2]), v3(wo[0], wo[1], wo[2]), material);
+  f3[0] = f.x; f3[1] = f.y; f3[2] = f.z;
+  *pdf = T.pdfBsdf(v3(0, 0, 0), b, v3(wi[0], wi[1], wi[2]), v3(wo[0], wo[1], wo[2]), material);
+}
+void oracle_bsdf_sample(int material, float wl, const float* wi, float* rnd4, float* wo, float* f3, float* pdf) {
+  Tracer<V3> T;
+  T.wl = wl;
+  T.rgb = v3(0.0f);
+  Basis b = makeBasis(v3(0, 0, 1));
+  V3 w = v3(0.0f);
+  float p = 1.0f;
+  V3 f = T.sampleBsdf(v3(0, 0, 0), b, v3(wi[0], wi[1], wi[2]), material, w, p, rnd4);
+  wo[0] = w.x; wo[1] = w.y; wo[2] = w.z;
+  f3[0] = f.x; f3[1] = f.y; f3[2] = f.z;
+  *pdf = p;
+}
+
+void oracle_seed(float* rng, int npix, unsigned seed) {
+  const uint32_t m[4] = {4194287u, 4194277u, 4194191u, 4194167u};
+  for (int i = 0; i < npix; ++i) {
+    uint64_t state = 0, inc = (54ull << 1u) | 1u;
+    pcg_next(state, inc);
+    state += (uint64_t)((uint32_t)i ^ seed);
+    pcg_next(state, inc);
+    for (int k = 0; k < 4; ++k) rng[4 * i + k] = float(1u + pcg_next(state, inc) % (m[k] - 1u));
+  }
+}
+
+// pt frames for the pixels i with i % stride == offset (stride 1 = all)
+int oracle_render(float* radiance, float* rng, int W, int H, int frames, int stride, int offset, int threads) {
+  long long npix = (long long)W * H;
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 64)
+#endif
+  for (long long i = offset; i < npix; i += stride) {
+    int ix = (int)(i % W), iy = (int)(i / W);
+    for (int f = 0; f < frames; ++f) {
+      if (C.dispersive) pathtracePixel<float>(ix, iy, radiance + 4 * i, rng + 4 * i);
+      else pathtracePixel<V3>(ix, iy, radiance + 4 * i, rng + 4 * i);
+    }
+  }
+  return 0;
+}
+
+int oracle_normals(float* radiance, float* rng, int W, int H, int frames, int threads) {
+  long long npix = (long long)W * H;
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 64)
+#endif
+  for (long long i = 0; i < npix; ++i)
+    for (int f = 0; f < frames; ++f) normalsPixel((int)(i % W), (int)(i / W), radiance + 4 * i, rng + 4 * i);
+  return 0;
+}
+
+// pick program (pick-fragment-shader.glsl:116-152)
+int oracle_pick(const float* xy, int n, float* out) {
+  Tracer<V3> T;
+  for (int k = 0; k < n; ++k) {
+    float dummy[4] = {1, 1, 1, 1};
+    V3 s, d;
+    float ap = P.camAperture;
+    P.camAperture = 0.0f;
+    constructPrimaryRay(xy[2 * k], xy[2 * k + 1], dummy, s, d, false);
+    P.camAperture = ap;
+    float sdf = 1.0e20f;
+    for (int m : {MAT_SURFA, MAT_METAL, MAT_DIELE}) if (hasMat(m)) sdf = fmin_(sdf, std::fabs(evalSdf(sdfs[m], s)));
+    float InitialSign = fsign(sdf), t = 0.0f;
+    int iters = 0, material = MAT_INVAL;
+    for (int q = 0; q < C.maxMarch; q++) {
+      t += InitialSign * sdf;
+      if (t >= P.maxLengthScale) break;
+      V3 pW = s + t * d;
+      sdf = 1.0e20f;
+      bool h = false;
+      for (int m : {MAT_SURFA, MAT_METAL, MAT_DIELE}) {
+        if (!hasMat(m)) continue;
+        float v = std::fabs(evalSdf(sdfs[m], pW));
+        if (v < P.minLengthScale) { material = m; h = true; break; }
+        sdf = fmin_(sdf, v);
+      }
+      if (h) break;
+      iters++;
+    }
+    bool isHit = !(t >= P.maxLengthScale) && !(P.maxStepsIsMiss != 0.0f && iters >= C.maxMarch);
+    float dist = P.maxLengthScale;
+    if (isHit) dist = length((s + t * d) - s);
+    out[2 * k] = dist;
+    out[2 * k + 1] = float(material);
+  }
+  return 0;
+}
+
+// tonemapper (tonemapper-fragment-shader.glsl:36-97) without the hue shift
+int oracle_tonemap(const float* radiance, long long npix, float* rgb) {
+  auto tm = [](float L) { return L / (1.0f + L); };
+  for (long long i = 0; i < npix; ++i) {
+    float X = radiance[4 * i], Y = radiance[4 * i + 1], Z = radiance[4 * i + 2];
+    float c[3] = {3.2406f * X - 1.5372f * Y - 0.4986f * Z, -0.9689f * X + 1.8758f * Y + 0.0415f * Z,
+                  0.0557f * X - 0.2040f * Y + 1.0570f * Z};
+    float delta = -fmin_(0.0f, fmin_(fmin_(c[0], c[1]), c[2]));
+    for (float& v : c) v += delta;
+    for (float& v : c) v = std::pow(v, P.invGamma);
+    float e = std::pow(2.0f, P.exposure);
+    for (float& v : c) v = tm(v * e);
+    float mean = (c[0] + c[1] + c[2]) / 3.0f;
+    for (float& v : c) { float dv = v - mean; v = mean + fsign(dv) * std::pow(std::fabs(dv), 1.0f / P.saturation); }
+    for (float& v : c) { float dv = v - 0.5f; v = 0.5f + fsign(dv) * std::pow(std::fabs(dv), 1.0f / P.contrast); }
+    rgb[3 * i] = c[0]; rgb[3 * i + 1] = c[1]; rgb[3 * i + 2] = c[2];
+  }
+  return 0;
+}
+
+}  // extern "C"

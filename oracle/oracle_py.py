@@ -1,0 +1,133 @@
+"""ctypes front of the CPU oracle (oracle/_build/liboracle.so).
+
+Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline use it, always as the checker of the HIP path.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+_lib = None
+
+DIELECTRIC_MODELS = {"AbbeDielectric": 0, "SellmeierDielectric": 1, "Sellmeier2Dielectric": 2,
+                     "PolyanskiyDielectric": 3, "CauchyDielectric": 4}
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.oracle_set.argtypes = [ctypes.c_char_p, fp, ctypes.c_int]
+        L.oracle_config.argtypes = [ctypes.c_int] * 6
+        L.oracle_sdf.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 4
+        L.oracle_albedo_hook.argtypes = [ctypes.c_int]
+        L.oracle_dielectric.argtypes = [ctypes.c_int, fp, ctypes.c_int]
+        L.oracle_metal.argtypes = [ctypes.c_int, fp, fp]
+        L.oracle_luts.argtypes = [fp, fp]
+        L.oracle_rand.argtypes = [fp]
+        L.oracle_rand.restype = ctypes.c_float
+        for n in ["oracle_ior_diele", "oracle_ior_metal", "oracle_k_metal"]:
+            getattr(L, n).argtypes = [ctypes.c_float]
+            getattr(L, n).restype = ctypes.c_float
+        L.oracle_fresnel_dielectric.argtypes = [ctypes.c_float, ctypes.c_float]
+        L.oracle_fresnel_dielectric.restype = ctypes.c_float
+        L.oracle_fresnel_metal.argtypes = [ctypes.c_float] * 3
+        L.oracle_fresnel_metal.restype = ctypes.c_float
+        L.oracle_microfacet_eval.argtypes = [fp, ctypes.c_float]
+        L.oracle_microfacet_eval.restype = ctypes.c_float
+        L.oracle_smith_g1.argtypes = [fp, fp, ctypes.c_float]
+        L.oracle_smith_g1.restype = ctypes.c_float
+        L.oracle_sdf_eval.argtypes = [ctypes.c_int, fp]
+        L.oracle_sdf_eval.restype = ctypes.c_float
+        L.oracle_bsdf.argtypes = [ctypes.c_int, ctypes.c_float, fp, fp, fp, fp, fp]
+        L.oracle_bsdf_sample.argtypes = [ctypes.c_int, ctypes.c_float, fp, fp, fp, fp, fp]
+        L.oracle_seed.argtypes = [fp, ctypes.c_int, ctypes.c_uint]
+        L.oracle_render.argtypes = [fp, fp] + [ctypes.c_int] * 6
+        L.oracle_normals.argtypes = [fp, fp] + [ctypes.c_int] * 4
+        L.oracle_pick.argtypes = [fp, ctypes.c_int, fp]
+        L.oracle_tonemap.argtypes = [fp, ctypes.c_longlong, fp]
+        _lib = L
+    return _lib
+
+
+def fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def f3(v):
+    return np.ascontiguousarray(v, np.float32)
+
+
+class Oracle:
+    """The oracle configured for one scene (a snelly_amd.scenes.Scene)."""
+
+    def __init__(self, scene, width, height):
+        from snelly_amd import host  # plumbing: uniform values, LUTs
+        self.scene, self.width, self.height = scene, width, height
+        L = lib()
+        L.oracle_reset()
+        r, mats = scene.renderer, scene.materials
+        scene.camera.aspect = width / height
+        u = host.frame_uniforms(r, scene.camera, scene.scales, mats, width, height)
+        for k, v in u.items():
+            arr = np.atleast_1d(np.asarray(v, np.float32)).ravel()
+            L.oracle_set(k.encode(), fp(arr), arr.size)
+        sphere_light = 1 if r.sphereLightPower > -7.0 else 0
+        L.oracle_config(int(r.dispersive), int(r.interactive), int(round(r.maxBounces)),
+                        int(round(r.maxMarchSteps)), int(round(r.maxSamplesPerFrame)), sphere_light)
+        for (m, kind, a, b, c, d) in scene.oracle_sdfs:
+            L.oracle_sdf(m, kind, a, b, c, d)
+        L.oracle_albedo_hook(scene.oracle_albedo_hook)
+        diel = mats.dielectric
+        model = DIELECTRIC_MODELS[type(diel).__name__]
+        coeffs = f3([diel.iorVal, diel.abbe] if model == 0 else diel.coeffs)
+        L.oracle_dielectric(model, fp(coeffs), coeffs.size)
+        met = mats.metal
+        if isinstance(met, host.TabulatedMetal):
+            L.oracle_metal(0, fp(met.n), fp(met.k))
+        else:
+            L.oracle_metal(1, None, None)
+        self.xyz = f3(host.xyz_lut())
+        self.icdf = f3(host.inverse_cdf(lambda wl: host.blackbody(wl, r.skyTemperature)))
+        L.oracle_luts(fp(self.xyz), fp(self.icdf))
+
+    @staticmethod
+    def seed(npix, seed):
+        rng = np.empty((npix, 4), np.float32)
+        lib().oracle_seed(fp(rng), npix, seed)
+        return rng
+
+    def new_state(self, seed=1234):
+        npix = self.width * self.height
+        rad = np.zeros((npix, 4), np.float32)
+        rad[:, 3] = 1.0
+        return rad, self.seed(npix, seed)
+
+    def render(self, rad, rng, frames, stride=1, offset=0, threads=0):
+        lib().oracle_render(fp(rad), fp(rng), self.width, self.height, frames, stride, offset, threads)
+
+    def normals(self, rad, rng, frames, threads=0):
+        lib().oracle_normals(fp(rad), fp(rng), self.width, self.height, frames, threads)
+
+    def pick(self, xy):
+        xy = f3(xy).reshape(-1, 2)
+        out = np.empty_like(xy)
+        lib().oracle_pick(fp(xy), xy.shape[0], fp(out))
+        return out
+
+    def tonemap(self, rad):
+        rad = f3(rad).reshape(-1, 4)
+        out = np.empty((rad.shape[0], 3), np.float32)
+        lib().oracle_tonemap(fp(rad), rad.shape[0], fp(out))
+        return out

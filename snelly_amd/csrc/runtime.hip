@@ -1,0 +1,723 @@
+// runtime.hip -- libsnelly_hip.so: the C ABI (include/snelly.h) over HIP on gfx950.
+//
+// A program is one hipRTC module built from a scene snippet (translated by
+// translate.cpp) plus the wavefront kernels (snl_pt.h, snl_kernels.h).  A state
+// holds one image's accumulation + RNG buffers and the SoA path buffers.  A
+// frame is a fixed launch sequence on the context's stream with no host
+// synchronisation: queue sizes live in device counters that each kernel reads.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <dlfcn.h>
+#include <sys/stat.h>
+
+#include <unistd.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstddef>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/snelly.h"
+#include "embedded_sources.h"
+#include "snl_wavefront.h"
+#include "translate.h"
+
+using namespace glsl;
+
+// ---------------------------------------------------------------- objects
+struct snl_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int numCU = 256;
+};
+
+struct snl_tex_s {
+  snl_ctx ctx;
+  float* d = nullptr;
+  int w = 0, h = 0, depth = 1, ch = 4, flags = 0;
+};
+
+struct snl_state_s {
+  snl_ctx ctx;
+  int w = 0, h = 0;
+  long long npix = 0;
+  float* radiance = nullptr;
+  float* rng = nullptr;
+  float* pf = nullptr;
+  int* pi = nullptr;
+  int* q = nullptr;    // 8 * cap ints: trace A/B, nee A/B, mat[4]
+  int* cnt = nullptr;  // C_COUNT ints
+};
+
+struct snl_prog_s {
+  snl_ctx ctx;
+  hipModule_t mod = nullptr;
+  hipFunction_t k_reset, k_rotate, k_camera, k_trace, k_bin, k_miss, k_surf, k_metal, k_diele, k_accum, k_normals,
+      k_pick, k_tonemap;
+  bool hasSurface = false, hasMetal = false, hasDiele = false;
+  int maxBounces = 3, maxMarch = 256, spf = 1;
+  SnlParams hostP;
+  std::vector<char> hostU;
+  SnlParams* dP = nullptr;
+  void* dU = nullptr;
+  bool dirty = true;
+  std::vector<snl::UniformInfo> user;
+  int traceBlocks = 0, traceChunk = 64, shadeBlocks = 0;
+  bool profiling = false;
+  snl_stats stats{};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, et0 = nullptr, et1 = nullptr;
+  std::vector<hipEvent_t> traceEvents;  // start/stop pairs of the trace launches (profiling)
+};
+
+namespace {
+
+thread_local std::string g_err;  // for calls without a context
+
+int fail(snl_ctx ctx, int code, const char* fmt, ...) {
+  char buf[2048];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->err = buf;
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                   \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) return fail(ctx, SNL_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+struct BuiltinInfo {
+  const char* name;
+  int offset, type, size;
+};
+const std::vector<BuiltinInfo>& builtins() {
+  static const std::vector<BuiltinInfo> v = {
+#define SNL_X(T, C, N) {#N, (int)offsetof(SnlParams, N), C, (int)sizeof(T)},
+      SNL_BUILTIN_SAMPLERS(SNL_X) SNL_BUILTIN_VALUES(SNL_X)
+#undef SNL_X
+  };
+  return v;
+}
+const std::set<std::string>& builtinNames() {
+  static std::set<std::string> s = [] {
+    std::set<std::string> r;
+    for (auto& b : builtins()) r.insert(b.name);
+    return r;
+  }();
+  return s;
+}
+
+unsigned long long fnv1a(const std::string& s, unsigned long long h = 1469598103934665603ull) {
+  for (unsigned char c : s) { h ^= c; h *= 1099511628211ull; }
+  return h;
+}
+
+std::string libDir() {
+  Dl_info info;
+  if (dladdr((void*)&fnv1a, &info) && info.dli_fname) {
+    std::string p = info.dli_fname;
+    size_t s = p.rfind('/');
+    if (s != std::string::npos) return p.substr(0, s);
+  }
+  return ".";
+}
+std::string cacheDir() {
+  const char* e = getenv("SNL_KCACHE");
+  std::string d = e && *e ? e : libDir() + "/_kcache";
+  mkdir(d.c_str(), 0755);
+  return d;
+}
+
+struct Compiled {
+  std::vector<char> code;
+  snl::TranslateResult tr;
+  std::set<std::string> defines;
+};
+
+// Assemble the hipRTC source, compile (or fetch from the on-disk cache).
+int compileProgram(snl_ctx ctx, const char* shader, const char* ior, const char* defines, int maxMarch,
+                   int maxBounces, int maxAtm, int spf, unsigned flags, Compiled& out, char* log, int loglen) {
+  std::string glsl = std::string(shader ? shader : "") + "\n" + (ior ? ior : "") + "\n";
+  out.defines = snl::parseDefines(defines ? defines : "");
+  if (!snl::translateScene(glsl, builtinNames(), out.defines, out.tr)) {
+    if (log && loglen > 0) snprintf(log, loglen, "%s", out.tr.error.c_str());
+    return fail(ctx, SNL_E_COMPILE, "scene translation failed: %s", out.tr.error.c_str());
+  }
+  std::ostringstream src;
+  src << "#include \"snl_prelude.h\"\n#include \"snl_wavefront.h\"\n";
+  src << (defines ? defines : "") << "\n";
+  src << "namespace glsl {\n" << out.tr.code << "\n}  // namespace glsl\n";
+  src << "#line 1 \"snl_pt.h\"\n#include \"snl_pt.h\"\n#include \"snl_kernels.h\"\n";
+  std::string source = src.str();
+
+  std::vector<std::string> opts = {"--offload-arch=gfx950", "-O3", "-std=c++17",
+                                   "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                                   "-DSNL_MAX_MARCH_STEPS=" + std::to_string(maxMarch),
+                                   "-DSNL_MAX_BOUNCES=" + std::to_string(maxBounces),
+                                   "-DSNL_MAX_ATMOSPHERE_SCATTERS=" + std::to_string(maxAtm),
+                                   "-DSNL_MAX_SAMPLES_PER_FRAME=" + std::to_string(spf)};
+  if (flags & SNL_BUILD_PRECISE) opts.push_back("-ffp-contract=off");
+  std::string key = source;
+  for (auto& o : opts) key += "\n" + o;
+  for (int i = 0; i < kNumEmbedded; ++i) key += kEmbeddedSources[i];
+  char name[64];
+  snprintf(name, sizeof name, "%016llx.hsaco", fnv1a(key));
+  std::string path = cacheDir() + "/" + name;
+  {
+    std::ifstream f(path, std::ios::binary);
+    if (f) {
+      out.code.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+      if (!out.code.empty()) return SNL_OK;
+    }
+  }
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, source.c_str(), "snl_program.hip", kNumEmbedded, kEmbeddedSources,
+                          kEmbeddedNames) != HIPRTC_SUCCESS)
+    return fail(ctx, SNL_E_COMPILE, "hiprtcCreateProgram failed");
+  std::vector<const char*> copts;
+  for (auto& o : opts) copts.push_back(o.c_str());
+  hiprtcResult r = hiprtcCompileProgram(prog, (int)copts.size(), copts.data());
+  size_t ls = 0;
+  hiprtcGetProgramLogSize(prog, &ls);
+  std::string plog(ls + 1, '\0');
+  if (ls) hiprtcGetProgramLog(prog, &plog[0]);
+  if (r != HIPRTC_SUCCESS) {
+    if (log && loglen > 0) snprintf(log, loglen, "%s", plog.c_str());
+    hiprtcDestroyProgram(&prog);
+    std::string head = plog.substr(0, 1500);
+    return fail(ctx, SNL_E_COMPILE, "hipRTC compile failed:\n%s", head.c_str());
+  }
+  size_t cs = 0;
+  hiprtcGetCodeSize(prog, &cs);
+  out.code.resize(cs);
+  hiprtcGetCode(prog, out.code.data());
+  hiprtcDestroyProgram(&prog);
+  std::string tmp = path + ".tmp" + std::to_string(getpid());
+  {
+    std::ofstream f(tmp, std::ios::binary);
+    f.write(out.code.data(), (std::streamsize)out.code.size());
+  }
+  rename(tmp.c_str(), path.c_str());
+  return SNL_OK;
+}
+
+void fillArgs(snl_prog p, snl_state st, SnlKArgs& a) {
+  memset(&a, 0, sizeof a);
+  a.P = p->dP;
+  a.U = p->dU;
+  long long cap = st->npix;
+  a.cap = cap;
+  a.pf = st->pf;
+  a.pi = st->pi;
+  a.qmat = st->q + 4 * cap;
+  a.cnt = st->cnt;
+  a.radiance = st->radiance;
+  a.rngstate = st->rng;
+  a.W = st->w;
+  a.H = st->h;
+  a.npix = (int)st->npix;
+  a.spf = p->spf;
+  a.chunk = p->traceChunk;
+}
+
+int launch(snl_ctx ctx, hipFunction_t f, int grid, int block, SnlKArgs& a) {
+  void* args[] = {&a};
+  HIPCHK(ctx, hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, 0, ctx->stream, args, nullptr));
+  return SNL_OK;
+}
+
+int upload(snl_prog p) {
+  if (!p->dirty) return SNL_OK;
+  HIPCHK(p->ctx, hipMemcpyAsync(p->dP, &p->hostP, sizeof(SnlParams), hipMemcpyHostToDevice, p->ctx->stream));
+  if (!p->hostU.empty())
+    HIPCHK(p->ctx, hipMemcpyAsync(p->dU, p->hostU.data(), p->hostU.size(), hipMemcpyHostToDevice, p->ctx->stream));
+  // the host shadow may change right after this call returns
+  HIPCHK(p->ctx, hipStreamSynchronize(p->ctx->stream));
+  p->dirty = false;
+  return SNL_OK;
+}
+
+// PCG32-seeded integer RNG states (see snelly.h: snl_state_seed)
+__device__ inline unsigned pcg_next(unsigned long long& state, unsigned long long inc) {
+  unsigned long long old = state;
+  state = old * 6364136223846793005ull + inc;
+  unsigned xorshifted = (unsigned)(((old >> 18u) ^ old) >> 27u);
+  unsigned rot = (unsigned)(old >> 59u);
+  return (xorshifted >> rot) | (xorshifted << ((-rot) & 31u));
+}
+__global__ void seed_kernel(float* radiance, float* rngstate, long long npix, unsigned seed, int mode) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npix) return;
+  if (mode & 1) {
+    unsigned long long state = 0ull, inc = (54ull << 1u) | 1u;
+    pcg_next(state, inc);
+    state += (unsigned long long)((unsigned)i ^ seed);
+    pcg_next(state, inc);
+    const unsigned m[4] = {4194287u, 4194277u, 4194191u, 4194167u};
+    for (int k = 0; k < 4; ++k) rngstate[4 * i + k] = float(1u + pcg_next(state, inc) % (m[k] - 1u));
+  }
+  if (mode & 2) {
+    radiance[4 * i] = 0.0f;
+    radiance[4 * i + 1] = 0.0f;
+    radiance[4 * i + 2] = 0.0f;
+    radiance[4 * i + 3] = 1.0f;
+  }
+}
+
+int blocksFor(long long n, int bs = 256) { return (int)((n + bs - 1) / bs); }
+
+}  // namespace
+
+// ---------------------------------------------------------------- API
+extern "C" {
+
+const char* snl_version(void) { return "snelly-mi355x 0.1 (gfx950)"; }
+
+const char* snl_last_error(snl_ctx ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+int snl_ctx_create(int device, snl_ctx* out) {
+  if (!out) return fail(nullptr, SNL_E_ARG, "out is NULL");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= device || device < 0)
+    return fail(nullptr, SNL_E_HIP, "no HIP device %d (count %d): %s", device, n, hipGetErrorString(e));
+  std::unique_ptr<snl_ctx_s> c(new snl_ctx_s);
+  c->device = device;
+  HIPCHK(nullptr, hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIPCHK(nullptr, hipGetDeviceProperties(&prop, device));
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(nullptr, SNL_E_UNSUPPORTED, "device %d is %s, this library is built for gfx950", device, prop.gcnArchName);
+  c->numCU = prop.multiProcessorCount;
+  HIPCHK(nullptr, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  *out = c.release();
+  return SNL_OK;
+}
+
+int snl_ctx_destroy(snl_ctx ctx) {
+  if (!ctx) return SNL_OK;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return SNL_OK;
+}
+
+int snl_sync(snl_ctx ctx) {
+  if (!ctx) return fail(nullptr, SNL_E_ARG, "ctx is NULL");
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return SNL_OK;
+}
+
+int snl_program_precompile(const char* shader, const char* ior, const char* defines, int maxMarch, int maxBounces,
+                           int maxAtm, int spf, unsigned flags, char* log, int loglen) {
+  Compiled c;
+  return compileProgram(nullptr, shader, ior, defines, maxMarch, maxBounces, maxAtm, spf, flags, c, log, loglen);
+}
+
+int snl_program_build(snl_ctx ctx, const char* shader, const char* ior, const char* defines, int maxMarch,
+                      int maxBounces, int maxAtm, int spf, unsigned flags, snl_prog* out, char* log, int loglen) {
+  if (!ctx || !out) return fail(ctx, SNL_E_ARG, "NULL argument");
+  if (maxMarch < 1 || maxBounces < 0 || spf < 1) return fail(ctx, SNL_E_ARG, "bad loop bounds");
+  Compiled c;
+  int rc = compileProgram(ctx, shader, ior, defines, maxMarch, maxBounces, maxAtm, spf, flags, c, log, loglen);
+  if (rc) return rc;
+  if (c.defines.count("HAS_ATMOSPHERE"))
+    return fail(ctx, SNL_E_UNSUPPORTED, "the homogeneous atmosphere (HAS_ATMOSPHERE) is not implemented yet");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  std::unique_ptr<snl_prog_s> p(new snl_prog_s);
+  p->ctx = ctx;
+  HIPCHK(ctx, hipModuleLoadData(&p->mod, c.code.data()));
+  struct { hipFunction_t* f; const char* n; } fns[] = {
+      {&p->k_reset, "snl_counters_reset"}, {&p->k_rotate, "snl_counters_rotate"}, {&p->k_camera, "snl_camera"},
+      {&p->k_trace, "snl_trace"}, {&p->k_bin, "snl_bin"}, {&p->k_miss, "snl_shade_miss"},
+      {&p->k_surf, "snl_shade_surface"}, {&p->k_metal, "snl_shade_metal"}, {&p->k_diele, "snl_shade_dielectric"},
+      {&p->k_accum, "snl_accumulate"}, {&p->k_normals, "snl_normals"}, {&p->k_pick, "snl_pick"},
+      {&p->k_tonemap, "snl_tonemap"}};
+  for (auto& f : fns) HIPCHK(ctx, hipModuleGetFunction(f.f, p->mod, f.n));
+  p->hasSurface = c.defines.count("HAS_SURFACE") > 0;
+  p->hasMetal = c.defines.count("HAS_METAL") > 0;
+  p->hasDiele = c.defines.count("HAS_DIELECTRIC") > 0;
+  p->maxBounces = maxBounces;
+  p->maxMarch = maxMarch;
+  p->spf = spf;
+  memset(&p->hostP, 0, sizeof(SnlParams));
+  p->user = c.tr.uniforms;
+  p->hostU.assign(c.tr.userBlockSize, 0);
+  HIPCHK(ctx, hipMalloc((void**)&p->dP, sizeof(SnlParams)));
+  HIPCHK(ctx, hipMalloc(&p->dU, p->hostU.size()));
+  p->traceBlocks = ctx->numCU * 8;
+  p->traceChunk = 64;
+  p->shadeBlocks = ctx->numCU * 8;
+  HIPCHK(ctx, hipEventCreate(&p->ev0));
+  HIPCHK(ctx, hipEventCreate(&p->ev1));
+  HIPCHK(ctx, hipEventCreate(&p->et0));
+  HIPCHK(ctx, hipEventCreate(&p->et1));
+  *out = p.release();
+  return SNL_OK;
+}
+
+int snl_program_destroy(snl_prog p) {
+  if (!p) return SNL_OK;
+  hipSetDevice(p->ctx->device);
+  hipStreamSynchronize(p->ctx->stream);
+  if (p->dP) hipFree(p->dP);
+  if (p->dU) hipFree(p->dU);
+  if (p->mod) hipModuleUnload(p->mod);
+  for (hipEvent_t e : {p->ev0, p->ev1, p->et0, p->et1}) if (e) hipEventDestroy(e);
+  for (hipEvent_t e : p->traceEvents) hipEventDestroy(e);
+  delete p;
+  return SNL_OK;
+}
+
+int snl_uniform_index(snl_prog p, const char* name) {
+  if (!p || !name) return -1;
+  const auto& b = builtins();
+  for (size_t i = 0; i < b.size(); ++i) if (!strcmp(b[i].name, name)) return (int)i;
+  for (size_t j = 0; j < p->user.size(); ++j) if (p->user[j].name == name) return 1000 + (int)j;
+  return -1;
+}
+
+static int typeClass(int t) { return (t == SNL_T_INT) ? 1 : (t >= SNL_T_SAMPLER2D ? 2 : 0); }
+static int typeFloats(int t) {
+  switch (t) {
+    case SNL_T_FLOAT: case SNL_T_INT: return 1;
+    case SNL_T_VEC2: return 2;
+    case SNL_T_VEC3: return 3;
+    case SNL_T_VEC4: case SNL_T_MAT2: return 4;
+    case SNL_T_MAT3: return 9;
+    case SNL_T_MAT4: return 16;
+  }
+  return 0;
+}
+
+int snl_uniform_set(snl_prog p, int idx, int type, const void* data, int count) {
+  if (!p) return fail(nullptr, SNL_E_ARG, "prog is NULL");
+  if (idx < 0) return SNL_OK;  // unknown uniform: silently ignored, as WebGL does for a null location
+  if (!data || count <= 0) return fail(p->ctx, SNL_E_ARG, "no data");
+  char* base;
+  int off, utype, bytes;
+  if (idx < 1000) {
+    const auto& b = builtins();
+    if (idx >= (int)b.size()) return fail(p->ctx, SNL_E_ARG, "bad uniform index %d", idx);
+    base = reinterpret_cast<char*>(&p->hostP);
+    off = b[idx].offset;
+    utype = b[idx].type;
+    bytes = b[idx].size;
+  } else {
+    int j = idx - 1000;
+    if (j >= (int)p->user.size()) return fail(p->ctx, SNL_E_ARG, "bad uniform index %d", idx);
+    base = p->hostU.data();
+    off = p->user[j].offset;
+    utype = p->user[j].type;
+    bytes = p->user[j].size * p->user[j].count;
+  }
+  // a float setter on an int uniform (or the reverse) is a GL error with no effect
+  if (typeClass(type) != typeClass(utype)) return SNL_OK;
+  if (typeClass(type) == 2) return fail(p->ctx, SNL_E_ARG, "use snl_uniform_texture for samplers");
+  int n = typeFloats(type) * count * 4;
+  memcpy(base + off, data, (size_t)(n < bytes ? n : bytes));
+  p->dirty = true;
+  return SNL_OK;
+}
+
+int snl_uniform_texture(snl_prog p, int idx, snl_tex t) {
+  if (!p) return fail(nullptr, SNL_E_ARG, "prog is NULL");
+  if (idx < 0) return SNL_OK;
+  char* base;
+  int off, utype;
+  if (idx < 1000) {
+    const auto& b = builtins();
+    if (idx >= (int)b.size()) return fail(p->ctx, SNL_E_ARG, "bad uniform index");
+    base = reinterpret_cast<char*>(&p->hostP);
+    off = b[idx].offset;
+    utype = b[idx].type;
+  } else {
+    int j = idx - 1000;
+    if (j >= (int)p->user.size()) return fail(p->ctx, SNL_E_ARG, "bad uniform index");
+    base = p->hostU.data();
+    off = p->user[j].offset;
+    utype = p->user[j].type;
+  }
+  if (utype == SNL_T_SAMPLER2D) {
+    sampler2D s{t ? t->d : nullptr, t ? t->w : 0, t ? t->h : 0, t ? t->ch : 4, t ? t->flags : 0};
+    memcpy(base + off, &s, sizeof s);
+  } else if (utype == SNL_T_SAMPLER3D) {
+    sampler3D s{t ? t->d : nullptr, t ? t->w : 0, t ? t->h : 0, t ? t->depth : 0, t ? t->ch : 4, t ? t->flags : 0};
+    memcpy(base + off, &s, sizeof s);
+  } else {
+    return SNL_OK;  // not a sampler: GL error, no effect
+  }
+  p->dirty = true;
+  return SNL_OK;
+}
+
+int snl_texture_create(snl_ctx ctx, int w, int h, int depth, int ch, int linear, int clamp, const float* texels,
+                       snl_tex* out) {
+  if (!ctx || !out || w <= 0 || h <= 0 || depth <= 0 || ch < 1 || ch > 4) return fail(ctx, SNL_E_ARG, "bad texture shape");
+  std::unique_ptr<snl_tex_s> t(new snl_tex_s);
+  t->ctx = ctx;
+  t->w = w; t->h = h; t->depth = depth; t->ch = ch;
+  t->flags = (linear ? 1 : 0) | (clamp ? 2 : 0);
+  size_t bytes = (size_t)w * h * depth * ch * sizeof(float);
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (hipMalloc((void**)&t->d, bytes) != hipSuccess) return fail(ctx, SNL_E_NOMEM, "texture allocation of %zu bytes failed", bytes);
+  if (texels) HIPCHK(ctx, hipMemcpyAsync(t->d, texels, bytes, hipMemcpyHostToDevice, ctx->stream));
+  else HIPCHK(ctx, hipMemsetAsync(t->d, 0, bytes, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  *out = t.release();
+  return SNL_OK;
+}
+
+int snl_texture_upload(snl_tex t, const float* texels) {
+  if (!t || !texels) return fail(t ? t->ctx : nullptr, SNL_E_ARG, "NULL argument");
+  size_t bytes = (size_t)t->w * t->h * t->depth * t->ch * sizeof(float);
+  HIPCHK(t->ctx, hipMemcpyAsync(t->d, texels, bytes, hipMemcpyHostToDevice, t->ctx->stream));
+  HIPCHK(t->ctx, hipStreamSynchronize(t->ctx->stream));
+  return SNL_OK;
+}
+
+int snl_texture_destroy(snl_tex t) {
+  if (!t) return SNL_OK;
+  hipStreamSynchronize(t->ctx->stream);
+  if (t->d) hipFree(t->d);
+  delete t;
+  return SNL_OK;
+}
+
+int snl_state_create(snl_ctx ctx, int w, int h, snl_state* out) {
+  if (!ctx || !out || w <= 0 || h <= 0) return fail(ctx, SNL_E_ARG, "bad state size");
+  std::unique_ptr<snl_state_s> s(new snl_state_s);
+  s->ctx = ctx;
+  s->w = w; s->h = h;
+  s->npix = (long long)w * h;
+  long long n = s->npix;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  struct { void** p; size_t bytes; } allocs[] = {
+      {(void**)&s->radiance, (size_t)n * 16}, {(void**)&s->rng, (size_t)n * 16},
+      {(void**)&s->pf, (size_t)n * F_COUNT * 4}, {(void**)&s->pi, (size_t)n * I_COUNT * 4},
+      {(void**)&s->q, (size_t)n * 8 * 4}, {(void**)&s->cnt, (size_t)C_COUNT * 4}};
+  for (auto& a : allocs) {
+    if (hipMalloc(a.p, a.bytes) != hipSuccess) {
+      for (auto& b : allocs) if (*b.p) hipFree(*b.p);
+      return fail(ctx, SNL_E_NOMEM, "state allocation of %zu bytes failed", a.bytes);
+    }
+  }
+  HIPCHK(ctx, hipMemsetAsync(s->cnt, 0, C_COUNT * 4, ctx->stream));
+  HIPCHK(ctx, hipMemsetAsync(s->pi, 0, (size_t)n * I_COUNT * 4, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  *out = s.release();
+  return snl_state_seed(*out, 0);
+}
+
+int snl_state_destroy(snl_state s) {
+  if (!s) return SNL_OK;
+  hipSetDevice(s->ctx->device);
+  hipStreamSynchronize(s->ctx->stream);
+  for (void* p : {(void*)s->radiance, (void*)s->rng, (void*)s->pf, (void*)s->pi, (void*)s->q, (void*)s->cnt})
+    if (p) hipFree(p);
+  delete s;
+  return SNL_OK;
+}
+
+static int seedMode(snl_state s, unsigned seed, int mode) {
+  if (!s) return fail(nullptr, SNL_E_ARG, "state is NULL");
+  hipLaunchKernelGGL(seed_kernel, dim3(blocksFor(s->npix)), dim3(256), 0, s->ctx->stream, s->radiance, s->rng,
+                     s->npix, seed, mode);
+  HIPCHK(s->ctx, hipGetLastError());
+  HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
+  return SNL_OK;
+}
+int snl_state_seed(snl_state s, unsigned seed) { return seedMode(s, seed, 3); }
+int snl_state_clear(snl_state s) { return seedMode(s, 0, 2); }
+
+int snl_state_read(snl_state s, int which, float* outp) {
+  if (!s || !outp || which < 0 || which > 1) return fail(s ? s->ctx : nullptr, SNL_E_ARG, "bad argument");
+  HIPCHK(s->ctx, hipMemcpyAsync(outp, which ? s->rng : s->radiance, (size_t)s->npix * 16, hipMemcpyDeviceToHost,
+                                s->ctx->stream));
+  HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
+  return SNL_OK;
+}
+
+int snl_state_write(snl_state s, int which, const float* in) {
+  if (!s || !in || which < 0 || which > 1) return fail(s ? s->ctx : nullptr, SNL_E_ARG, "bad argument");
+  HIPCHK(s->ctx, hipMemcpyAsync(which ? s->rng : s->radiance, in, (size_t)s->npix * 16, hipMemcpyHostToDevice,
+                                s->ctx->stream));
+  HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
+  return SNL_OK;
+}
+
+int snl_render(snl_prog p, snl_state st, int frames) {
+  if (!p || !st || frames < 0) return fail(p ? p->ctx : nullptr, SNL_E_ARG, "bad argument");
+  snl_ctx ctx = p->ctx;
+  if (st->ctx != ctx) return fail(ctx, SNL_E_ARG, "program and state belong to different contexts");
+  int rc = upload(p);
+  if (rc) return rc;
+  hipStream_t s = ctx->stream;
+  SnlKArgs a;
+  fillArgs(p, st, a);
+  long long cap = st->npix;
+  int* qt[2] = {st->q, st->q + cap};
+  int* qn[2] = {st->q + 2 * cap, st->q + 3 * cap};
+  int pixBlocks = blocksFor(st->npix);
+  int shadeBlocks = std::min(pixBlocks, p->shadeBlocks);
+  p->stats = snl_stats{};
+  a.vertex = -2;  // zero every counter, the step counter included
+  if ((rc = launch(ctx, p->k_reset, 1, 64, a))) return rc;
+  HIPCHK(ctx, hipEventRecord(p->ev0, s));
+  double traceMs = 0.0;
+  int traceLaunches = 0;
+  for (int f = 0; f < frames; ++f) {
+    for (int smp = 0; smp < p->spf; ++smp) {
+      a.sample = smp;
+      int cur = 0;
+      a.qtrace_in = qt[1]; a.qnee_in = qn[1]; a.qtrace_out = qt[0]; a.qnee_out = qn[0];
+      a.vertex = 0;
+      if ((rc = launch(ctx, p->k_reset, 1, 64, a))) return rc;
+      if ((rc = launch(ctx, p->k_camera, pixBlocks, 256, a))) return rc;
+      for (int v = 0; v <= p->maxBounces + 1; ++v) {
+        bool last = (v == p->maxBounces + 1);
+        a.vertex = last ? -1 : v;
+        if ((rc = launch(ctx, p->k_rotate, 1, 64, a))) return rc;
+        // the queues written by the previous stage are read now
+        a.qtrace_in = qt[cur]; a.qnee_in = qn[cur];
+        a.qtrace_out = qt[1 - cur]; a.qnee_out = qn[1 - cur];
+        cur = 1 - cur;
+        if (p->profiling) {
+          while ((int)p->traceEvents.size() < 2 * (traceLaunches + 1)) {
+            hipEvent_t e;
+            HIPCHK(ctx, hipEventCreate(&e));
+            p->traceEvents.push_back(e);
+          }
+          HIPCHK(ctx, hipEventRecord(p->traceEvents[2 * traceLaunches], s));
+        }
+        if ((rc = launch(ctx, p->k_trace, p->traceBlocks, 256, a))) return rc;
+        if (p->profiling) HIPCHK(ctx, hipEventRecord(p->traceEvents[2 * traceLaunches + 1], s));
+        ++traceLaunches;
+        if (last) break;
+        if ((rc = launch(ctx, p->k_bin, shadeBlocks, 256, a))) return rc;
+        if ((rc = launch(ctx, p->k_miss, shadeBlocks, 256, a))) return rc;
+        if (p->hasSurface && (rc = launch(ctx, p->k_surf, shadeBlocks, 256, a))) return rc;
+        if (p->hasMetal && (rc = launch(ctx, p->k_metal, shadeBlocks, 256, a))) return rc;
+        if (p->hasDiele && (rc = launch(ctx, p->k_diele, shadeBlocks, 256, a))) return rc;
+      }
+      if ((rc = launch(ctx, p->k_accum, pixBlocks, 256, a))) return rc;
+    }
+  }
+  HIPCHK(ctx, hipEventRecord(p->ev1, s));
+  HIPCHK(ctx, hipEventSynchronize(p->ev1));
+  float ms = 0.0f;
+  HIPCHK(ctx, hipEventElapsedTime(&ms, p->ev0, p->ev1));
+  if (p->profiling) {
+    for (int k = 0; k < traceLaunches; ++k) {
+      float t = 0.0f;
+      HIPCHK(ctx, hipEventElapsedTime(&t, p->traceEvents[2 * k], p->traceEvents[2 * k + 1]));
+      traceMs += t;
+    }
+  }
+  long long steps = 0;
+  HIPCHK(ctx, hipMemcpyAsync(&steps, st->cnt + C_TRACE_STEPS, 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(ctx, hipStreamSynchronize(s));
+  p->stats.ms_total = ms;
+  p->stats.ms_trace = traceMs;
+  p->stats.trace_steps = steps;
+  p->stats.paths = (long long)frames * p->spf * st->npix;
+  p->stats.trace_launches = traceLaunches;
+  return SNL_OK;
+}
+
+int snl_render_normals(snl_prog p, snl_state st, int frames) {
+  if (!p || !st || frames < 0) return fail(p ? p->ctx : nullptr, SNL_E_ARG, "bad argument");
+  int rc = upload(p);
+  if (rc) return rc;
+  SnlKArgs a;
+  fillArgs(p, st, a);
+  for (int f = 0; f < frames; ++f)
+    if ((rc = launch(p->ctx, p->k_normals, blocksFor(st->npix), 256, a))) return rc;
+  HIPCHK(p->ctx, hipStreamSynchronize(p->ctx->stream));
+  return SNL_OK;
+}
+
+int snl_pick(snl_prog p, const float* xy, int n, float* outp) {
+  if (!p || !xy || !outp || n < 0) return fail(p ? p->ctx : nullptr, SNL_E_ARG, "bad argument");
+  if (n == 0) return SNL_OK;
+  int rc = upload(p);
+  if (rc) return rc;
+  snl_ctx ctx = p->ctx;
+  float *dxy = nullptr, *dout = nullptr;
+  HIPCHK(ctx, hipMalloc((void**)&dxy, (size_t)n * 8));
+  if (hipMalloc((void**)&dout, (size_t)n * 8) != hipSuccess) { hipFree(dxy); return fail(ctx, SNL_E_NOMEM, "pick buffers"); }
+  SnlKArgs a;
+  memset(&a, 0, sizeof a);
+  a.P = p->dP;
+  a.U = p->dU;
+  void* args[] = {&a, &dxy, &dout, &n};
+  // every copy goes through the context's (non-blocking) stream
+  hipError_t e = hipMemcpyAsync(dxy, xy, (size_t)n * 8, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipModuleLaunchKernel(p->k_pick, blocksFor(n), 1, 1, 256, 1, 1, 0, ctx->stream, args, nullptr);
+  if (e == hipSuccess) e = hipMemcpyAsync(outp, dout, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  hipFree(dxy);
+  hipFree(dout);
+  if (e != hipSuccess) return fail(ctx, SNL_E_HIP, "pick: %s", hipGetErrorString(e));
+  return SNL_OK;
+}
+
+int snl_tonemap(snl_prog p, snl_state st, float* rgb, unsigned char* rgba8) {
+  if (!p || !st) return fail(p ? p->ctx : nullptr, SNL_E_ARG, "bad argument");
+  int rc = upload(p);
+  if (rc) return rc;
+  snl_ctx ctx = p->ctx;
+  long long n = st->npix;
+  float* drgb = nullptr;
+  unsigned char* d8 = nullptr;
+  if (rgb) HIPCHK(ctx, hipMalloc((void**)&drgb, (size_t)n * 12));
+  if (rgba8 && hipMalloc((void**)&d8, (size_t)n * 4) != hipSuccess) { if (drgb) hipFree(drgb); return fail(ctx, SNL_E_NOMEM, "tonemap buffer"); }
+  int npix = (int)n;
+  const SnlParams* dP = p->dP;
+  float* rad = st->radiance;
+  void* args[] = {&dP, &rad, &drgb, &d8, &npix};
+  hipError_t e = hipModuleLaunchKernel(p->k_tonemap, blocksFor(n), 1, 1, 256, 1, 1, 0, ctx->stream, args, nullptr);
+  if (e == hipSuccess && rgb) e = hipMemcpyAsync(rgb, drgb, (size_t)n * 12, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess && rgba8) e = hipMemcpyAsync(rgba8, d8, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (drgb) hipFree(drgb);
+  if (d8) hipFree(d8);
+  if (e != hipSuccess) return fail(ctx, SNL_E_HIP, "tonemap: %s", hipGetErrorString(e));
+  return SNL_OK;
+}
+
+int snl_get_stats(snl_prog p, snl_stats* out) {
+  if (!p || !out) return fail(nullptr, SNL_E_ARG, "bad argument");
+  *out = p->stats;
+  return SNL_OK;
+}
+
+int snl_set_tuning(snl_prog p, int traceBlocks, int traceChunk, int shadeBlocks) {
+  if (!p) return fail(nullptr, SNL_E_ARG, "prog is NULL");
+  if (traceBlocks > 0) p->traceBlocks = traceBlocks;
+  if (traceChunk > 0) p->traceChunk = traceChunk;
+  if (shadeBlocks > 0) p->shadeBlocks = shadeBlocks;
+  return SNL_OK;
+}
+
+int snl_set_profiling(snl_prog p, int on) {
+  if (!p) return fail(nullptr, SNL_E_ARG, "prog is NULL");
+  p->profiling = on != 0;
+  return SNL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,180 @@
+"""Parity of the HIP path (libsnelly_hip.so through its C ABI) with the CPU oracle
+on the same seeded inputs.
+
+Integer work (RNG seeding, RNG state evolution) must be bit-exact.  Float work
+is compared per pixel: programs built with SNL_BUILD_PRECISE (no FMA
+contraction) follow the oracle's float operations, so almost every pixel's
+path makes identical random decisions and agrees to 1e-4 relative; the rest
+(a transcendental rounding flips one branch of a path) are bounded by the
+image-level relative RMSE.  The default (FMA) build is held to the same
+image-level bound.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_py
+from snelly_amd import host, native, scenes
+from test_oracle import furnace_check, silhouette, unbiased
+
+pytestmark = pytest.mark.gpu
+
+PRECISE = native.BUILD_PRECISE
+
+
+def session(ctx, scene, w, h, flags=PRECISE):
+    return host.Session(scene, w, h, ctx, flags=flags)
+
+
+def rrmse(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def pixel_grid(w, h):
+    return np.stack(np.meshgrid(np.arange(w) + 0.5, np.arange(h) + 0.5), -1).reshape(-1, 2).astype(np.float32)
+
+
+@pytest.mark.parametrize("w,h,seed", [(64, 64, 1234), (37, 23, 0), (1, 1, 42), (128, 96, 0xDEADBEEF)])
+def test_seed_bit_exact(ctx, w, h, seed):
+    st = native.State(ctx, w, h, seed)
+    got = st.read(1).reshape(-1, 4)
+    rad = st.read(0).reshape(-1, 4)
+    st.close()
+    assert np.array_equal(got, oracle_py.Oracle.seed(w * h, seed))
+    assert np.array_equal(rad, np.tile([0.0, 0.0, 0.0, 1.0], (w * h, 1)).astype(np.float32))
+
+
+@pytest.mark.parametrize("make", [scenes.sphere_plane, scenes.menger_metal, scenes.mandelbox_dielectric])
+def test_first_hit_depth(ctx, make):
+    """pick program per pixel: distance to 1e-4 relative, same material"""
+    W, H = 64, 48
+    s = make(W, H, 1)
+    sess = session(ctx, s, W, H)
+    o = oracle_py.Oracle(s, W, H)
+    xy = pixel_grid(W, H)
+    g, r = sess.prog.pick(xy), o.pick(xy)
+    sess.close()
+    same = g[:, 1] == r[:, 1]
+    assert same.mean() >= 0.995
+    rel = np.abs(g[same, 0] - r[same, 0]) / np.abs(r[same, 0])
+    assert np.mean(rel <= 1e-4) >= 0.995
+    assert np.median(rel) < 1e-6
+
+
+@pytest.mark.parametrize("make", [scenes.sphere_plane, scenes.menger_metal])
+def test_first_hit_normals(ctx, make):
+    """normals render mode (jittered primary ray, SDF gradient) per pixel"""
+    W, H = 64, 48
+    s = make(W, H, 1)
+    sess = session(ctx, s, W, H)
+    sess.prog.render_normals(sess.state, 4)
+    g = sess.radiance().reshape(-1, 4)
+    rng_g = sess.state.read(1).reshape(-1, 4)
+    sess.close()
+    o = oracle_py.Oracle(s, W, H)
+    rad, rng = o.new_state()
+    o.normals(rad, rng, 4)
+    assert np.array_equal(rng_g, rng)
+    sil = silhouette(o, W, H)
+    rel = np.max(np.abs(g[:, :3] - rad[:, :3]) / np.maximum(np.abs(rad[:, :3]), 1e-3), axis=1)
+    assert np.mean(rel[~sil] <= 1e-4) >= 0.99
+    assert np.array_equal(g[:, 3], rad[:, 3])
+
+
+def compare_render(ctx, s, W, H, frames, flags=PRECISE):
+    sess = session(ctx, s, W, H, flags)
+    sess.render(frames)
+    g = sess.radiance().reshape(-1, 4)
+    rng_g = sess.state.read(1).reshape(-1, 4)
+    sess.close()
+    o = oracle_py.Oracle(s, W, H)
+    rad, rng = o.new_state()
+    o.render(rad, rng, frames)
+    close = np.all(np.abs(g[:, :3] - rad[:, :3]) <= 1e-4 * np.maximum(np.abs(rad[:, :3]), 1e-3), axis=1)
+    rng_same = np.all(rng_g == rng, axis=1)
+    assert np.array_equal(g[:, 3], rad[:, 3])
+    return g, rad, close.mean(), rng_same.mean(), o
+
+
+def test_pathtrace_sphere_plane(ctx):
+    """config-1 scene at 64x64, 16 spp: per-pixel agreement with the oracle"""
+    s = scenes.sphere_plane(64, 64, 16)
+    g, r, close, rng_same, _ = compare_render(ctx, s, 64, 64, 16)
+    assert close >= 0.99 and rng_same >= 0.99
+    assert rrmse(unbiased(g), unbiased(r)) < 2e-3
+
+
+def test_pathtrace_menger_metal(ctx):
+    s = scenes.menger_metal(64, 64, 8)
+    g, r, close, rng_same, _ = compare_render(ctx, s, 64, 64, 8)
+    assert close >= 0.95 and rng_same >= 0.95
+    assert rrmse(unbiased(g), unbiased(r)) < 1e-2
+
+
+def test_pathtrace_mandelbox_dielectric(ctx):
+    s = scenes.mandelbox_dielectric(48, 32, 4, iterations=8)
+    s.renderer.maxBounces = 8
+    g, r, close, rng_same, _ = compare_render(ctx, s, 48, 32, 4)
+    assert close >= 0.8
+    assert rrmse(unbiased(g), unbiased(r)) < 5e-2
+
+
+def test_pathtrace_fast_build_statistics(ctx):
+    """the default FMA build against the oracle: image-level agreement"""
+    s = scenes.sphere_plane(64, 64, 16)
+    g, r, close, _, _ = compare_render(ctx, s, 64, 64, 16, flags=0)
+    assert close >= 0.9
+    assert rrmse(unbiased(g), unbiased(r)) < 5e-3
+
+
+def test_furnace(ctx):
+    W = H = 32
+    s = scenes.furnace(W, H, 64)
+    g, r, close, rng_same, o = compare_render(ctx, s, W, H, 64)
+    assert rng_same == 1.0  # every path draws the same number of random numbers
+    furnace_check(unbiased(g), silhouette(o, W, H), 64)
+
+
+def test_samples_per_frame_and_interactive_skip(ctx):
+    """maxSamplesPerFrame = 2 (sample loop 1954-1972) and the interactive skip
+    (main 1994-2005, skipProbability 0.5) against the oracle"""
+    s = scenes.sphere_plane(32, 32, 4)
+    s.renderer.maxSamplesPerFrame = 2
+    s.renderer.interactive = True
+    s.renderer.skipProbability = 0.5
+    g, r, close, rng_same, _ = compare_render(ctx, s, 32, 32, 4)
+    assert close >= 0.99 and rng_same >= 0.99
+
+
+def test_tonemap(ctx):
+    W, H = 32, 32
+    s = scenes.sphere_plane(W, H, 4)
+    s.renderer.exposure = 0.5
+    s.renderer.saturation = 1.2
+    s.renderer.contrast = 1.1
+    sess = session(ctx, s, W, H)
+    sess.render(4)
+    rgb, rgba8 = sess.prog.tonemap(sess.state)
+    rad = sess.radiance()
+    sess.close()
+    o = oracle_py.Oracle(s, W, H)
+    ref = o.tonemap(rad)
+    assert np.allclose(rgb.reshape(-1, 3), ref, rtol=1e-5, atol=1e-6)
+    assert np.array_equal(rgba8[..., :3].reshape(-1, 3), np.floor(np.clip(ref, 0, 1) * 255 + 0.5).astype(np.uint8))
+
+
+def test_uniform_semantics(ctx):
+    """unknown names and type-mismatched setters are silent no-ops (WebGL)"""
+    s = scenes.sphere_plane(8, 8, 1)
+    sess = session(ctx, s, 8, 8)
+    assert sess.prog.index("noSuchUniform") == -1
+    assert not sess.prog.set("noSuchUniform", 1.0)
+    sess.prog.set("maxStepsIsMiss", 2.5)          # float into an int uniform: ignored
+    sess.prog.set("camFovy", np.int32(10))         # int into a float uniform: ignored
+    sess.render(1)
+    ref = session(ctx, s, 8, 8)
+    ref.render(1)
+    assert np.array_equal(sess.radiance(), ref.radiance())
+    sess.close()
+    ref.close()

@@ -74,13 +74,23 @@ def main():
               f"and is run on a single MI355X here", file=sys.stderr)
         sys.exit(2)
 
-    from snelly_amd import host, native, scenes
+    from snelly_amd import autotune, host, native, scenes
 
     W, H, spp = args.width, args.height, args.spp
     scene = scenes.menger_metal(W, H, spp)
     ctx = native.Context(0)
     sess = host.Session(scene, W, H, ctx)
+    # the plain wavefront engine (one launch sequence per bounce), for comparison
+    sess.prog.engine(native.ENGINE_WAVEFRONT)
     sess.prog.profiling(True)
+    sess.state.seed(7)
+    sess.render(spp)
+    wf = sess.prog.stats()
+    wavefront = {"path_samples_per_s": round(W * H * spp / (wf.ms_total * 1e-3), 1), "ms": round(wf.ms_total, 2),
+                 "trace_ms": round(wf.ms_trace, 2),
+                 "trace_tflops": round(wf.trace_steps * scene.flops_per_step / (wf.ms_trace * 1e-3) / 1e12, 3)}
+    # the persistent engine with an autotuned launch shape (untimed)
+    best, _, _ = autotune.tune(sess, frames=8)
     for w in range(args.warmup):
         sess.state.seed(1000 + w)
         sess.render(spp)
@@ -107,13 +117,16 @@ def main():
     samples = W * H * spp * args.steps
     value = samples / elapsed
 
+    # snl_persist is the whole frame; its launches on the tile streams overlap,
+    # so the rate is total algorithmic FLOPs over the GPU time of the renders
     flops = trace_steps * scene.flops_per_step
     achieved = flops / (trace_ms * 1e-3) / 1e12
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
-                "kernel": "snl_trace", "avg_launch_ms": round(trace_ms / launches, 4),
+                "kernel": "snl_persist", "launches": launches, "gpu_ms": round(trace_ms, 2),
                 "flops_per_launch": flops / launches, "flops_per_step": scene.flops_per_step,
-                "steps_per_launch": trace_steps / launches}
+                "steps_per_launch": trace_steps / launches,
+                "flops_counted": "SDF march steps only (shading excluded)"}
     rr = None
     if len(images) == 2:
         a, b = images
@@ -124,7 +137,9 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (analytic SDF scene, PCG-seeded per-pixel RNG)",
         "config": {"workload": "menger4_metal_spectral", "width": W, "height": H, "spp": spp, "bounces": 8,
-                   "mode": "plain", "path_samples_per_step": W * H * spp},
+                   "mode": "plain", "engine": "persistent", "launch_shape": best,
+                   "path_samples_per_step": W * H * spp},
+        "engines": {"wavefront": wavefront},
         "roofline": roofline,
         "rrmse_per_step": None if rr is None else round(rr, 5),
         "spp_for_rrmse_1pct": None if rr is None else int(math.ceil(spp * (rr / 0.01) ** 2)),

@@ -119,8 +119,18 @@ int snl_get_stats(snl_prog prog, snl_stats* out);
 /* launch-shape knobs: trace grid (blocks), jobs fetched per wave, shade grid;
  * 0 keeps the current value.  Also set by the autotuner. */
 int snl_set_tuning(snl_prog prog, int trace_blocks, int trace_chunk, int shade_blocks);
-/* 1 = time every trace launch with events (adds a sync per launch) */
+/* 1 = time every trace launch with events (wavefront engine) */
 int snl_set_profiling(snl_prog prog, int on);
+
+/* Render engines.  WAVEFRONT: per bounce, a camera / trace / material-sorted
+ * shade launch sequence over SoA path queues.  PERSISTENT: one launch per
+ * frame and pixel tile in which every lane runs whole paths from registers
+ * (no per-bounce barrier); tiles are pipelined on separate streams.  Both draw
+ * the same random numbers per path.  Programs whose surface has sub-surface
+ * scattering always render with the wavefront engine.  0 keeps a knob. */
+#define SNL_ENGINE_WAVEFRONT 0
+#define SNL_ENGINE_PERSISTENT 1
+int snl_set_engine(snl_prog prog, int engine, int tiles, int persist_blocks, int shade_thresh, int chunk);
 
 #ifdef __cplusplus
 }

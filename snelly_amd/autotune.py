@@ -1,0 +1,55 @@
+"""Autotuner for the persistent engine's launch shape.
+
+Knobs: pixel tiles pipelined on separate streams, resident blocks per launch
+(waves per CU = 4 x blocks / CUs), the shading threshold (lanes of a wave that
+must wait before the wave runs the shading code) and the number of pixels a
+wave takes from its tile's queue per atomic.  The tuner times short renders of
+the scene being rendered (the state is re-seeded afterwards by the caller) and
+keeps the fastest shape; coordinate descent from the defaults keeps the number
+of trial renders small.
+"""
+import itertools
+import time
+
+from . import native
+
+
+def time_render(sess, frames):
+    sess.ctx.sync()
+    t = time.perf_counter()
+    sess.prog.render(sess.state, frames)
+    sess.ctx.sync()
+    return time.perf_counter() - t
+
+
+def tune(sess, frames=4, cu=256, verbose=False):
+    prog = sess.prog
+    space = {
+        "tiles": [1, 2, 4],
+        "blocks": [cu * 2, cu * 3, cu * 4, cu * 6, cu * 8],
+        "thresh": [8, 16, 24, 32, 48],
+        "chunk": [32, 64, 128, 256],
+    }
+    best = {"tiles": 2, "blocks": cu * 4, "thresh": 32, "chunk": 64}
+
+    def run(cfg):
+        prog.engine(native.ENGINE_PERSISTENT, cfg["tiles"], cfg["blocks"], cfg["thresh"], cfg["chunk"])
+        return time_render(sess, frames)
+
+    run(best)  # warm the code object and caches
+    best_t = run(best)
+    trials = [(dict(best), best_t)]
+    for _ in range(2):
+        for knob, values in space.items():
+            for v in values:
+                if v == best[knob]:
+                    continue
+                cfg = dict(best, **{knob: v})
+                t = run(cfg)
+                trials.append((cfg, t))
+                if verbose:
+                    print("autotune", cfg, "%.2f ms" % (t * 1e3), flush=True)
+                if t < best_t:
+                    best, best_t = cfg, t
+    prog.engine(native.ENGINE_PERSISTENT, best["tiles"], best["blocks"], best["thresh"], best["chunk"])
+    return best, best_t, trials

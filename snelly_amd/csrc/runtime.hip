@@ -34,9 +34,13 @@
 using namespace glsl;
 
 // ---------------------------------------------------------------- objects
+#define SNL_MAX_TILES 4
 struct snl_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t tileStream[SNL_MAX_TILES] = {};  // default-priority streams for pipelined pixel tiles
+  hipEvent_t tileEvent[SNL_MAX_TILES] = {};
+  hipEvent_t joinEvent = nullptr;
   std::string err;
   int numCU = 256;
 };
@@ -57,13 +61,16 @@ struct snl_state_s {
   int* pi = nullptr;
   int* q = nullptr;    // 8 * cap ints: trace A/B, nee A/B, mat[4]
   int* cnt = nullptr;  // C_COUNT ints
+  int* tileCnt = nullptr;  // SNL_MAX_TILES * C_COUNT ints (persistent engine)
 };
 
 struct snl_prog_s {
   snl_ctx ctx;
   hipModule_t mod = nullptr;
   hipFunction_t k_reset, k_rotate, k_camera, k_trace, k_bin, k_miss, k_surf, k_metal, k_diele, k_accum, k_normals,
-      k_pick, k_tonemap;
+      k_pick, k_tonemap, k_persist, k_persist_reset;
+  int engine = SNL_ENGINE_PERSISTENT;
+  int tiles = 2, persistBlocks = 0, shadeThresh = 32, persistChunk = 64;
   bool hasSurface = false, hasMetal = false, hasDiele = false;
   int maxBounces = 3, maxMarch = 256, spf = 1;
   SnlParams hostP;
@@ -161,7 +168,7 @@ int compileProgram(snl_ctx ctx, const char* shader, const char* ior, const char*
   src << "#include \"snl_prelude.h\"\n#include \"snl_wavefront.h\"\n";
   src << (defines ? defines : "") << "\n";
   src << "namespace glsl {\n" << out.tr.code << "\n}  // namespace glsl\n";
-  src << "#line 1 \"snl_pt.h\"\n#include \"snl_pt.h\"\n#include \"snl_kernels.h\"\n";
+  src << "#line 1 \"snl_pt.h\"\n#include \"snl_pt.h\"\n#include \"snl_kernels.h\"\n#include \"snl_persist.h\"\n";
   std::string source = src.str();
 
   std::vector<std::string> opts = {"--offload-arch=gfx950", "-O3", "-std=c++17",
@@ -304,6 +311,11 @@ int snl_ctx_create(int device, snl_ctx* out) {
     return fail(nullptr, SNL_E_UNSUPPORTED, "device %d is %s, this library is built for gfx950", device, prop.gcnArchName);
   c->numCU = prop.multiProcessorCount;
   HIPCHK(nullptr, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (int k = 0; k < SNL_MAX_TILES; ++k) {
+    HIPCHK(nullptr, hipStreamCreateWithFlags(&c->tileStream[k], hipStreamNonBlocking));
+    HIPCHK(nullptr, hipEventCreateWithFlags(&c->tileEvent[k], hipEventDisableTiming));
+  }
+  HIPCHK(nullptr, hipEventCreateWithFlags(&c->joinEvent, hipEventDisableTiming));
   *out = c.release();
   return SNL_OK;
 }
@@ -312,6 +324,11 @@ int snl_ctx_destroy(snl_ctx ctx) {
   if (!ctx) return SNL_OK;
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamDestroy(ctx->stream);
+  for (int k = 0; k < SNL_MAX_TILES; ++k) {
+    if (ctx->tileStream[k]) hipStreamDestroy(ctx->tileStream[k]);
+    if (ctx->tileEvent[k]) hipEventDestroy(ctx->tileEvent[k]);
+  }
+  if (ctx->joinEvent) hipEventDestroy(ctx->joinEvent);
   delete ctx;
   return SNL_OK;
 }
@@ -346,7 +363,7 @@ int snl_program_build(snl_ctx ctx, const char* shader, const char* ior, const ch
       {&p->k_trace, "snl_trace"}, {&p->k_bin, "snl_bin"}, {&p->k_miss, "snl_shade_miss"},
       {&p->k_surf, "snl_shade_surface"}, {&p->k_metal, "snl_shade_metal"}, {&p->k_diele, "snl_shade_dielectric"},
       {&p->k_accum, "snl_accumulate"}, {&p->k_normals, "snl_normals"}, {&p->k_pick, "snl_pick"},
-      {&p->k_tonemap, "snl_tonemap"}};
+      {&p->k_tonemap, "snl_tonemap"}, {&p->k_persist, "snl_persist"}, {&p->k_persist_reset, "snl_persist_reset"}};
   for (auto& f : fns) HIPCHK(ctx, hipModuleGetFunction(f.f, p->mod, f.n));
   p->hasSurface = c.defines.count("HAS_SURFACE") > 0;
   p->hasMetal = c.defines.count("HAS_METAL") > 0;
@@ -362,6 +379,9 @@ int snl_program_build(snl_ctx ctx, const char* shader, const char* ior, const ch
   p->traceBlocks = ctx->numCU * 8;
   p->traceChunk = 64;
   p->shadeBlocks = ctx->numCU * 8;
+  int occ = 0;
+  if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, p->k_persist, 256, 0) != hipSuccess || occ < 1) occ = 2;
+  p->persistBlocks = ctx->numCU * occ;
   HIPCHK(ctx, hipEventCreate(&p->ev0));
   HIPCHK(ctx, hipEventCreate(&p->ev1));
   HIPCHK(ctx, hipEventCreate(&p->et0));
@@ -509,7 +529,8 @@ int snl_state_create(snl_ctx ctx, int w, int h, snl_state* out) {
   struct { void** p; size_t bytes; } allocs[] = {
       {(void**)&s->radiance, (size_t)n * 16}, {(void**)&s->rng, (size_t)n * 16},
       {(void**)&s->pf, (size_t)n * F_COUNT * 4}, {(void**)&s->pi, (size_t)n * I_COUNT * 4},
-      {(void**)&s->q, (size_t)n * 8 * 4}, {(void**)&s->cnt, (size_t)C_COUNT * 4}};
+      {(void**)&s->q, (size_t)n * 8 * 4}, {(void**)&s->cnt, (size_t)C_COUNT * 4},
+      {(void**)&s->tileCnt, (size_t)SNL_MAX_TILES * C_COUNT * 4}};
   for (auto& a : allocs) {
     if (hipMalloc(a.p, a.bytes) != hipSuccess) {
       for (auto& b : allocs) if (*b.p) hipFree(*b.p);
@@ -517,6 +538,7 @@ int snl_state_create(snl_ctx ctx, int w, int h, snl_state* out) {
     }
   }
   HIPCHK(ctx, hipMemsetAsync(s->cnt, 0, C_COUNT * 4, ctx->stream));
+  HIPCHK(ctx, hipMemsetAsync(s->tileCnt, 0, SNL_MAX_TILES * C_COUNT * 4, ctx->stream));
   HIPCHK(ctx, hipMemsetAsync(s->pi, 0, (size_t)n * I_COUNT * 4, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   *out = s.release();
@@ -527,7 +549,8 @@ int snl_state_destroy(snl_state s) {
   if (!s) return SNL_OK;
   hipSetDevice(s->ctx->device);
   hipStreamSynchronize(s->ctx->stream);
-  for (void* p : {(void*)s->radiance, (void*)s->rng, (void*)s->pf, (void*)s->pi, (void*)s->q, (void*)s->cnt})
+  for (void* p : {(void*)s->radiance, (void*)s->rng, (void*)s->pf, (void*)s->pi, (void*)s->q, (void*)s->cnt,
+                  (void*)s->tileCnt})
     if (p) hipFree(p);
   delete s;
   return SNL_OK;
@@ -560,8 +583,7 @@ int snl_state_write(snl_state s, int which, const float* in) {
   return SNL_OK;
 }
 
-int snl_render(snl_prog p, snl_state st, int frames) {
-  if (!p || !st || frames < 0) return fail(p ? p->ctx : nullptr, SNL_E_ARG, "bad argument");
+static int renderWavefront(snl_prog p, snl_state st, int frames) {
   snl_ctx ctx = p->ctx;
   if (st->ctx != ctx) return fail(ctx, SNL_E_ARG, "program and state belong to different contexts");
   int rc = upload(p);
@@ -636,6 +658,83 @@ int snl_render(snl_prog p, snl_state st, int frames) {
   p->stats.trace_steps = steps;
   p->stats.paths = (long long)frames * p->spf * st->npix;
   p->stats.trace_launches = traceLaunches;
+  return SNL_OK;
+}
+
+// Persistent engine: per frame and pixel tile one launch of snl_persist; the
+// tiles run on their own streams so one tile's tail overlaps another's work.
+static int renderPersistent(snl_prog p, snl_state st, int frames) {
+  snl_ctx ctx = p->ctx;
+  int rc = upload(p);
+  if (rc) return rc;
+  int K = std::max(1, std::min(p->tiles, SNL_MAX_TILES));
+  if ((long long)K > st->npix) K = 1;
+  p->stats = snl_stats{};
+  SnlKArgs base;
+  fillArgs(p, st, base);
+  base.chunk = p->persistChunk;
+  base.shadeThresh = p->shadeThresh;
+  // zero the tile counters (step counters included) on the main stream, then fork
+  SnlKArgs z = base;
+  z.vertex = -2;
+  for (int k = 0; k < K; ++k) {
+    z.cnt = st->tileCnt + k * C_COUNT;
+    if ((rc = launch(ctx, p->k_reset, 1, 64, z))) return rc;
+  }
+  HIPCHK(ctx, hipEventRecord(p->ev0, ctx->stream));
+  HIPCHK(ctx, hipEventRecord(ctx->joinEvent, ctx->stream));
+  int blocks = std::max(1, p->persistBlocks / K);
+  for (int k = 0; k < K; ++k) {
+    hipStream_t ts = ctx->tileStream[k];
+    HIPCHK(ctx, hipStreamWaitEvent(ts, ctx->joinEvent, 0));
+    SnlKArgs a = base;
+    a.cnt = st->tileCnt + k * C_COUNT;
+    a.itemBase = (int)(st->npix * k / K);
+    a.itemEnd = (int)(st->npix * (k + 1) / K);
+    void* args[] = {&a};
+    for (int f = 0; f < frames; ++f) {
+      HIPCHK(ctx, hipModuleLaunchKernel(p->k_persist_reset, 1, 1, 1, 64, 1, 1, 0, ts, args, nullptr));
+      HIPCHK(ctx, hipModuleLaunchKernel(p->k_persist, blocks, 1, 1, 256, 1, 1, 0, ts, args, nullptr));
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->tileEvent[k], ts));
+    HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->tileEvent[k], 0));
+  }
+  HIPCHK(ctx, hipEventRecord(p->ev1, ctx->stream));
+  HIPCHK(ctx, hipEventSynchronize(p->ev1));
+  float ms = 0.0f;
+  HIPCHK(ctx, hipEventElapsedTime(&ms, p->ev0, p->ev1));
+  long long steps = 0;
+  for (int k = 0; k < K; ++k) {
+    long long sk = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&sk, st->tileCnt + k * C_COUNT + C_TRACE_STEPS, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    steps += sk;
+  }
+  p->stats.ms_total = ms;
+  p->stats.ms_trace = ms;  // the persistent kernel is the whole frame
+  p->stats.trace_steps = steps;
+  p->stats.paths = (long long)frames * p->spf * st->npix;
+  p->stats.trace_launches = frames * K;
+  return SNL_OK;
+}
+
+int snl_render(snl_prog p, snl_state st, int frames) {
+  if (!p || !st || frames < 0) return fail(p ? p->ctx : nullptr, SNL_E_ARG, "bad argument");
+  if (st->ctx != p->ctx) return fail(p->ctx, SNL_E_ARG, "program and state belong to different contexts");
+  // the persistent kernel does not run the sub-surface walk: such programs use the wavefront
+  bool needWavefront = p->hostP.subsurfaceMFP > 0.0f;
+  if (p->engine == SNL_ENGINE_PERSISTENT && !needWavefront) return renderPersistent(p, st, frames);
+  return renderWavefront(p, st, frames);
+}
+
+int snl_set_engine(snl_prog p, int engine, int tiles, int persist_blocks, int shade_thresh, int chunk) {
+  if (!p) return fail(nullptr, SNL_E_ARG, "prog is NULL");
+  if (engine != SNL_ENGINE_WAVEFRONT && engine != SNL_ENGINE_PERSISTENT) return fail(p->ctx, SNL_E_ARG, "bad engine");
+  p->engine = engine;
+  if (tiles > 0) p->tiles = std::min(tiles, SNL_MAX_TILES);
+  if (persist_blocks > 0) p->persistBlocks = persist_blocks;
+  if (shade_thresh > 0) p->shadeThresh = std::min(shade_thresh, 64);
+  if (chunk > 0) p->persistChunk = chunk;
   return SNL_OK;
 }
 

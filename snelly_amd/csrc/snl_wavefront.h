@@ -45,6 +45,8 @@ struct SnlKArgs {
   int W, H, npix;
   int vertex, sample, spf;
   int chunk;          // trace refill chunk per wave
+  int itemBase, itemEnd;  // persistent kernel: pixel range of this tile
+  int shadeThresh;        // persistent kernel: lanes waiting before a wave shades
 };
 
 #define PF(k, i) a.pf[(long long)(k) * a.cap + (i)]

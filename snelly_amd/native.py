@@ -21,8 +21,9 @@ EXPORTS = [
     "snl_uniform_texture", "snl_texture_create", "snl_texture_upload", "snl_texture_destroy",
     "snl_state_create", "snl_state_destroy", "snl_state_seed", "snl_state_clear", "snl_state_read",
     "snl_state_write", "snl_render", "snl_render_normals", "snl_pick", "snl_tonemap", "snl_sync",
-    "snl_get_stats", "snl_set_tuning", "snl_set_profiling",
+    "snl_get_stats", "snl_set_tuning", "snl_set_profiling", "snl_set_engine",
 ]
+ENGINE_WAVEFRONT, ENGINE_PERSISTENT = 0, 1
 
 
 class SnellyError(RuntimeError):
@@ -73,6 +74,7 @@ def lib():
         "snl_get_stats": (ip, [vp, ctypes.POINTER(Stats)]),
         "snl_set_tuning": (ip, [vp, ip, ip, ip]),
         "snl_set_profiling": (ip, [vp, ip]),
+        "snl_set_engine": (ip, [vp, ip, ip, ip, ip, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -237,6 +239,9 @@ class Program:
 
     def tune(self, trace_blocks=0, trace_chunk=0, shade_blocks=0):
         self.ctx.check(lib().snl_set_tuning(self.h, trace_blocks, trace_chunk, shade_blocks))
+
+    def engine(self, engine, tiles=0, persist_blocks=0, shade_thresh=0, chunk=0):
+        self.ctx.check(lib().snl_set_engine(self.h, engine, tiles, persist_blocks, shade_thresh, chunk))
 
     def profiling(self, on):
         self.ctx.check(lib().snl_set_profiling(self.h, int(on)))

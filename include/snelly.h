@@ -36,6 +36,7 @@ extern "C" {
 
 /* program build flags */
 #define SNL_BUILD_PRECISE 1  /* no FMA contraction (-ffp-contract=off) */
+#define SNL_BUILD_FAST 2     /* approximate (1-ulp) float divide and sqrt outside the RNG */
 
 typedef struct snl_ctx_s* snl_ctx;
 typedef struct snl_prog_s* snl_prog;

@@ -52,8 +52,14 @@ SNL_HD float round(float x) { return snl_floorf(x + 0.5f); }
 SNL_HD float roundEven(float x) { return ::rintf(x); }
 SNL_HD float fract(float x) { return x - snl_floorf(x); }
 SNL_HD float mod(float x, float y) { return x - y * snl_floorf(x / y); }
+// GLSL leaves min/max of NaN undefined; fminf/fmaxf map to one v_min/v_max_f32
+#if defined(__HIP_DEVICE_COMPILE__)
+SNL_HD float min(float a, float b) { return __builtin_fminf(a, b); }
+SNL_HD float max(float a, float b) { return __builtin_fmaxf(a, b); }
+#else
 SNL_HD float min(float a, float b) { return b < a ? b : a; }
 SNL_HD float max(float a, float b) { return a < b ? b : a; }
+#endif
 SNL_HD int min(int a, int b) { return b < a ? b : a; }
 SNL_HD int max(int a, int b) { return a < b ? b : a; }
 SNL_HD unsigned min(unsigned a, unsigned b) { return b < a ? b : a; }

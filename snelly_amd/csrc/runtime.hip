@@ -170,6 +170,10 @@ int compileProgram(snl_ctx ctx, const char* shader, const char* ior, const char*
   src << "namespace glsl {\n" << out.tr.code << "\n}  // namespace glsl\n";
   src << "#line 1 \"snl_pt.h\"\n#include \"snl_pt.h\"\n#include \"snl_kernels.h\"\n#include \"snl_persist.h\"\n";
   std::string source = src.str();
+  if (const char* dump = getenv("SNL_DUMP_SOURCE")) {  // for offline ISA inspection
+    std::ofstream f(dump);
+    f << source;
+  }
 
   std::vector<std::string> opts = {"--offload-arch=gfx950", "-O3", "-std=c++17",
                                    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -178,6 +182,8 @@ int compileProgram(snl_ctx ctx, const char* shader, const char* ior, const char*
                                    "-DSNL_MAX_ATMOSPHERE_SCATTERS=" + std::to_string(maxAtm),
                                    "-DSNL_MAX_SAMPLES_PER_FRAME=" + std::to_string(spf)};
   if (flags & SNL_BUILD_PRECISE) opts.push_back("-ffp-contract=off");
+  if ((flags & SNL_BUILD_FAST) && !(flags & SNL_BUILD_PRECISE))
+    opts.push_back("-fno-hip-fp32-correctly-rounded-divide-sqrt");
   std::string key = source;
   for (auto& o : opts) key += "\n" + o;
   for (int i = 0; i < kNumEmbedded; ++i) key += kEmbeddedSources[i];

@@ -26,10 +26,22 @@ SNL_HD float snl_rand(vec4& rnd) {
   const vec4 r(1112.0f, 367.0f, 92.0f, 265.0f);
   const vec4 a(3423.0f, 2646.0f, 1707.0f, 1999.0f);
   const vec4 m(4194287.0f, 4194277.0f, 4194191.0f, 4194167.0f);
-  vec4 beta = floor(rnd / q);
+  // the divisions are correctly rounded in every build mode (floor(rnd/q) must
+  // be exact), hence the explicit round-to-nearest intrinsic
+#if defined(__HIP_DEVICE_COMPILE__)
+  vec4 quo(__fdiv_rn(rnd.x, q.x), __fdiv_rn(rnd.y, q.y), __fdiv_rn(rnd.z, q.z), __fdiv_rn(rnd.w, q.w));
+#else
+  vec4 quo = rnd / q;
+#endif
+  vec4 beta = floor(quo);
   vec4 p = a * (rnd - beta * q) - beta * r;
   beta = (1.0f - sign(p)) * 0.5f * m;
   rnd = p + beta;
-  return fract(dot(rnd / m, vec4(1.0f, -1.0f, 1.0f, -1.0f)));
+#if defined(__HIP_DEVICE_COMPILE__)
+  vec4 u(__fdiv_rn(rnd.x, m.x), __fdiv_rn(rnd.y, m.y), __fdiv_rn(rnd.z, m.z), __fdiv_rn(rnd.w, m.w));
+#else
+  vec4 u = rnd / m;
+#endif
+  return fract(dot(u, vec4(1.0f, -1.0f, 1.0f, -1.0f)));
 }
 }  // namespace glsl

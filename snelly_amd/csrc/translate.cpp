@@ -8,7 +8,7 @@
 // `mat3 rot;` that INIT() fills in exampleScenes/fractals/menger.html) are
 // per-invocation as in a fragment shader.  Uniform declarations are lifted into
 // a reflected block (struct SnlUser) read through a const pointer; references
-// to them and to the built-in uniforms become __U->name / __P->name.
+// to them and to the built-in uniforms become snlU_->name / snlP_->name.
 //
 // The rewrite is token-level: qualifiers in/out/inout become by-value/by-ref
 // parameters, float literals get an f suffix (GLSL float literals are single
@@ -265,8 +265,8 @@ class Translator {
     std::ostringstream o;
     o << blk.str();
     o << "struct SnlScene {\n";
-    o << "  const SnlParams* __restrict__ __P;\n  const SnlUser* __restrict__ __U;\n";
-    o << "  SNL_HD SnlScene(const SnlParams* p, const SnlUser* u) : __P(p), __U(u) {}\n";
+    o << "  const SnlParams* __restrict__ snlP_;\n  const SnlUser* __restrict__ snlU_;\n";
+    o << "  SNL_HD SnlScene(const SnlParams* p, const SnlUser* u) : snlP_(p), snlU_(u) {}\n";
     auto flag = [&](const char* name, const char* def) {
       o << "  static constexpr bool " << name << " = " << (defines_.count(def) ? "true" : "false") << ";\n";
     };
@@ -332,8 +332,8 @@ class Translator {
         bool member = false;
         while (p > b) { --p; if (toks_[p].k == T_WS) continue; member = (toks_[p].k == T_PUNCT && (toks_[p].s == "." || toks_[p].s == "->")); break; }
         if (!member) {
-          if (uniformNames_.count(t.s)) { o << "__U->" << t.s; continue; }
-          if (builtinUniforms_.count(t.s)) { o << "__P->" << t.s; continue; }
+          if (uniformNames_.count(t.s)) { o << "snlU_->" << t.s; continue; }
+          if (builtinUniforms_.count(t.s)) { o << "snlP_->" << t.s; continue; }
           if (t.s == "not") { o << "not_"; continue; }
           if (t.s == "uint") { o << "unsigned"; continue; }
           if (renamed().count(t.s)) { o << t.s << "_"; continue; }

@@ -37,6 +37,19 @@ def rrmse(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
+def assert_same_estimator(g, r, g_other_seed, max_paired_fraction=0.5):
+    """g and r render the same pixels from the same RNG states; where a rounding
+    difference sends a path down another branch the two samples are
+    independent.  Require (1) no systematic difference: every channel's mean
+    difference within 4 standard errors, and (2) the paired difference well
+    below the difference between two independent-seed renders."""
+    d = g - r
+    n = d.shape[0]
+    se = d.std(axis=0) / math.sqrt(n)
+    assert np.all(np.abs(d.mean(axis=0)) <= 4.0 * se + 1e-6 * np.abs(r.mean(axis=0)))
+    assert rrmse(g, r) < max_paired_fraction * rrmse(g_other_seed, r)
+
+
 def pixel_grid(w, h):
     return np.stack(np.meshgrid(np.arange(w) + 0.5, np.arange(h) + 0.5), -1).reshape(-1, 2).astype(np.float32)
 
@@ -138,6 +151,28 @@ def test_pathtrace_fast_build_statistics(ctx, engine):
     g, r, close, _, _ = compare_render(ctx, s, 64, 64, 16, flags=0, engine=engine)
     assert close >= 0.9
     assert rrmse(unbiased(g), unbiased(r)) < 1e-2
+
+
+@pytest.mark.parametrize("engine", ENGINES, ids=ENGINE_IDS)
+def test_pathtrace_fast_math_build_statistics(ctx, engine):
+    """SNL_BUILD_FAST (1-ulp divide and sqrt outside the RNG): same 1% bound"""
+    s = scenes.menger_metal(64, 64, 16)
+    g, r, close, _, _ = compare_render(ctx, s, 64, 64, 16, flags=native.BUILD_FAST, engine=engine)
+    sess = session(ctx, s, 64, 64, native.BUILD_FAST, engine)
+    sess.state.seed(4321)
+    sess.render(16)
+    g2 = sess.radiance().reshape(-1, 4)
+    sess.close()
+    assert close >= 0.5
+    assert_same_estimator(unbiased(g), unbiased(r), unbiased(g2))
+
+
+def test_fast_math_keeps_rng_exact(ctx):
+    """the RNG's divisions stay correctly rounded in the fast build: furnace
+    paths draw a fixed number of random numbers, so the states match exactly"""
+    s = scenes.furnace(32, 32, 16)
+    _, _, _, rng_same, _ = compare_render(ctx, s, 32, 32, 16, flags=native.BUILD_FAST)
+    assert rng_same == 1.0
 
 
 @pytest.mark.parametrize("engine", ENGINES, ids=ENGINE_IDS)

@@ -70,8 +70,26 @@ def build_runtime():
     os.replace(tmp, LIB)
 
 
+NODE_INCLUDE = "/usr/include/node"
+ADDON = os.path.join(HERE, "snelly_napi.node")
+
+
+def build_addon():
+    """The Node-API addon (snelly_napi.node) over libsnelly_hip.so, for the
+    JavaScript host in js/.  Skipped when Node's headers are not installed."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        print("build: node_api.h not found, skipping the Node addon")
+        return
+    tmp = ADDON + ".tmp"
+    run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-DNODE_GYP_MODULE_NAME=snelly_napi", "-I", NODE_INCLUDE,
+         os.path.join(HERE, "napi", "snelly_napi.cpp"), "-o", tmp, "-L", HERE, "-lsnelly_hip",
+         "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, ADDON)
+
+
 def main():
     build_runtime()
+    build_addon()
 
 
 if __name__ == "__main__":

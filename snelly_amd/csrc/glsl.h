@@ -154,6 +154,42 @@ struct sw4 {
 
 struct mat2; struct mat3; struct mat4;
 
+// GLSL constructors may be given more components than the vector holds as
+// long as the last argument is (partly) used -- vec3(a, b, v2) takes v2.x.
+// Such calls land on a variadic constructor that flattens its arguments.
+template <class T> struct snl_ncomp { static constexpr int n = 1; };
+template <> struct snl_ncomp<vec2> { static constexpr int n = 2; };
+template <> struct snl_ncomp<vec3> { static constexpr int n = 3; };
+template <> struct snl_ncomp<vec4> { static constexpr int n = 4; };
+template <> struct snl_ncomp<ivec2> { static constexpr int n = 2; };
+template <> struct snl_ncomp<ivec3> { static constexpr int n = 3; };
+template <> struct snl_ncomp<ivec4> { static constexpr int n = 4; };
+template <int N, class V, int A, int B> struct snl_ncomp<sw2<N, V, A, B>> { static constexpr int n = 2; };
+template <int N, class V, int A, int B, int C> struct snl_ncomp<sw3<N, V, A, B, C>> { static constexpr int n = 3; };
+template <int N, class V, int A, int B, int C, int D> struct snl_ncomp<sw4<N, V, A, B, C, D>> { static constexpr int n = 4; };
+template <class... A> struct snl_sumc { static constexpr int n = 0; };
+template <class H, class... R> struct snl_sumc<H, R...> { static constexpr int n = snl_ncomp<H>::n + snl_sumc<R...>::n; };
+template <bool B> struct snl_if {};
+template <> struct snl_if<true> { typedef int type; };
+template <class T> SNL_HD void snl_put(float* t, int& k, const T& v) { t[k++] = float(v); }
+// an argument that may stand for an out/inout parameter of vector type V:
+// V itself or a swizzle of a vector that converts to V (GLSL copies in and out)
+template <class A, class V> struct snl_lv { static constexpr bool ok = false; };
+template <class V> struct snl_lv<V, V> { static constexpr bool ok = true; };
+template <int N, class V, int A, int B> struct snl_lv<sw2<N, V, A, B>, V> { static constexpr bool ok = true; };
+template <int N, class V, int A, int B, int C> struct snl_lv<sw3<N, V, A, B, C>, V> { static constexpr bool ok = true; };
+template <int N, class V, int A, int B, int C, int D> struct snl_lv<sw4<N, V, A, B, C, D>, V> { static constexpr bool ok = true; };
+#define SNL_VARIADIC_CTOR(T, K)                                                              \
+  template <class A0, class A1, class... R, typename snl_if<(snl_sumc<A0, A1, R...>::n >= K)>::type = 0> \
+  SNL_HD T(const A0& a0, const A1& a1, const R&... r) {                                      \
+    float t[snl_sumc<A0, A1, R...>::n];                                                      \
+    int k = 0;                                                                               \
+    snl_put(t, k, a0);                                                                       \
+    snl_put(t, k, a1);                                                                       \
+    (snl_put(t, k, r), ...);                                                                 \
+    for (int i = 0; i < K; ++i) e[i] = t[i];                                                 \
+  }
+
 // Component-wise arithmetic for a float vector type T with K components,
 // written once and stamped out for vec2/vec3/vec4.  Declared as friends so
 // that argument-dependent lookup finds them for swizzle proxies too.
@@ -197,9 +233,11 @@ struct vec2 {
   SNL_HD explicit vec2(float v) : x(v), y(v) {}
   SNL_HD vec2(float a, float b) : x(a), y(b) {}
   SNL_HD vec2(const vec2& o) : x(o.x), y(o.y) {}
+  SNL_VARIADIC_CTOR(vec2, 2)
   SNL_HD explicit vec2(const vec3& o);
   SNL_HD explicit vec2(const vec4& o);
   SNL_HD explicit vec2(const ivec2& o);
+  SNL_HD explicit vec2(const bvec2& o);
   SNL_HD vec2& operator=(const vec2& o) { x = o.x; y = o.y; return *this; }
   SNL_VEC_OPS(vec2, 2)
   SNL_HD vec2& operator*=(const mat2& m);
@@ -218,8 +256,10 @@ struct vec3 {
   SNL_HD vec3(const vec2& a, float c) : x(a.x), y(a.y), z(c) {}
   SNL_HD vec3(float a, const vec2& b) : x(a), y(b.x), z(b.y) {}
   SNL_HD vec3(const vec3& o) : x(o.x), y(o.y), z(o.z) {}
+  SNL_VARIADIC_CTOR(vec3, 3)
   SNL_HD explicit vec3(const vec4& o);
   SNL_HD explicit vec3(const ivec3& o);
+  SNL_HD explicit vec3(const bvec3& o);
   SNL_HD vec3& operator=(const vec3& o) { x = o.x; y = o.y; z = o.z; return *this; }
   SNL_VEC_OPS(vec3, 3)
   SNL_HD vec3& operator*=(const mat3& m);
@@ -242,7 +282,9 @@ struct vec4 {
   SNL_HD vec4(float a, float b, const vec2& v) : x(a), y(b), z(v.x), w(v.y) {}
   SNL_HD vec4(const vec2& u, const vec2& v) : x(u.x), y(u.y), z(v.x), w(v.y) {}
   SNL_HD vec4(const vec4& o) : x(o.x), y(o.y), z(o.z), w(o.w) {}
+  SNL_VARIADIC_CTOR(vec4, 4)
   SNL_HD explicit vec4(const ivec4& o);
+  SNL_HD explicit vec4(const bvec4& o);
   SNL_HD vec4& operator=(const vec4& o) { x = o.x; y = o.y; z = o.z; w = o.w; return *this; }
   SNL_VEC_OPS(vec4, 4)
   SNL_HD vec4& operator*=(const mat4& m);
@@ -251,6 +293,27 @@ struct vec4 {
 SNL_HD vec2::vec2(const vec3& o) : x(o.x), y(o.y) {}
 SNL_HD vec2::vec2(const vec4& o) : x(o.x), y(o.y) {}
 SNL_HD vec3::vec3(const vec4& o) : x(o.x), y(o.y), z(o.z) {}
+
+SNL_HD void snl_put(float* t, int& k, const vec2& v) { t[k++] = v.x; t[k++] = v.y; }
+SNL_HD void snl_put(float* t, int& k, const vec3& v) { t[k++] = v.x; t[k++] = v.y; t[k++] = v.z; }
+SNL_HD void snl_put(float* t, int& k, const vec4& v) { t[k++] = v.x; t[k++] = v.y; t[k++] = v.z; t[k++] = v.w; }
+template <int N, class V, int A, int B>
+SNL_HD void snl_put(float* t, int& k, const sw2<N, V, A, B>& v) { t[k++] = v.e[A]; t[k++] = v.e[B]; }
+template <int N, class V, int A, int B, int C>
+SNL_HD void snl_put(float* t, int& k, const sw3<N, V, A, B, C>& v) { t[k++] = v.e[A]; t[k++] = v.e[B]; t[k++] = v.e[C]; }
+template <int N, class V, int A, int B, int C, int D>
+SNL_HD void snl_put(float* t, int& k, const sw4<N, V, A, B, C, D>& v) {
+  t[k++] = v.e[A]; t[k++] = v.e[B]; t[k++] = v.e[C]; t[k++] = v.e[D];
+}
+
+// GLSL float(x): a scalar conversion, or the first component of a vector
+template <class T> SNL_HD float snl_float(const T& v) { return float(v); }
+SNL_HD float snl_float(const vec2& v) { return v.x; }
+SNL_HD float snl_float(const vec3& v) { return v.x; }
+SNL_HD float snl_float(const vec4& v) { return v.x; }
+template <int N, class V, int A, int B> SNL_HD float snl_float(const sw2<N, V, A, B>& v) { return v.e[A]; }
+template <int N, class V, int A, int B, int C> SNL_HD float snl_float(const sw3<N, V, A, B, C>& v) { return v.e[A]; }
+template <int N, class V, int A, int B, int C, int D> SNL_HD float snl_float(const sw4<N, V, A, B, C, D>& v) { return v.e[A]; }
 
 // swizzle proxy conversions (need the complete vector types)
 template <class V> struct vmake;
@@ -329,6 +392,14 @@ SNL_HD vec4::vec4(const ivec4& o) : x(float(o.x)), y(float(o.y)), z(float(o.z)),
 struct bvec2 { bool x, y; SNL_HD bvec2() : x(false), y(false) {} SNL_HD bvec2(bool a, bool b) : x(a), y(b) {} SNL_HD explicit bvec2(bool a) : x(a), y(a) {} };
 struct bvec3 { bool x, y, z; SNL_HD bvec3() : x(false), y(false), z(false) {} SNL_HD bvec3(bool a, bool b, bool c) : x(a), y(b), z(c) {} SNL_HD explicit bvec3(bool a) : x(a), y(a), z(a) {} };
 struct bvec4 { bool x, y, z, w; SNL_HD bvec4() : x(false), y(false), z(false), w(false) {} SNL_HD bvec4(bool a, bool b, bool c, bool d) : x(a), y(b), z(c), w(d) {} SNL_HD explicit bvec4(bool a) : x(a), y(a), z(a), w(a) {} };
+SNL_HD vec2::vec2(const bvec2& o) : x(o.x ? 1.0f : 0.0f), y(o.y ? 1.0f : 0.0f) {}
+SNL_HD vec3::vec3(const bvec3& o) : x(o.x ? 1.0f : 0.0f), y(o.y ? 1.0f : 0.0f), z(o.z ? 1.0f : 0.0f) {}
+SNL_HD vec4::vec4(const bvec4& o) : x(o.x ? 1.0f : 0.0f), y(o.y ? 1.0f : 0.0f), z(o.z ? 1.0f : 0.0f), w(o.w ? 1.0f : 0.0f) {}
+SNL_HD void snl_put(float* t, int& k, const ivec2& v) { t[k++] = float(v.x); t[k++] = float(v.y); }
+SNL_HD void snl_put(float* t, int& k, const ivec3& v) { t[k++] = float(v.x); t[k++] = float(v.y); t[k++] = float(v.z); }
+SNL_HD void snl_put(float* t, int& k, const ivec4& v) {
+  t[k++] = float(v.x); t[k++] = float(v.y); t[k++] = float(v.z); t[k++] = float(v.w);
+}
 SNL_HD bool any(bvec2 b) { return b.x || b.y; }
 SNL_HD bool any(bvec3 b) { return b.x || b.y || b.z; }
 SNL_HD bool any(bvec4 b) { return b.x || b.y || b.z || b.w; }
@@ -417,6 +488,8 @@ struct mat2 {
   SNL_HD explicit mat2(float d) { c[0] = vec2(d, 0.0f); c[1] = vec2(0.0f, d); }
   SNL_HD mat2(float a, float b, float cc, float d) { c[0] = vec2(a, b); c[1] = vec2(cc, d); }
   SNL_HD mat2(const vec2& a, const vec2& b) { c[0] = a; c[1] = b; }
+  SNL_HD explicit mat2(const vec4& v) { c[0] = vec2(v.x, v.y); c[1] = vec2(v.z, v.w); }
+  SNL_HD explicit mat2(const mat3& m);
   SNL_HD vec2& operator[](int i) { return c[i]; }
   SNL_HD const vec2& operator[](int i) const { return c[i]; }
 };
@@ -444,6 +517,7 @@ struct mat4 {
   SNL_HD vec4& operator[](int i) { return c[i]; }
   SNL_HD const vec4& operator[](int i) const { return c[i]; }
 };
+SNL_HD mat2::mat2(const mat3& m) { c[0] = vec2(m.c[0]); c[1] = vec2(m.c[1]); }
 SNL_HD mat3::mat3(const mat4& m) { c[0] = vec3(m.c[0]); c[1] = vec3(m.c[1]); c[2] = vec3(m.c[2]); }
 
 #define SNL_MAT_OPS(M, V, K)                                                                \
@@ -454,6 +528,14 @@ SNL_HD mat3::mat3(const mat4& m) { c[0] = vec3(m.c[0]); c[1] = vec3(m.c[1]); c[2
   SNL_HD M operator*(float s, const M& a) { M r; for (int j = 0; j < K; ++j) r.c[j] = s * a.c[j]; return r; } \
   SNL_HD M operator+(const M& a, const M& b) { M r; for (int j = 0; j < K; ++j) r.c[j] = a.c[j] + b.c[j]; return r; } \
   SNL_HD M operator-(const M& a, const M& b) { M r; for (int j = 0; j < K; ++j) r.c[j] = a.c[j] - b.c[j]; return r; } \
+  SNL_HD M operator-(const M& a) { M r; for (int j = 0; j < K; ++j) r.c[j] = -a.c[j]; return r; } \
+  SNL_HD M operator/(const M& a, float s) { M r; for (int j = 0; j < K; ++j) r.c[j] = a.c[j] / s; return r; } \
+  SNL_HD M operator+(const M& a, float s) { M r; for (int j = 0; j < K; ++j) r.c[j] = a.c[j] + s; return r; } \
+  SNL_HD M operator-(const M& a, float s) { M r; for (int j = 0; j < K; ++j) r.c[j] = a.c[j] - s; return r; } \
+  SNL_HD M& operator*=(M& a, float s) { for (int j = 0; j < K; ++j) a.c[j] *= s; return a; } \
+  SNL_HD M& operator/=(M& a, float s) { for (int j = 0; j < K; ++j) a.c[j] /= s; return a; } \
+  SNL_HD M& operator+=(M& a, const M& b) { for (int j = 0; j < K; ++j) a.c[j] += b.c[j]; return a; } \
+  SNL_HD M& operator-=(M& a, const M& b) { for (int j = 0; j < K; ++j) a.c[j] -= b.c[j]; return a; } \
   SNL_HD M transpose(const M& a) { M r; for (int i = 0; i < K; ++i) for (int j = 0; j < K; ++j) r.c[i].e[j] = a.c[j].e[i]; return r; } \
   SNL_HD M matrixCompMult(const M& a, const M& b) { M r; for (int j = 0; j < K; ++j) r.c[j] = a.c[j] * b.c[j]; return r; }
 SNL_MAT_OPS(mat2, vec2, 2) SNL_MAT_OPS(mat3, vec3, 3) SNL_MAT_OPS(mat4, vec4, 4)

@@ -25,9 +25,9 @@ __device__ inline vec4 ld4(const float* p, long long i) {
 __device__ inline void st4(float* p, long long i, const vec4& v) {
   reinterpret_cast<float4*>(p)[i] = make_float4(v.x, v.y, v.z, v.w);
 }
-__device__ inline vec4 ldRng(const SnlKArgs& a, int i) { return vec4(PF(F_RNG, i), PF(F_RNG + 1, i), PF(F_RNG + 2, i), PF(F_RNG + 3, i)); }
+__device__ inline vec4 ldRng(const SnlKArgs& a, int i) { return vec4(SNL_PF(F_RNG, i), SNL_PF(F_RNG + 1, i), SNL_PF(F_RNG + 2, i), SNL_PF(F_RNG + 3, i)); }
 __device__ inline void stRng(const SnlKArgs& a, int i, const vec4& r) {
-  PF(F_RNG, i) = r.x; PF(F_RNG + 1, i) = r.y; PF(F_RNG + 2, i) = r.z; PF(F_RNG + 3, i) = r.w;
+  SNL_PF(F_RNG, i) = r.x; SNL_PF(F_RNG + 1, i) = r.y; SNL_PF(F_RNG + 2, i) = r.z; SNL_PF(F_RNG + 3, i) = r.w;
 }
 
 // ---------------------------------------------------------------- counters
@@ -75,14 +75,14 @@ __device__ inline void camera_body(const SnlKArgs& a) {
         float wavelength_nm = 390.0f + (750.0f - 390.0f) * 0.5f;
         vec3 xyz(0.0f), rgb(0.0f);
 #endif
-        PF(F_WL, i) = wavelength_nm;
+        SNL_PF(F_WL, i) = wavelength_nm;
         st3(a, F_XYZ, i, xyz);
         st3(a, F_RGB, i, rgb);
         stRT(a, F_LS, i, rt1(0.0f));
       }
     } else {
       rnd = ldRng(a, i);
-      flags = PI(I_FLAGS, i) & PF_SKIP;
+      flags = SNL_PI(I_FLAGS, i) & PF_SKIP;
     }
     if (!(flags & PF_SKIP)) {
       vec2 pixel(float(i % a.W) + 0.5f, float(i / a.W) + 0.5f);
@@ -98,13 +98,13 @@ __device__ inline void camera_body(const SnlKArgs& a) {
       st3(a, F_DX, i, primaryDir);
       stRT(a, F_TP, i, rt1(1.0f));
       stRT(a, F_L, i, rt1(0.0f));
-      PF(F_MIS, i) = 1.0f; PF(F_MIS + 1, i) = 1.0f; PF(F_MIS + 2, i) = 1.0f;
+      SNL_PF(F_MIS, i) = 1.0f; SNL_PF(F_MIS + 1, i) = 1.0f; SNL_PF(F_MIS + 2, i) = 1.0f;
       flags |= inDiele ? PF_INDIELE : 0;   // vertex 0, no atmosphere scatters
       alive = true;
     }
     stRng(a, i, rnd);
-    PI(I_FLAGS, i) = flags;
-    PI(I_NEE, i) = 0;
+    SNL_PI(I_FLAGS, i) = flags;
+    SNL_PI(I_NEE, i) = 0;
   }
   int slot = snl_append(&a.cnt[C_TRACE_OUT], alive);
   if (alive) a.qtrace_out[slot] = i;
@@ -164,11 +164,11 @@ __device__ inline void trace_body(const SnlKArgs& a) {
         } else {
           path = a.qnee_in[job - nT];
           nee = true;
-          mask = PI(I_NEE, path);
+          mask = SNL_PI(I_NEE, path);
           light = __ffs(mask) - 1;
           o = ld3(a, F_OX, path);
           d = ld3(a, F_NEE + light * F_NEE_STRIDE, path);
-          tmax = PF(F_NEE + light * F_NEE_STRIDE + 3, path);
+          tmax = SNL_PF(F_NEE + light * F_NEE_STRIDE + 3, path);
           acc = rt1(0.0f);
         }
         init = true;
@@ -202,7 +202,7 @@ __device__ inline void trace_body(const SnlKArgs& a) {
       if (done) {
         if (!nee) {
           // continuation ray: record the hit for the shade stage
-          PI(I_HMAT, path) = hit ? m : MAT_INVAL;
+          SNL_PI(I_HMAT, path) = hit ? m : MAT_INVAL;
           if (hit) st3(a, F_HX, path, p);
           path = -1;
         } else {
@@ -217,13 +217,13 @@ __device__ inline void trace_body(const SnlKArgs& a) {
           Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
           if (averageComponent(Li) > RADIANCE_EPSILON) {
             RT f = ldRT(a, b + 4, path);
-            acc += f * Li / PF(b + 10, path) * PF(b + 11, path) * PF(b + 12, path);
+            acc += f * Li / SNL_PF(b + 10, path) * SNL_PF(b + 11, path) * SNL_PF(b + 12, path);
           }
           mask &= ~(1 << light);
           if (mask) {
             light = __ffs(mask) - 1;
             d = ld3(a, F_NEE + light * F_NEE_STRIDE, path);
-            tmax = PF(F_NEE + light * F_NEE_STRIDE + 3, path);
+            tmax = SNL_PF(F_NEE + light * F_NEE_STRIDE + 3, path);
             init = true;
             steps = 0;
           } else {
@@ -231,7 +231,7 @@ __device__ inline void trace_body(const SnlKArgs& a) {
             RT L = ldRT(a, F_L, path);
             L += ldRT(a, F_NTP, path) * Ldirect;
             stRT(a, F_L, path, L);
-            PI(I_NEE, path) = 0;
+            SNL_PI(I_NEE, path) = 0;
             path = -1;
           }
         }
@@ -253,7 +253,7 @@ extern "C" __global__ void __launch_bounds__(256) snl_bin(SnlKArgs a) {
     int j = base + threadIdx.x;
     bool act = j < n;
     int path = act ? a.qtrace_in[j] : 0;
-    int m = act ? PI(I_HMAT, path) : -3;
+    int m = act ? SNL_PI(I_HMAT, path) : -3;
     int q = m == MAT_SURFA ? 1 : (m == MAT_METAL ? 2 : (m == MAT_DIELE ? 3 : 0));
     for (int k = 0; k < 4; ++k) {
       bool mine = act && q == k;
@@ -273,10 +273,10 @@ __device__ inline void shade_miss_body(const SnlKArgs& a) {
     int j = base + threadIdx.x;
     if (j >= n) continue;
     int i = a.qmat[j];
-    if (PI(I_HMAT, i) != MAT_INVAL) continue;  // a max-step "hit" with no material: the path ends
+    if (SNL_PI(I_HMAT, i) != MAT_INVAL) continue;  // a max-step "hit" with no material: the path ends
     vec3 pW = ld3(a, F_OX, i), rayDir = ld3(a, F_DX, i), rgb = ld3(a, F_RGB, i);
     RT throughput = ldRT(a, F_TP, i), L = ldRT(a, F_L, i);
-    int flags = PI(I_FLAGS, i);
+    int flags = SNL_PI(I_FLAGS, i);
     int vertex = (flags >> PF_VSHIFT) & 0xff;
     bool lightHit = false;
 #ifdef HAS_SPHERE_LIGHT
@@ -288,7 +288,7 @@ __device__ inline void shade_miss_body(const SnlKArgs& a) {
         float lightSegmentLength = length(pLightHit - pW);
         RT Tr = pt.transmittanceOverFreeSegment(pW, rayDir, lightSegmentLength, rgb);
         float scatter_prob = pt.atmosphericScatteringProbability(Tr, rgb);
-        L += throughput * Tr * PF(F_MIS + 2, i) * LiSph / (1.0f - scatter_prob);
+        L += throughput * Tr * SNL_PF(F_MIS + 2, i) * LiSph / (1.0f - scatter_prob);
       }
     }
 #endif
@@ -296,7 +296,7 @@ __device__ inline void shade_miss_body(const SnlKArgs& a) {
       RT Tr = pt.transmittanceOverFreeSegment(pW, rayDir, P.maxLengthScale, rgb);
       float scatter_prob = pt.atmosphericScatteringProbability(Tr, rgb);
       if (averageComponent(Tr) > RADIANCE_EPSILON) {
-        float misSky = PF(F_MIS, i), misSun = PF(F_MIS + 1, i);
+        float misSky = SNL_PF(F_MIS, i), misSun = SNL_PF(F_MIS + 1, i);
         if (!(vertex == 0 && !P.envMapVisible) && misSky > 0.0f)
           L += throughput * Tr * misSky * pt.environmentRadiance(rayDir, rgb) / (1.0f - scatter_prob);
         if (!(vertex == 0 && !P.sunVisibleDirectly) && misSun > 0.0f)
@@ -325,10 +325,10 @@ __device__ inline void shade_hit(const SnlKArgs& a) {
       i = a.qmat[(long long)qoff * a.cap + j];
       vec3 pW = ld3(a, F_OX, i), rayDir = ld3(a, F_DX, i), pW_next = ld3(a, F_HX, i);
       vec3 rgb = ld3(a, F_RGB, i);
-      float wavelength_nm = PF(F_WL, i);
+      float wavelength_nm = SNL_PF(F_WL, i);
       RT throughput = ldRT(a, F_TP, i), L = ldRT(a, F_L, i);
-      float misWeightSky = PF(F_MIS, i), misWeightSun = PF(F_MIS + 1, i), misWeightSph = PF(F_MIS + 2, i);
-      int flags = PI(I_FLAGS, i);
+      float misWeightSky = SNL_PF(F_MIS, i), misWeightSun = SNL_PF(F_MIS + 1, i), misWeightSph = SNL_PF(F_MIS + 2, i);
+      int flags = SNL_PI(I_FLAGS, i);
       int vertex = (flags >> PF_VSHIFT) & 0xff;
       bool inDielectric = (flags & PF_INDIELE) != 0;
       vec4 rnd = ldRng(a, i);
@@ -406,10 +406,10 @@ __device__ inline void shade_hit(const SnlKArgs& a) {
               float lp = max(PDF_EPSILON, pt.pdfBsdf(pW, basis, winputL, wL, hitMaterial, wavelength_nm, rgb, true));
               RT fl = pt.evaluateBsdf(pW, basis, winputL, wL, hitMaterial, wavelength_nm, rgb, true);
               const int b = F_NEE + 0 * F_NEE_STRIDE;
-              st3(a, b, i, wW); PF(b + 3, i) = P.maxLengthScale;
+              st3(a, b, i, wW); SNL_PF(b + 3, i) = P.maxLengthScale;
               stRT(a, b + 4, i, fl); stRT(a, b + 7, i, Li);
-              PF(b + 10, i) = max(PDF_EPSILON, skyPdf); PF(b + 11, i) = abs(dot(wW, basis.nW));
-              PF(b + 12, i) = powerHeuristic(skyPdf, lp);
+              SNL_PF(b + 10, i) = max(PDF_EPSILON, skyPdf); SNL_PF(b + 11, i) = abs(dot(wW, basis.nW));
+              SNL_PF(b + 12, i) = powerHeuristic(skyPdf, lp);
               nmask |= 1;
             }
           }
@@ -419,10 +419,10 @@ __device__ inline void shade_hit(const SnlKArgs& a) {
               float lp = max(PDF_EPSILON, pt.pdfBsdf(pW, basis, winputL, wL, hitMaterial, wavelength_nm, rgb, true));
               RT fl = pt.evaluateBsdf(pW, basis, winputL, wL, hitMaterial, wavelength_nm, rgb, true);
               const int b = F_NEE + 1 * F_NEE_STRIDE;
-              st3(a, b, i, wW); PF(b + 3, i) = P.maxLengthScale;
+              st3(a, b, i, wW); SNL_PF(b + 3, i) = P.maxLengthScale;
               stRT(a, b + 4, i, fl); stRT(a, b + 7, i, Li);
-              PF(b + 10, i) = max(PDF_EPSILON, sunPdf); PF(b + 11, i) = abs(dot(wW, basis.nW));
-              PF(b + 12, i) = powerHeuristic(sunPdf, lp);
+              SNL_PF(b + 10, i) = max(PDF_EPSILON, sunPdf); SNL_PF(b + 11, i) = abs(dot(wW, basis.nW));
+              SNL_PF(b + 12, i) = powerHeuristic(sunPdf, lp);
               nmask |= 2;
             }
           }
@@ -437,16 +437,16 @@ __device__ inline void shade_hit(const SnlKArgs& a) {
               float lp = max(PDF_EPSILON, pt.pdfBsdf(pW, basis, winputL, wL, hitMaterial, wavelength_nm, rgb, true));
               RT fl = pt.evaluateBsdf(pW, basis, winputL, wL, hitMaterial, wavelength_nm, rgb, true);
               const int b = F_NEE + 2 * F_NEE_STRIDE;
-              st3(a, b, i, wW); PF(b + 3, i) = hitDist;
+              st3(a, b, i, wW); SNL_PF(b + 3, i) = hitDist;
               stRT(a, b + 4, i, fl); stRT(a, b + 7, i, Li);
-              PF(b + 10, i) = max(PDF_EPSILON, sphPdf); PF(b + 11, i) = abs(dot(wW, basis.nW));
-              PF(b + 12, i) = powerHeuristic(sphPdf, lp);
+              SNL_PF(b + 10, i) = max(PDF_EPSILON, sphPdf); SNL_PF(b + 11, i) = abs(dot(wW, basis.nW));
+              SNL_PF(b + 12, i) = powerHeuristic(sphPdf, lp);
               nmask |= 4;
             }
           }
 #endif
           if (nmask) {
-            PI(I_NEE, i) = nmask;
+            SNL_PI(I_NEE, i) = nmask;
             stRT(a, F_NTP, i, throughput);
             hasNee = true;
           }
@@ -559,8 +559,8 @@ __device__ inline void shade_hit(const SnlKArgs& a) {
       st3(a, F_DX, i, rayDir);
       stRT(a, F_TP, i, throughput);
       stRT(a, F_L, i, L);
-      PF(F_MIS, i) = misWeightSky; PF(F_MIS + 1, i) = misWeightSun; PF(F_MIS + 2, i) = misWeightSph;
-      PI(I_FLAGS, i) = flags;
+      SNL_PF(F_MIS, i) = misWeightSky; SNL_PF(F_MIS + 1, i) = misWeightSun; SNL_PF(F_MIS + 2, i) = misWeightSph;
+      SNL_PI(I_FLAGS, i) = flags;
       stRng(a, i, rnd);
     }
     int s1 = snl_append(&a.cnt[C_TRACE_OUT], cont);
@@ -580,7 +580,7 @@ extern "C" __global__ void __launch_bounds__(256) snl_shade_dielectric(SnlKArgs 
 extern "C" __global__ void __launch_bounds__(256) snl_accumulate(SnlKArgs a) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.npix) return;
-  int flags = PI(I_FLAGS, i);
+  int flags = SNL_PI(I_FLAGS, i);
   bool skip = (flags & PF_SKIP) != 0;
   if (!skip) stRT(a, F_LS, i, ldRT(a, F_LS, i) + ldRT(a, F_L, i));
   if (a.sample != a.spf - 1) return;

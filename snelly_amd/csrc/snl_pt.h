@@ -754,20 +754,20 @@ struct PT {
 
 SNL_HD RT ldRT(const SnlKArgs& a, int k, int i) {
 #ifdef DISPERSION_ENABLED
-  return PF(k, i);
+  return SNL_PF(k, i);
 #else
-  return vec3(PF(k, i), PF(k + 1, i), PF(k + 2, i));
+  return vec3(SNL_PF(k, i), SNL_PF(k + 1, i), SNL_PF(k + 2, i));
 #endif
 }
 SNL_HD void stRT(const SnlKArgs& a, int k, int i, const RT& v) {
 #ifdef DISPERSION_ENABLED
-  PF(k, i) = v;
+  SNL_PF(k, i) = v;
 #else
-  PF(k, i) = v.x; PF(k + 1, i) = v.y; PF(k + 2, i) = v.z;
+  SNL_PF(k, i) = v.x; SNL_PF(k + 1, i) = v.y; SNL_PF(k + 2, i) = v.z;
 #endif
 }
-SNL_HD vec3 ld3(const SnlKArgs& a, int k, int i) { return vec3(PF(k, i), PF(k + 1, i), PF(k + 2, i)); }
-SNL_HD void st3(const SnlKArgs& a, int k, int i, const vec3& v) { PF(k, i) = v.x; PF(k + 1, i) = v.y; PF(k + 2, i) = v.z; }
+SNL_HD vec3 ld3(const SnlKArgs& a, int k, int i) { return vec3(SNL_PF(k, i), SNL_PF(k + 1, i), SNL_PF(k + 2, i)); }
+SNL_HD void st3(const SnlKArgs& a, int k, int i, const vec3& v) { SNL_PF(k, i) = v.x; SNL_PF(k + 1, i) = v.y; SNL_PF(k + 2, i) = v.z; }
 
 #if defined(__HIP_DEVICE_COMPILE__)
 __device__ inline int snl_lane() { return __lane_id(); }

@@ -49,8 +49,8 @@ struct SnlKArgs {
   int shadeThresh;        // persistent kernel: lanes waiting before a wave shades
 };
 
-#define PF(k, i) a.pf[(long long)(k) * a.cap + (i)]
-#define PI(k, i) a.pi[(long long)(k) * a.cap + (i)]
+#define SNL_PF(k, i) a.pf[(long long)(k) * a.cap + (i)]
+#define SNL_PI(k, i) a.pi[(long long)(k) * a.cap + (i)]
 
 
 }  // namespace glsl

@@ -188,12 +188,26 @@ class Translator {
         if (j < toks_.size() && toks_[j].k == T_ID) structs_.insert(toks_[j].s);
       }
     }
+    // and the uniform names, which #define bodies may mention before the declaration
+    for (size_t i = 0; i < toks_.size(); ++i) {
+      if (!(toks_[i].k == T_ID && toks_[i].s == "uniform")) continue;
+      size_t j = next(i);
+      while (j < toks_.size() && dropped().count(toks_[j].s)) j = next(j);
+      j = next(j);  // past the type
+      while (j < toks_.size() && toks_[j].k == T_ID) {
+        if (!builtinUniforms_.count(toks_[j].s)) preUniforms_.insert(toks_[j].s);
+        j = next(j);
+        while (j < toks_.size() && !(toks_[j].k == T_PUNCT && (toks_[j].s == "," || toks_[j].s == ";"))) j = next(j);
+        if (j >= toks_.size() || toks_[j].s == ";") break;
+        j = next(j);
+      }
+    }
     std::ostringstream body;
     size_t i = 0;
     while (i < toks_.size()) {
       const Tok& t = toks_[i];
       if (t.k == T_WS) { body << t.s; ++i; continue; }
-      if (t.k == T_PP) { body << ppLine(t.s) << "\n"; ++i; continue; }
+      if (t.k == T_PP) { noteMacro(t.s); body << ppLine(t.s) << "\n"; ++i; continue; }
       // find the extent of this top-level declaration
       size_t end = i;
       int depth = 0;
@@ -284,6 +298,9 @@ class Translator {
     o << "  SNL_HD float rand(vec4& rnd) { return snl_rand(rnd); }\n";
     o << "#line 1 \"scene.glsl\"\n";
     o << body.str() << "\n};\n";
+    // the scene's own macros end with the scene: kernel code that follows must
+    // not see, say, a "#define PI 3.14159"
+    for (const auto& m : macros_) o << "#undef " << m << "\n";
     out.code = o.str();
     return true;
   }
@@ -303,15 +320,102 @@ class Translator {
     size_t p = s.find_first_not_of("# \t");
     std::string w = p == std::string::npos ? "" : s.substr(p, s.find_first_of(" \t(", p) - p);
     if (w == "version" || w == "extension" || w == "pragma") return "";
+    if (w == "define") return defineLine(s, p + 6);
     return s;
   }
+  // "#define NAME[(params)] body": the body gets the same literal and uniform
+  // rewriting as code (a "#define PI 3.14159" must expand to a float)
+  std::string defineLine(const std::string& s, size_t p) const {
+    size_t b = s.find_first_not_of(" \t", p);
+    if (b == std::string::npos) return s;
+    size_t e = b;
+    while (e < s.size() && (isalnum((unsigned char)s[e]) || s[e] == '_')) ++e;
+    std::set<std::string> params;
+    if (e < s.size() && s[e] == '(') {
+      size_t c = s.find(')', e);
+      if (c == std::string::npos) return s;
+      std::string plist = s.substr(e + 1, c - e - 1);
+      std::string cur;
+      for (char ch : plist + ",") {
+        if (ch == ',') { if (!cur.empty()) params.insert(cur); cur.clear(); }
+        else if (isalnum((unsigned char)ch) || ch == '_') cur += ch;
+      }
+      e = c + 1;
+    }
+    std::string err;
+    std::vector<Tok> bt = tokenize(s.substr(e), err);
+    if (!err.empty()) return s;
+    std::ostringstream o;
+    o << s.substr(0, e);
+    for (size_t i = 0; i < bt.size(); ++i) {
+      const Tok& t = bt[i];
+      if (t.k == T_NUM && isFloatLiteral(t.s)) {
+        std::string v = t.s;
+        if (v.back() != 'f' && v.back() != 'F') v += "f";
+        if (v[0] == '.') v = "0" + v;
+        o << v;
+        continue;
+      }
+      if (t.k == T_ID && !params.count(t.s)) {
+        bool member = false;
+        for (size_t q = i; q > 0;) {
+          --q;
+          if (bt[q].k == T_WS) continue;
+          member = bt[q].k == T_PUNCT && (bt[q].s == "." || bt[q].s == "->");
+          break;
+        }
+        if (!member && preUniforms_.count(t.s)) { o << "snlU_->" << t.s; continue; }
+        if (!member && builtinUniforms_.count(t.s)) { o << "snlP_->" << t.s; continue; }
+      }
+      o << (t.s == "\\\n" ? " " : t.s);
+    }
+    return o.str();
+  }
+  void noteMacro(const std::string& s) {
+    size_t p = s.find_first_not_of("# \t");
+    if (p == std::string::npos || s.compare(p, 6, "define") != 0) return;
+    size_t b = s.find_first_not_of(" \t", p + 6);
+    if (b == std::string::npos) return;
+    size_t e = b;
+    while (e < s.size() && (isalnum((unsigned char)s[e]) || s[e] == '_')) ++e;
+    std::string name = s.substr(b, e - b);
+    // feature switches the kernels test stay defined
+    if (name.empty() || name.compare(0, 4, "HAS_") == 0 || name == "DISPERSION_ENABLED" || name == "INTERACTIVE_MODE")
+      return;
+    macros_.insert(name);
+  }
   bool isType(const std::string& s) const { return builtinTypes().count(s) || structs_.count(s); }
+
+  // A local or parameter named like a uniform hides it inside its scope, as in
+  // GLSL: names declared in the open scopes are never rewritten to block reads.
+  bool shadowed(const std::string& n) const {
+    for (const auto& sc : scopes_) if (sc.count(n)) return true;
+    return false;
+  }
+  void declare(const std::string& n) const {
+    if (!scopes_.empty()) scopes_.back().insert(n);
+  }
 
   // rewrite a token range [b, e) of expressions / statements
   std::string rewrite(size_t b, size_t e) const {
     std::ostringstream o;
+    // declaration tracking: after "TYPE name", names following top-level
+    // commas up to ';' are declared too
+    bool inDecl = false, expectName = false;
+    int declDepth = 0;
     for (size_t i = b; i < e; ++i) {
       const Tok& t = toks_[i];
+      if (t.k == T_PUNCT) {
+        const std::string& p = t.s;
+        if (p == "{") scopes_.emplace_back();
+        else if (p == "}" && !scopes_.empty()) scopes_.pop_back();
+        if (inDecl) {
+          if (p == "(" || p == "[" || p == "{") ++declDepth;
+          else if (p == ")" || p == "]" || p == "}") { if (--declDepth < 0) inDecl = false; }
+          else if (p == ";") inDecl = false;
+          else if (p == "," && declDepth == 0) expectName = true;
+        }
+      }
       if (t.k == T_PP) { o << "\n" << ppLine(t.s) << "\n"; continue; }
       if (t.k == T_NUM) {
         std::string s = t.s;
@@ -331,9 +435,33 @@ class Translator {
         size_t p = i;
         bool member = false;
         while (p > b) { --p; if (toks_[p].k == T_WS) continue; member = (toks_[p].k == T_PUNCT && (toks_[p].s == "." || toks_[p].s == "->")); break; }
+        if (expectName && !member) {
+          declare(t.s);
+          expectName = false;
+        }
+        if (!member && isType(t.s)) {
+          // "TYPE name" or "TYPE[N] name" declares name in the current scope
+          size_t q = next(i);
+          if (q < e && toks_[q].k == T_PUNCT && toks_[q].s == "[") {
+            while (q < e && !(toks_[q].k == T_PUNCT && toks_[q].s == "]")) ++q;
+            q = next(q);
+          }
+          if (q < e && toks_[q].k == T_ID) {
+            inDecl = true;
+            declDepth = 0;
+            expectName = true;
+          }
+        }
         if (!member) {
-          if (uniformNames_.count(t.s)) { o << "snlU_->" << t.s; continue; }
-          if (builtinUniforms_.count(t.s)) { o << "snlP_->" << t.s; continue; }
+          if (!shadowed(t.s)) {
+            if (uniformNames_.count(t.s)) { o << "snlU_->" << t.s; continue; }
+            if (builtinUniforms_.count(t.s)) { o << "snlP_->" << t.s; continue; }
+          }
+          // GLSL float(v) of a vector takes its first component
+          if (t.s == "float") {
+            size_t q = next(i);
+            if (q < e && toks_[q].k == T_PUNCT && toks_[q].s == "(") { o << "snl_float"; continue; }
+          }
           if (t.s == "not") { o << "not_"; continue; }
           if (t.s == "uint") { o << "unsigned"; continue; }
           if (renamed().count(t.s)) { o << t.s << "_"; continue; }
@@ -485,6 +613,19 @@ class Translator {
       }
       if (atEnd || (depth == 0 && toks_[i].k == T_PUNCT && toks_[i].s == ",")) { params.push_back({s, i}); s = i + 1; }
     }
+    // the parameters' names open the function's scope
+    scopes_.clear();
+    scopes_.emplace_back();
+    for (auto& pr : params) {
+      std::string name;
+      for (size_t i = pr.first; i < pr.second; ++i) {
+        if (toks_[i].k == T_PUNCT && toks_[i].s == "[") break;
+        if (toks_[i].k == T_ID) name = toks_[i].s;
+      }
+      if (!name.empty()) scopes_.back().insert(name);
+    }
+    struct PInfo { std::string type, name; bool ref, vec, array; };
+    std::vector<PInfo> pinfo;
     bool first = true;
     for (auto& pr : params) {
       std::vector<size_t> w;
@@ -508,9 +649,53 @@ class Translator {
       first = false;
       if (isConst && !ref) o << "const ";
       o << type << (ref ? "& " : " ") << rest;
+      bool arr = rest.find('[') != std::string::npos;
+      pinfo.push_back({type, rest, ref, type == "vec2" || type == "vec3" || type == "vec4", arr});
     }
     o << ")";
     o << rewrite(rp + 1, e + 1) << "\n";
+    scopes_.clear();
+    // GLSL passes a swizzle (p.xz) to an out/inout vector parameter by copy-in /
+    // copy-out; a C++ reference cannot bind to it, so add an overload that takes
+    // the swizzle, runs the function on a copy and writes the copy back
+    bool wrap = false;
+    for (auto& pi : pinfo) {
+      if (pi.array) return o.str();
+      wrap |= pi.ref && pi.vec;
+    }
+    if (wrap) {
+      size_t nameIdx = lp;
+      while (nameIdx > b && toks_[nameIdx].k != T_ID) --nameIdx;
+      std::string ret = rewrite(b, nameIdx), fname = toks_[nameIdx].s;
+      std::ostringstream tp, args, pre, post, call;
+      int nt = 0;
+      for (size_t q = 0; q < pinfo.size(); ++q) {
+        const PInfo& pi = pinfo[q];
+        std::string an = "snl_a" + std::to_string(q);
+        if (q) { args << ", "; call << ", "; }
+        if (pi.ref && pi.vec) {
+          std::string tn = "SnlA" + std::to_string(q);
+          tp << (nt++ ? ", " : "") << "class " << tn;
+          args << tn << "& " << an;
+          pre << " " << pi.type << " snl_t" << q << " = " << an << ";";
+          post << " " << an << " = snl_t" << q << ";";
+          call << "snl_t" << q;
+        } else {
+          args << pi.type << (pi.ref ? "& " : " ") << an;
+          call << an;
+        }
+      }
+      std::ostringstream cond;
+      int c = 0;
+      for (size_t q = 0; q < pinfo.size(); ++q)
+        if (pinfo[q].ref && pinfo[q].vec)
+          cond << (c++ ? " && " : "") << "snl_lv<SnlA" << q << ", " << pinfo[q].type << ">::ok";
+      bool isVoid = ret.find("void") != std::string::npos;
+      o << "template <" << tp.str() << ", typename snl_if<(" << cond.str() << ")>::type = 0>\n"
+        << "SNL_HD " << ret << " " << fname << "(" << args.str() << ") {" << pre.str() << " "
+        << (isVoid ? "" : "auto snl_r = ") << fname << "(" << call.str() << ");" << post.str()
+        << (isVoid ? "" : " return snl_r;") << " }\n";
+    }
     return o.str();
   }
 
@@ -520,6 +705,8 @@ class Translator {
   std::set<std::string> structs_;
   std::vector<Uniform> uniforms_;
   std::set<std::string> uniformNames_;
+  std::set<std::string> macros_, preUniforms_;
+  mutable std::vector<std::set<std::string>> scopes_;
 };
 
 }  // namespace

@@ -14,6 +14,7 @@
 #include <unistd.h>
 
 #include <algorithm>
+#include <cctype>
 #include <cstdarg>
 #include <cstddef>
 #include <cstdio>
@@ -22,6 +23,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -62,6 +64,7 @@ struct snl_state_s {
   int* q = nullptr;    // 8 * cap ints: trace A/B, nee A/B, mat[4]
   int* cnt = nullptr;  // C_COUNT ints
   int* tileCnt = nullptr;  // SNL_MAX_TILES * C_COUNT ints (persistent engine)
+  float* radPrev = nullptr;  // radiance as of the frame start, for samplers left on texture unit 0
 };
 
 struct snl_prog_s {
@@ -84,6 +87,10 @@ struct snl_prog_s {
   snl_stats stats{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr, et0 = nullptr, et1 = nullptr;
   std::vector<hipEvent_t> traceEvents;  // start/stop pairs of the trace launches (profiling)
+  // 2D samplers the scene code reads (uniform indices) and those the host has
+  // bound; an unbound one samples texture unit 0, the radiance being read
+  std::vector<int> sampled2D;
+  std::set<int> bound;
 };
 
 namespace {
@@ -380,6 +387,23 @@ int snl_program_build(snl_ctx ctx, const char* shader, const char* ior, const ch
   memset(&p->hostP, 0, sizeof(SnlParams));
   p->user = c.tr.uniforms;
   p->hostU.assign(c.tr.userBlockSize, 0);
+  {
+    std::string text = std::string(shader) + "\n" + (ior ? ior : "");
+    auto mentions = [&](const std::string& name) {
+      for (size_t at = text.find(name); at != std::string::npos; at = text.find(name, at + 1)) {
+        bool l = at == 0 || !(isalnum((unsigned char)text[at - 1]) || text[at - 1] == '_');
+        size_t e = at + name.size();
+        bool r = e >= text.size() || !(isalnum((unsigned char)text[e]) || text[e] == '_');
+        if (l && r) return true;
+      }
+      return false;
+    };
+    const auto& b = builtins();
+    for (int i = 0; i < (int)b.size(); ++i)
+      if (b[i].type == SNL_T_SAMPLER2D && mentions(b[i].name)) p->sampled2D.push_back(i);
+    for (int j = 0; j < (int)p->user.size(); ++j)
+      if (p->user[j].type == SNL_T_SAMPLER2D) p->sampled2D.push_back(1000 + j);
+  }
   HIPCHK(ctx, hipMalloc((void**)&p->dP, sizeof(SnlParams)));
   HIPCHK(ctx, hipMalloc(&p->dU, p->hostU.size()));
   p->traceBlocks = ctx->numCU * 8;
@@ -478,6 +502,8 @@ int snl_uniform_texture(snl_prog p, int idx, snl_tex t) {
     off = p->user[j].offset;
     utype = p->user[j].type;
   }
+  if (t) p->bound.insert(idx);
+  else p->bound.erase(idx);
   if (utype == SNL_T_SAMPLER2D) {
     sampler2D s{t ? t->d : nullptr, t ? t->w : 0, t ? t->h : 0, t ? t->ch : 4, t ? t->flags : 0};
     memcpy(base + off, &s, sizeof s);
@@ -556,7 +582,7 @@ int snl_state_destroy(snl_state s) {
   hipSetDevice(s->ctx->device);
   hipStreamSynchronize(s->ctx->stream);
   for (void* p : {(void*)s->radiance, (void*)s->rng, (void*)s->pf, (void*)s->pi, (void*)s->q, (void*)s->cnt,
-                  (void*)s->tileCnt})
+                  (void*)s->tileCnt, (void*)s->radPrev})
     if (p) hipFree(p);
   delete s;
   return SNL_OK;
@@ -589,11 +615,44 @@ int snl_state_write(snl_state s, int which, const float* in) {
   return SNL_OK;
 }
 
+// A GL sampler uniform that is never assigned a unit reads unit 0, where the
+// reference's Renderer has bound the radiance texture being read this frame
+// (js/renderer.js:590, 706-713) -- e.g. TabulatedMetal's kTex, whose texture is
+// bound under the name "k_tex" (js/materials.js:664-665).  Such samplers get a
+// NEAREST/CLAMP view of a per-frame snapshot of the radiance buffer, since the
+// kernels accumulate into that buffer while the frame runs.  Returns whether
+// snapshots are needed; copyUnit0 takes one.
+static int prepareUnit0(snl_prog p, snl_state st, bool& active) {
+  active = false;
+  for (int idx : p->sampled2D) active |= !p->bound.count(idx);
+  if (!active) return SNL_OK;
+  snl_ctx ctx = p->ctx;
+  if (!st->radPrev) {
+    if (hipMalloc((void**)&st->radPrev, (size_t)st->npix * 16) != hipSuccess)
+      return fail(ctx, SNL_E_NOMEM, "radiance snapshot allocation failed");
+  }
+  sampler2D view{st->radPrev, st->w, st->h, 4, 2};
+  for (int idx : p->sampled2D) {
+    if (p->bound.count(idx)) continue;
+    char* base = idx < 1000 ? reinterpret_cast<char*>(&p->hostP) : p->hostU.data();
+    int off = idx < 1000 ? builtins()[idx].offset : p->user[idx - 1000].offset;
+    memcpy(base + off, &view, sizeof view);
+  }
+  p->dirty = true;
+  return SNL_OK;
+}
+static int copyUnit0(snl_ctx ctx, snl_state st, hipStream_t s) {
+  HIPCHK(ctx, hipMemcpyAsync(st->radPrev, st->radiance, (size_t)st->npix * 16, hipMemcpyDeviceToDevice, s));
+  return SNL_OK;
+}
+
 static int renderWavefront(snl_prog p, snl_state st, int frames) {
   snl_ctx ctx = p->ctx;
   if (st->ctx != ctx) return fail(ctx, SNL_E_ARG, "program and state belong to different contexts");
-  int rc = upload(p);
+  bool unit0 = false;
+  int rc = prepareUnit0(p, st, unit0);
   if (rc) return rc;
+  if ((rc = upload(p))) return rc;
   hipStream_t s = ctx->stream;
   SnlKArgs a;
   fillArgs(p, st, a);
@@ -609,6 +668,7 @@ static int renderWavefront(snl_prog p, snl_state st, int frames) {
   double traceMs = 0.0;
   int traceLaunches = 0;
   for (int f = 0; f < frames; ++f) {
+    if (unit0 && (rc = copyUnit0(ctx, st, s))) return rc;
     for (int smp = 0; smp < p->spf; ++smp) {
       a.sample = smp;
       int cur = 0;
@@ -671,10 +731,14 @@ static int renderWavefront(snl_prog p, snl_state st, int frames) {
 // tiles run on their own streams so one tile's tail overlaps another's work.
 static int renderPersistent(snl_prog p, snl_state st, int frames) {
   snl_ctx ctx = p->ctx;
-  int rc = upload(p);
+  bool unit0 = false;
+  int rc = prepareUnit0(p, st, unit0);
   if (rc) return rc;
+  if ((rc = upload(p))) return rc;
   int K = std::max(1, std::min(p->tiles, SNL_MAX_TILES));
-  if ((long long)K > st->npix) K = 1;
+  // a frame must see the whole previous frame through the unit-0 snapshot,
+  // so such programs run their frames in order on one stream
+  if ((long long)K > st->npix || unit0) K = 1;
   p->stats = snl_stats{};
   SnlKArgs base;
   fillArgs(p, st, base);
@@ -699,6 +763,7 @@ static int renderPersistent(snl_prog p, snl_state st, int frames) {
     a.itemEnd = (int)(st->npix * (k + 1) / K);
     void* args[] = {&a};
     for (int f = 0; f < frames; ++f) {
+      if (unit0 && (rc = copyUnit0(ctx, st, ts))) return rc;
       HIPCHK(ctx, hipModuleLaunchKernel(p->k_persist_reset, 1, 1, 1, 64, 1, 1, 0, ts, args, nullptr));
       HIPCHK(ctx, hipModuleLaunchKernel(p->k_persist, blocks, 1, 1, 256, 1, 1, 0, ts, args, nullptr));
     }

@@ -1,0 +1,153 @@
+"""Parity against the reference's own shaders.
+
+tests/golden/ref_<case>.npz hold what shaders/pathtracer-fragment-shader.glsl,
+pick-fragment-shader.glsl and normals-fragment-shader.glsl compute for the
+scenes of tests/golden/reference_cases.py, from integer RNG seeds
+(make_reference_fixtures.py: a GLES 3 implementation with WebGL's zero
+initialisation of locals and accurate transcendentals).
+
+The oracle (CPU) and the HIP path (GPU) are held to the same bars:
+  * the sample count channel is identical;
+  * the per-pixel RNG states after all frames are bit-identical for >= 99% of
+    pixels -- every random decision of those paths was the same;
+  * per pixel, radiance within 1e-3 relative for >= 99% of pixels (1e-4 for
+    most: central-difference normals, eps = 2e-4, turn last-ulp SDF differences
+    into ~1e-4 relative noise in the shading);
+  * first hits: pick distance within 1e-4 relative and the same material for
+    >= 99.5% of pixels; the normals pass within 1e-3 for >= 99.5%.
+The Mandelbox dielectric (16 bounces through a fractal) is chaotic: paths
+separate after a few internal bounces, so its radiance is compared as
+statistics (mean per channel within 4 standard errors) and its first hits per
+pixel.
+"""
+import json
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+
+import reference_cases  # noqa: E402
+
+CASES = list(reference_cases.CASES)
+CHAOTIC = {"mandelbox_dielectric"}
+ORACLE_CASES = [c for c in CASES if reference_cases.build(c)[0].oracle_sdfs]
+
+
+def fixture(name):
+    z = np.load(os.path.join(HERE, "golden", f"ref_{name}.npz"))
+    meta = json.loads(bytes(z["meta"]).decode())
+    return {k: z[k] for k in z.files if k != "meta"}, meta
+
+
+def rel_close(a, b, tol):
+    return np.all(np.abs(a[:, :3] - b[:, :3]) <= tol * np.maximum(np.abs(b[:, :3]), 1e-3), axis=1)
+
+
+def check_pathtrace(name, rad, rng, ref):
+    """the bars of the module docstring for one render against its fixture"""
+    assert np.array_equal(rad[:, 3], ref["radiance"][:, 3])
+    if name in CHAOTIC:
+        d = rad[:, :3].astype(np.float64) - ref["radiance"][:, :3]
+        se = d.std(axis=0) / math.sqrt(d.shape[0])
+        assert np.all(np.abs(d.mean(axis=0)) <= 4.0 * se + 1e-6), (d.mean(axis=0), se)
+        return
+    same_rng = np.all(rng == ref["rng"], axis=1).mean()
+    c3 = rel_close(rad, ref["radiance"], 1e-3).mean()
+    c4 = rel_close(rad, ref["radiance"], 1e-4).mean()
+    assert same_rng >= 0.99, same_rng
+    assert c3 >= 0.99, (c3, c4)
+    assert c4 >= 0.9, (c3, c4)
+
+
+def check_pick(pk, ref):
+    rp = ref["pick"]
+    same = pk[:, 1] == rp[:, 1]
+    assert same.mean() >= 0.995
+    rel = np.abs(pk[same, 0] - rp[same, 0]) / np.abs(rp[same, 0])
+    assert np.mean(rel <= 1e-4) >= 0.995
+
+
+def check_normals(name, nrad, nrng, ref):
+    assert np.array_equal(nrng, ref["normals_rng"])
+    tol, frac = (1e-2, 0.95) if name in CHAOTIC else (1e-3, 0.995)
+    assert rel_close(nrad, ref["normals"], tol).mean() >= frac
+
+
+def pixel_grid(w, h):
+    return np.stack(np.meshgrid(np.arange(w) + 0.5, np.arange(h) + 0.5), -1).reshape(-1, 2).astype(np.float32)
+
+
+# ---------------------------------------------------------------- CPU: fixtures and oracle
+
+@pytest.mark.parametrize("name", CASES)
+def test_fixture_shapes(name):
+    ref, meta = fixture(name)
+    s, W, H, frames, seed = reference_cases.build(name)
+    assert (meta["width"], meta["height"], meta["frames"], meta["seed"]) == (W, H, frames, seed)
+    n = W * H
+    for k in ("radiance", "rng", "normals", "normals_rng"):
+        assert ref[k].shape == (n, 4) and ref[k].dtype == np.float32
+    assert ref["pick"].shape == (n, 2)
+    assert np.all(ref["radiance"][:, 3] == frames + 1)  # cleared to N = 1, one sample per frame
+    assert np.all(np.isfinite(ref["radiance"]))
+
+
+@pytest.mark.parametrize("name", ORACLE_CASES)
+def test_oracle_pathtrace_matches_reference(name):
+    import oracle_py
+    ref, _ = fixture(name)
+    s, W, H, frames, seed = reference_cases.build(name)
+    o = oracle_py.Oracle(s, W, H)
+    rad, rng = o.new_state(seed)
+    o.render(rad, rng, frames)
+    check_pathtrace(name, rad, rng, ref)
+
+
+@pytest.mark.parametrize("name", ORACLE_CASES)
+def test_oracle_first_hit_matches_reference(name):
+    import oracle_py
+    ref, _ = fixture(name)
+    s, W, H, frames, seed = reference_cases.build(name)
+    o = oracle_py.Oracle(s, W, H)
+    check_pick(o.pick(pixel_grid(W, H)), ref)
+    nrad, nrng = o.new_state(seed)
+    o.normals(nrad, nrng, 1)
+    check_normals(name, nrad, nrng, ref)
+
+
+# ---------------------------------------------------------------- GPU: the HIP path
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("engine", ["persistent", "wavefront"])
+@pytest.mark.parametrize("name", CASES)
+def test_gpu_pathtrace_matches_reference(ctx, name, engine):
+    from snelly_amd import host, native
+    ref, _ = fixture(name)
+    s, W, H, frames, seed = reference_cases.build(name)
+    sess = host.Session(s, W, H, ctx, seed=seed, flags=native.BUILD_PRECISE)
+    sess.prog.engine(native.ENGINE_PERSISTENT if engine == "persistent" else native.ENGINE_WAVEFRONT)
+    sess.render(frames)
+    rad = sess.radiance().reshape(-1, 4)
+    rng = sess.state.read(1).reshape(-1, 4)
+    sess.close()
+    check_pathtrace(name, rad, rng, ref)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", CASES)
+def test_gpu_first_hit_matches_reference(ctx, name):
+    from snelly_amd import host, native
+    ref, _ = fixture(name)
+    s, W, H, frames, seed = reference_cases.build(name)
+    sess = host.Session(s, W, H, ctx, seed=seed, flags=native.BUILD_PRECISE)
+    check_pick(sess.prog.pick(pixel_grid(W, H)), ref)
+    sess.prog.render_normals(sess.state, 1)
+    nrad = sess.radiance().reshape(-1, 4)
+    nrng = sess.state.read(1).reshape(-1, 4)
+    sess.close()
+    check_normals(name, nrad, nrng, ref)

@@ -95,7 +95,7 @@ struct Config {
 } C;
 
 // ---------------------------------------------------------------- scene (hand-written SDFs)
-enum { SDF_NONE, SDF_SPHERE, SDF_SPHERE_PLANE, SDF_MENGER, SDF_MANDELBOX, SDF_TORUS };
+enum { SDF_NONE, SDF_SPHERE, SDF_SPHERE_PLANE, SDF_MENGER, SDF_MANDELBOX, SDF_TORUS, SDF_PLANE };
 struct SdfSpec { int kind = SDF_NONE; float a = 0, b = 0, c = 0, d = 0; };
 SdfSpec sdfs[3];        // indexed by material id (DIELE, METAL, SURFA)
 int albedoHook = 0;     // 1: sphere/plane albedo (0.8 above y=-1+1e-3, else 0.5)
@@ -188,6 +188,7 @@ T evalSdfT(const SdfSpec& s, const T X[3]) {
       T qx = sqrt_(X[0] * X[0] + X[2] * X[2]) - lit<T>(s.a);
       return sqrt_(qx * qx + X[1] * X[1]) - lit<T>(s.b);
     }
+    case SDF_PLANE: return X[1] + lit<T>(s.a);  // the plane y = -a
   }
   return lit<T>(1.0e20f);
 }
@@ -259,9 +260,24 @@ float IOR_METAL(float wl) {
   if (metalModel == 1) { float t = (wl - 390.0f) / 360.0f; return 0.2f + 0.8f * t; }
   return lerpTable(metalN.data(), 64, (wl - 390.0f) * float(1.0 / (750.0 - 390.0)));
 }
-float K_METAL(float wl) {
+float K_METAL_table(float wl) {
   if (metalModel == 1) { float t = (wl - 390.0f) / 360.0f; return 3.5f - 1.5f * t; }
   return lerpTable(metalK.data(), 64, (wl - 390.0f) * float(1.0 / (750.0 - 390.0)));
+}
+// TabulatedMetal's K_METAL samples the sampler kTex, which the reference never
+// assigns a unit (it binds the k texture as "k_tex", js/materials.js:664-665),
+// so it reads unit 0: the radiance texture being read this frame
+// (js/renderer.js:590, 706-713), NEAREST and CLAMP_TO_EDGE, red channel.
+std::vector<float> unit0Radiance;
+int unit0W = 0, unit0H = 0;
+float K_METAL(float wl) {
+  if (metalModel == 1) return K_METAL_table(wl);
+  float u = (wl - 390.0f) * float(1.0 / (750.0 - 390.0));
+  if (unit0W <= 0) return 0.0f;
+  int i = (int)std::floor(u * float(unit0W)), j = (int)std::floor(0.5f * float(unit0H));
+  i = i < 0 ? 0 : (i >= unit0W ? unit0W - 1 : i);
+  j = j < 0 ? 0 : (j >= unit0H ? unit0H - 1 : j);
+  return unit0Radiance[4 * ((size_t)j * unit0W + i)];
 }
 
 std::vector<float> lutXYZ(4096, 0.0f), lutICDF(4096, 0.0f);
@@ -1004,7 +1020,7 @@ void oracle_luts(const float* xyz4096, const float* icdf4096) {
 float oracle_rand(float* state4) { return randf(state4); }
 float oracle_ior_diele(float wl) { return IOR_DIELE(wl); }
 float oracle_ior_metal(float wl) { return IOR_METAL(wl); }
-float oracle_k_metal(float wl) { return K_METAL(wl); }
+float oracle_k_metal(float wl) { return K_METAL_table(wl); }
 float oracle_fresnel_dielectric(float c, float eta) { return fresnelDielectricReflectance(c, eta); }
 float oracle_fresnel_metal(float c, float n, float k) { return fresnelMetalReflectance(c, n, k); }
 float oracle_microfacet_eval(const float* m, float r) { return microfacetEval(v3(m[0], m[1], m[2]), r); }
@@ -1059,6 +1075,24 @@ int oracle_render(float* radiance, float* rng, int W, int H, int frames, int str
   long long npix = (long long)W * H;
 #ifdef _OPENMP
   if (threads > 0) omp_set_num_threads(threads);
+#endif
+  if (metalModel == 0 && hasMat(MAT_METAL)) {
+    // K_METAL reads the radiance as of the frame start: frames in order
+    unit0W = W;
+    unit0H = H;
+    for (int f = 0; f < frames; ++f) {
+      unit0Radiance.assign(radiance, radiance + 4 * npix);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64)
+#endif
+      for (long long i = offset; i < npix; i += stride) {
+        if (C.dispersive) pathtracePixel<float>((int)(i % W), (int)(i / W), radiance + 4 * i, rng + 4 * i);
+        else pathtracePixel<V3>((int)(i % W), (int)(i / W), radiance + 4 * i, rng + 4 * i);
+      }
+    }
+    return 0;
+  }
+#ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 64)
 #endif
   for (long long i = offset; i < npix; i += stride) {

@@ -33,7 +33,7 @@ def _tabulated_metal(s):
     s.glsl = scenes.TORUS_GLSL.replace("@MAT@", "METAL")
     s.materials.metal = host.metal("Gold")
     s.materials.metal.roughness = 0.1
-    s.oracle_sdfs = []
+    s.oracle_sdfs = [(1, 5, 1.0, 0.3, 0, 0)]  # torus R = 1, r = 0.3
 
 
 def _bk7_sphere(s):
@@ -48,7 +48,8 @@ float SDF_SURFACE(vec3 X) { return X.y + 1.0; }
     s.renderer.sunPower = 0.5
     s.renderer.sunLongitude = 60.0
     s.materials.surface.diffuseAlbedo = [0.7, 0.7, 0.7]
-    s.oracle_sdfs = []
+    s.oracle_sdfs = [(0, 1, 1.0, 0, 0, 0), (2, 6, 1.0, 0, 0, 0)]  # unit sphere; plane y = -1
+    s.oracle_albedo_hook = 0
 
 
 CASES = {

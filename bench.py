@@ -7,7 +7,10 @@ samples per pixel from a freshly seeded accumulation state, i.e. 67,108,864
 path samples.  Scene data are synthetic (an analytic SDF); nothing is read from
 disk.  Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--cpu-seconds 15]
+Also measured (the "batched" object): the 64-scene batched workload rendered
+plain (one launch per scene and frame) and as one batch.
+
+  python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--cpu-seconds 15] [--no-batch]
 """
 import argparse
 import json
@@ -58,6 +61,49 @@ def cpu_baseline(scene, width, height, spp, budget_s):
                       f"{dt:.1f} s on {threads} threads"}
 
 
+def batched(ctx, steps, warmup):
+    """The batched workload (64 seeded scenes, scenes.batch_jobs): the same jobs
+    rendered plain -- one scene per launch and one launch per frame, job after
+    job, as the reference draws -- and as one snl_render_batch call."""
+    from snelly_amd import host, native, scenes
+    jobs = scenes.batch_jobs()
+    sessions = [host.Session(s, s.width, s.height, ctx, seed=500 + j, flags=native.BUILD_FAST)
+                for j, s in enumerate(jobs)]
+    samples = sum(s.width * s.height * s.spp for s in jobs)
+    ctx.sync()
+    t = time.perf_counter()
+    for se, s in zip(sessions, jobs):
+        for _ in range(s.spp):
+            se.render(1)
+    ctx.sync()
+    plain_s = time.perf_counter() - t
+    batch = [(se.prog, se.state, s.spp) for se, s in zip(sessions, jobs)]
+    for w in range(warmup):
+        native.render_batch(batch)
+    times, gpu_ms = [], []
+    for k in range(steps):
+        for j, se in enumerate(sessions):
+            se.state.seed(900 + 64 * k + j)
+        ctx.sync()
+        t = time.perf_counter()
+        st = native.render_batch(batch)
+        ctx.sync()
+        times.append(time.perf_counter() - t)
+        gpu_ms.append(st.ms_total)
+    for se in sessions:
+        se.close()
+    batch_s = sum(times) / len(times)
+    fams = {}
+    for s in jobs:
+        fams[s.name] = fams.get(s.name, 0) + 1
+    return {"workload": "batch64_mixed_sdf", "jobs": len(jobs), "path_samples_per_batch": samples,
+            "distinct_programs": len({(s.glsl, s.name) for s in jobs}),
+            "plain_path_samples_per_s": round(samples / plain_s, 1), "plain_s": round(plain_s, 3),
+            "batched_path_samples_per_s": round(samples / batch_s, 1), "batched_s": round(batch_s, 3),
+            "batched_gpu_ms": round(sum(gpu_ms) / len(gpu_ms), 2), "speedup_vs_plain": round(plain_s / batch_s, 2),
+            "build": "fast", "jobs_by_scene": fams}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +114,7 @@ def main():
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the path is per-image data parallel "
@@ -149,6 +196,8 @@ def main():
         "spp_for_rrmse_1pct": None if rr is None else int(math.ceil(spp * (rr / 0.01) ** 2)),
     }
     sess.close()
+    if not args.no_batch:
+        out["batched"] = batched(ctx, max(1, min(args.steps, 3)), min(args.warmup, 1))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scenes.menger_metal(W, H, spp), W, H, spp, args.cpu_seconds)
     print(json.dumps(out), flush=True)

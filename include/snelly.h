@@ -125,13 +125,26 @@ int snl_set_profiling(snl_prog prog, int on);
 
 /* Render engines.  WAVEFRONT: per bounce, a camera / trace / material-sorted
  * shade launch sequence over SoA path queues.  PERSISTENT: one launch per
- * frame and pixel tile in which every lane runs whole paths from registers
- * (no per-bounce barrier); tiles are pipelined on separate streams.  Both draw
- * the same random numbers per path.  Programs whose surface has sub-surface
- * scattering always render with the wavefront engine.  0 keeps a knob. */
+ * pixel tile in which every lane runs whole paths from registers (no
+ * per-bounce barrier) and a pixel's frames back to back; tiles run on separate
+ * streams.  Both draw the same random numbers per path.  Programs whose
+ * surface has sub-surface scattering always render with the wavefront engine.
+ * 0 keeps a knob. */
 #define SNL_ENGINE_WAVEFRONT 0
 #define SNL_ENGINE_PERSISTENT 1
 int snl_set_engine(snl_prog prog, int engine, int tiles, int persist_blocks, int shade_thresh, int chunk);
+
+/* Batched rendering: frames[j] frames of every job j = (progs[j], states[j]),
+ * each with its program's current uniforms, submitted together -- the jobs'
+ * persistent launches are spread over the context's streams, largest first,
+ * so that many small images fill the device at once.  The reference renders
+ * one scene per canvas per frame (js/renderer.js:462-644); per job the result
+ * is identical to snl_render(progs[j], states[j], frames[j]).  All programs
+ * belong to one context; every job has its own state.  Blocks until the batch
+ * is done; `stats` (may be NULL) receives the batch's wall time and summed
+ * counters. */
+int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, const int* frames,
+                     snl_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -166,6 +166,17 @@ struct Compiled {
 int compileProgram(snl_ctx ctx, const char* shader, const char* ior, const char* defines, int maxMarch,
                    int maxBounces, int maxAtm, int spf, unsigned flags, Compiled& out, char* log, int loglen) {
   std::string glsl = std::string(shader ? shader : "") + "\n" + (ior ? ior : "") + "\n";
+  // Renderer.compileShaders substitutes these tokens throughout the assembled
+  // source, scene code included (js/renderer.js:288-294)
+  const std::pair<const char*, int> tokens[] = {{"__MAX_MARCH_STEPS__", maxMarch},
+                                                {"__MAX_BOUNCES__", maxBounces},
+                                                {"__MAX_ATMOSPHERE_SCATTERS__", maxAtm},
+                                                {"__MAX_SAMPLES_PER_FRAME__", spf}};
+  for (const auto& tk : tokens) {
+    std::string v = std::to_string(tk.second);
+    for (size_t at = glsl.find(tk.first); at != std::string::npos; at = glsl.find(tk.first, at + v.size()))
+      glsl.replace(at, strlen(tk.first), v);
+  }
   out.defines = snl::parseDefines(defines ? defines : "");
   if (!snl::translateScene(glsl, builtinNames(), out.defines, out.tr)) {
     if (log && loglen > 0) snprintf(log, loglen, "%s", out.tr.error.c_str());
@@ -185,7 +196,6 @@ int compileProgram(snl_ctx ctx, const char* shader, const char* ior, const char*
   std::vector<std::string> opts = {"--offload-arch=gfx950", "-O3", "-std=c++17",
                                    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                                    "-DSNL_MAX_MARCH_STEPS=" + std::to_string(maxMarch),
-                                   "-DSNL_MAX_BOUNCES=" + std::to_string(maxBounces),
                                    "-DSNL_MAX_ATMOSPHERE_SCATTERS=" + std::to_string(maxAtm),
                                    "-DSNL_MAX_SAMPLES_PER_FRAME=" + std::to_string(spf)};
   if (flags & SNL_BUILD_PRECISE) opts.push_back("-ffp-contract=off");
@@ -252,6 +262,8 @@ void fillArgs(snl_prog p, snl_state st, SnlKArgs& a) {
   a.npix = (int)st->npix;
   a.spf = p->spf;
   a.chunk = p->traceChunk;
+  a.frames = 1;
+  a.maxBounces = p->maxBounces;
 }
 
 int launch(snl_ctx ctx, hipFunction_t f, int grid, int block, SnlKArgs& a) {
@@ -762,8 +774,16 @@ static int renderPersistent(snl_prog p, snl_state st, int frames) {
     a.itemBase = (int)(st->npix * k / K);
     a.itemEnd = (int)(st->npix * (k + 1) / K);
     void* args[] = {&a};
-    for (int f = 0; f < frames; ++f) {
-      if (unit0 && (rc = copyUnit0(ctx, st, ts))) return rc;
+    if (unit0) {
+      // one launch per frame, each after a snapshot of the radiance
+      for (int f = 0; f < frames; ++f) {
+        if ((rc = copyUnit0(ctx, st, ts))) return rc;
+        HIPCHK(ctx, hipModuleLaunchKernel(p->k_persist_reset, 1, 1, 1, 64, 1, 1, 0, ts, args, nullptr));
+        HIPCHK(ctx, hipModuleLaunchKernel(p->k_persist, blocks, 1, 1, 256, 1, 1, 0, ts, args, nullptr));
+      }
+    } else if (frames > 0) {
+      // every pixel runs all its frames in one lane: one launch per tile
+      a.frames = frames;
       HIPCHK(ctx, hipModuleLaunchKernel(p->k_persist_reset, 1, 1, 1, 64, 1, 1, 0, ts, args, nullptr));
       HIPCHK(ctx, hipModuleLaunchKernel(p->k_persist, blocks, 1, 1, 256, 1, 1, 0, ts, args, nullptr));
     }
@@ -785,7 +805,95 @@ static int renderPersistent(snl_prog p, snl_state st, int frames) {
   p->stats.ms_trace = ms;  // the persistent kernel is the whole frame
   p->stats.trace_steps = steps;
   p->stats.paths = (long long)frames * p->spf * st->npix;
-  p->stats.trace_launches = frames * K;
+  p->stats.trace_launches = unit0 ? frames * K : K;
+  return SNL_OK;
+}
+
+int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, const int* frames,
+                     snl_stats* stats) {
+  if (njobs < 0 || (njobs > 0 && (!progs || !states || !frames))) return fail(nullptr, SNL_E_ARG, "bad batch");
+  if (stats) *stats = snl_stats{};
+  if (njobs == 0) return SNL_OK;
+  snl_ctx ctx = progs[0] ? progs[0]->ctx : nullptr;
+  if (!ctx) return fail(nullptr, SNL_E_ARG, "NULL program in batch");
+  std::set<snl_state> seen;
+  for (int j = 0; j < njobs; ++j) {
+    if (!progs[j] || !states[j] || frames[j] < 0) return fail(ctx, SNL_E_ARG, "bad job %d", j);
+    if (progs[j]->ctx != ctx || states[j]->ctx != ctx) return fail(ctx, SNL_E_ARG, "job %d: other context", j);
+    if (!seen.insert(states[j]).second) return fail(ctx, SNL_E_ARG, "job %d: state used twice", j);
+    if (progs[j]->hostP.subsurfaceMFP > 0.0f)
+      return fail(ctx, SNL_E_UNSUPPORTED, "job %d: sub-surface scattering renders only with snl_render", j);
+  }
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  // parameters up, per-job counters zeroed, all on the main stream
+  std::vector<char> unit0(njobs, 0);
+  int rc;
+  for (int j = 0; j < njobs; ++j) {
+    bool u = false;
+    if ((rc = prepareUnit0(progs[j], states[j], u))) return rc;
+    unit0[j] = u;
+    if ((rc = upload(progs[j]))) return rc;
+    SnlKArgs z;
+    fillArgs(progs[j], states[j], z);
+    z.vertex = -2;
+    z.cnt = states[j]->tileCnt;
+    if ((rc = launch(ctx, progs[j]->k_reset, 1, 64, z))) return rc;
+  }
+  // largest job first onto the least loaded stream (work ~ pixels * frames)
+  std::vector<int> order(njobs);
+  for (int j = 0; j < njobs; ++j) order[j] = j;
+  auto work = [&](int j) { return (double)states[j]->npix * frames[j] * progs[j]->spf; };
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
+  const int NS = SNL_MAX_TILES;
+  double load[NS] = {};
+  HIPCHK(ctx, hipEventRecord(progs[0]->ev0, ctx->stream));
+  HIPCHK(ctx, hipEventRecord(ctx->joinEvent, ctx->stream));
+  for (int k = 0; k < NS; ++k) HIPCHK(ctx, hipStreamWaitEvent(ctx->tileStream[k], ctx->joinEvent, 0));
+  for (int j : order) {
+    int k = (int)(std::min_element(load, load + NS) - load);
+    load[k] += work(j);
+    snl_prog p = progs[j];
+    snl_state st = states[j];
+    hipStream_t ts = ctx->tileStream[k];
+    SnlKArgs a;
+    fillArgs(p, st, a);
+    a.chunk = p->persistChunk;
+    a.shadeThresh = p->shadeThresh;
+    a.cnt = st->tileCnt;
+    a.itemBase = 0;
+    a.itemEnd = (int)st->npix;
+    // one lane per pixel up to the program's full persistent grid
+    long long want = (st->npix + 255) / 256;
+    int blocks = (int)std::max(1LL, std::min<long long>(p->persistBlocks, want));
+    void* args[] = {&a};
+    int launches = unit0[j] ? frames[j] : (frames[j] > 0 ? 1 : 0);
+    a.frames = unit0[j] ? 1 : frames[j];
+    for (int f = 0; f < launches; ++f) {
+      if (unit0[j] && (rc = copyUnit0(ctx, st, ts))) return rc;
+      HIPCHK(ctx, hipModuleLaunchKernel(p->k_persist_reset, 1, 1, 1, 64, 1, 1, 0, ts, args, nullptr));
+      HIPCHK(ctx, hipModuleLaunchKernel(p->k_persist, blocks, 1, 1, 256, 1, 1, 0, ts, args, nullptr));
+    }
+  }
+  for (int k = 0; k < NS; ++k) {
+    HIPCHK(ctx, hipEventRecord(ctx->tileEvent[k], ctx->tileStream[k]));
+    HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->tileEvent[k], 0));
+  }
+  HIPCHK(ctx, hipEventRecord(progs[0]->ev1, ctx->stream));
+  HIPCHK(ctx, hipEventSynchronize(progs[0]->ev1));
+  float ms = 0.0f;
+  HIPCHK(ctx, hipEventElapsedTime(&ms, progs[0]->ev0, progs[0]->ev1));
+  if (stats) {
+    stats->ms_total = ms;
+    stats->ms_trace = ms;
+    for (int j = 0; j < njobs; ++j) {
+      long long sk = 0;
+      HIPCHK(ctx, hipMemcpyAsync(&sk, states[j]->tileCnt + C_TRACE_STEPS, 8, hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+      stats->trace_steps += sk;
+      stats->paths += (long long)frames[j] * progs[j]->spf * states[j]->npix;
+      stats->trace_launches += unit0[j] ? frames[j] : 1;
+    }
+  }
   return SNL_OK;
 }
 

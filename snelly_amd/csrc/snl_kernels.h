@@ -550,7 +550,7 @@ __device__ inline void shade_hit(const SnlKArgs& a) {
           }
         }
       }
-      if (alive && !(maxComponent(throughput) < THROUGHPUT_EPSILON) && vertex + 1 <= SNL_MAX_BOUNCES) {
+      if (alive && !(maxComponent(throughput) < THROUGHPUT_EPSILON) && vertex + 1 <= a.maxBounces) {
         cont = true;
         flags = (flags & ~(0xff << PF_VSHIFT)) | ((vertex + 1) << PF_VSHIFT);
       }

@@ -1,12 +1,15 @@
-// snl_persist.h -- persistent path kernel: one launch renders one frame of a
-// range of pixels with no per-bounce barrier.
+// snl_persist.h -- persistent path kernel: one launch renders `frames` frames
+// of a range of pixels with no per-bounce or per-frame barrier.
 //
 // Every lane runs whole camera paths (cameraPath, pathtracer-fragment-shader.glsl
 // 1625-1926) as a small state machine held in registers:
 //   CAMERA -> TRACE (continuation ray) -> SHADE -> TRACE (shadow rays of the
 //   vertex's NEE, one after another) -> TRACE (next continuation) ... -> DONE
-// and takes a new pixel from the tile's queue as soon as its pixel's frame is
-// accumulated.  A wave advances all its tracing lanes by one SDF step per
+// and takes a new pixel from the tile's queue once its pixel has run all its
+// frames.  A pixel's frames run back to back in one lane, its RNG state and
+// running mean held in registers: the per-pixel sequence of operations is the
+// same as one launch per frame (pixels never read each other's state), so the
+// result is too.  A wave advances all its tracing lanes by one SDF step per
 // iteration; lanes that need shading wait until at least `shadeThresh` of them
 // do (or nothing is tracing), so the expensive shading code runs with most
 // lanes active while marching keeps the rest busy.  Random numbers are drawn
@@ -30,7 +33,8 @@ __device__ inline void persist_body(const SnlKArgs& a) {
   int rb = 0, re = 0;
   bool exhausted = false;
   int st = PS_IDLE;
-  int pix = 0, sample = 0;
+  int pix = 0, sample = 0, frame = 0;
+  vec4 accum(0.0f);  // the pixel's (X, Y, Z, N) over its frames
   // path state
   vec3 o(0.0f), d(0.0f), hp(0.0f);
   RT tp = rt1(1.0f), L = rt1(0.0f), Ls = rt1(0.0f);
@@ -94,6 +98,7 @@ __device__ inline void persist_body(const SnlKArgs& a) {
       if (st == PS_IDLE && rank < avail) {
         pix = a.itemBase + rb + rank;
         sample = 0;
+        frame = 0;
         st = PS_CAMERA;
       }
       rb += min(avail, (int)__popcll(idleMask));
@@ -291,7 +296,7 @@ __device__ inline void persist_body(const SnlKArgs& a) {
           misSph = powerHeuristic(bsdfPdf, sphPdf);
           o = pW;
           contDir = rayDir;
-          contPending = !(maxComponent(tp) < THROUGHPUT_EPSILON) && vertex + 1 <= SNL_MAX_BOUNCES;
+          contPending = !(maxComponent(tp) < THROUGHPUT_EPSILON) && vertex + 1 <= a.maxBounces;
           if (contPending) ++vertex;
           if (neeMask) startLight(__ffs(neeMask) - 1);
           else if (contPending) { contPending = false; startRay(contDir, P.maxLengthScale, false); }
@@ -314,19 +319,27 @@ __device__ inline void persist_body(const SnlKArgs& a) {
 #else
           vec3 colorXYZ = rgbToXyz(Lf);
 #endif
-          vec4 old = ld4(a.radiance, pix);
-          float oldN = old.w, newN = oldN + 1.0f;
-          st4(a.radiance, pix, vec4((oldN * vec3(old) + colorXYZ) / newN, newN));
+          float oldN = accum.w, newN = oldN + 1.0f;
+          accum = vec4((oldN * vec3(accum) + colorXYZ) / newN, newN);
         }
-        st4(a.rngstate, pix, rnd);
-        st = PS_IDLE;
+        if (++frame < a.frames) {
+          sample = 0;
+          st = PS_CAMERA;
+        } else {
+          st4(a.radiance, pix, accum);
+          st4(a.rngstate, pix, rnd);
+          st = PS_IDLE;
+        }
       }
     }
 
     if (st == PS_CAMERA) {
       // pathtrace() up to the first traceRay (1934-1970)
       if (sample == 0) {
-        rnd = ld4(a.rngstate, pix);
+        if (frame == 0) {
+          rnd = ld4(a.rngstate, pix);
+          accum = ld4(a.radiance, pix);
+        }
         skip = false;
 #ifdef INTERACTIVE_MODE
         if (snl_rand(rnd) < P.skipProbability) skip = true;
@@ -347,8 +360,14 @@ __device__ inline void persist_body(const SnlKArgs& a) {
         }
       }
       if (skip) {
-        st4(a.rngstate, pix, rnd);
-        st = PS_IDLE;
+        // a skipped frame leaves the pixel's mean as it is (1994-2004)
+        if (++frame < a.frames) {
+          sample = 0;
+        } else {
+          st4(a.radiance, pix, accum);
+          st4(a.rngstate, pix, rnd);
+          st = PS_IDLE;
+        }
       } else {
         vec2 pixel(float(pix % a.W) + 0.5f, float(pix / a.W) + 0.5f);
         float jx = 0.5f * P.filterRadius * sample_jitter(snl_rand(rnd));

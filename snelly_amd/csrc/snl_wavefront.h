@@ -47,6 +47,8 @@ struct SnlKArgs {
   int chunk;          // trace refill chunk per wave
   int itemBase, itemEnd;  // persistent kernel: pixel range of this tile
   int shadeThresh;        // persistent kernel: lanes waiting before a wave shades
+  int frames;             // persistent kernel: frames each pixel runs before the lane moves on
+  int maxBounces;         // __MAX_BOUNCES__: the last vertex index a path may reach
 };
 
 #define SNL_PF(k, i) a.pf[(long long)(k) * a.cap + (i)]

@@ -148,7 +148,7 @@ class TabulatedMetal(Metal):
     reference binds k under the name "k_tex" while the shader samples "kTex"
     (js/materials.js:665), so K_METAL reads texture unit 0; this host keeps that
     binding: the runtime points a sampler the host never bound at a per-frame
-    snapshot of the radiance buffer, as GL texture unit 0 holds it)."""
+    snapshot of the radiance buffer, as GL texture unit 0 holds it."""
 
     def __init__(self, name, n, k):
         super().__init__(name)

@@ -22,7 +22,7 @@ EXPORTS = [
     "snl_uniform_texture", "snl_texture_create", "snl_texture_upload", "snl_texture_destroy",
     "snl_state_create", "snl_state_destroy", "snl_state_seed", "snl_state_clear", "snl_state_read",
     "snl_state_write", "snl_render", "snl_render_normals", "snl_pick", "snl_tonemap", "snl_sync",
-    "snl_get_stats", "snl_set_tuning", "snl_set_profiling", "snl_set_engine",
+    "snl_get_stats", "snl_set_tuning", "snl_set_profiling", "snl_set_engine", "snl_render_batch",
 ]
 ENGINE_WAVEFRONT, ENGINE_PERSISTENT = 0, 1
 
@@ -76,6 +76,8 @@ def lib():
         "snl_set_tuning": (ip, [vp, ip, ip, ip]),
         "snl_set_profiling": (ip, [vp, ip]),
         "snl_set_engine": (ip, [vp, ip, ip, ip, ip, ip]),
+        "snl_render_batch": (ip, [ip, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(ip),
+                                  ctypes.POINTER(Stats)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -93,6 +95,19 @@ def check(rc, ctx=None):
     if rc != 0:
         msg = lib().snl_last_error(ctx).decode(errors="replace")
         raise SnellyError(f"snelly error {rc}: {msg}")
+
+
+def render_batch(jobs):
+    """snl_render_batch over [(Program, State, frames), ...]; returns the Stats"""
+    n = len(jobs)
+    progs = (ctypes.c_void_p * n)(*[p.h for p, _, _ in jobs])
+    states = (ctypes.c_void_p * n)(*[s.h for _, s, _ in jobs])
+    frames = (ctypes.c_int * n)(*[int(f) for _, _, f in jobs])
+    st = Stats()
+    if n == 0:
+        return st
+    jobs[0][0].ctx.check(lib().snl_render_batch(n, progs, states, frames, ctypes.byref(st)))
+    return st
 
 
 def precompile(shader, ior, defines, max_march, max_bounces, max_atm, spf, flags=0):

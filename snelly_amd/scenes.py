@@ -231,3 +231,103 @@ def mandelbox_dielectric(width=1920, height=1080, spp=256, scale=2.5, iterations
     s.user_uniforms = {"Scale": float(scale), "MinRad2": 0.25}
     s.oracle_sdfs = [(0, 4, float(scale), float(iterations), float(bailout), 0.25), (2, 1, 0.5, 0, 0, 0)]
     return s
+
+
+# ---------------------------------------------------------------- batched workload
+MANDELBOX_SDF_GLSL = """
+uniform float Scale;
+uniform float MinRad2;
+float mandelbox(vec3 pos)
+{
+    vec4 scale = vec4(Scale, Scale, Scale, abs(Scale)) / MinRad2;
+    float absScalem1 = abs(Scale - 1.0);
+    float AbsScaleRaisedTo1mIters = pow(abs(Scale), float(1-12));
+    vec4 p = vec4(pos,1);
+    vec4 p0 = p;
+    for (int i=0; i<12; i++)
+    {
+        p.xyz = clamp(p.xyz, -1.0, 1.0)*2.0 - p.xyz;
+        float r2 = dot(p.xyz, p.xyz);
+        p *= clamp(max(MinRad2/r2, MinRad2), 0.0, 1.0);
+        p = p*scale + p0;
+        if (r2>10.0) break;
+    }
+    return ((length(p.xyz) - absScalem1) / p.w - AbsScaleRaisedTo1mIters);
+}
+float SDF_@MAT@(vec3 X) { return mandelbox(X); }
+"""
+
+BATCH_FAMILIES = ["sphere", "torus", "menger2", "menger3", "menger4", "menger5", "mandelbox"]
+BATCH_MATERIALS = ["METAL", "DIELECTRIC", "SURFACE"]
+
+
+def batch_job(family, material, side, spp, bounces, rs):
+    """One job of the batched workload: `family` in BATCH_FAMILIES as `material`,
+    a side x side image, `spp` frames, `bounces`; other parameters from rs."""
+    if family == "sphere":
+        glsl, extent, flops = SPHERE_GLSL, 1.0, 8 + 6
+    elif family == "torus":
+        glsl, extent, flops = TORUS_GLSL, 1.3, None
+    elif family.startswith("menger"):
+        n = int(family[-1])
+        glsl = MENGER_GLSL.replace("__ITER__", str(n)).replace("float SDF_METAL(in vec3 X)", "float SDF_@MAT@(in vec3 X)")
+        extent, flops = 1.0, 8 + MENGER_SDF_FLOPS[n]
+    else:
+        glsl, extent, flops = MANDELBOX_SDF_GLSL, None, None
+    glsl = glsl.replace("@MAT@", material)
+    s = Scene(f"{family}_{material.lower()}", glsl, side, side, spp)
+    r = s.renderer
+    r.dispersive = True
+    r.maxBounces = bounces
+    r.maxMarchSteps = 256
+    r.skyPower = 0.0
+    r.sunPower = -7.0
+    s.camera.fov = 45.0
+    s.camera.aperture = -math.inf
+    if family.startswith("menger"):
+        s.user_uniforms = {"size": 1.0, "scale": 1.0}
+    if family == "mandelbox":
+        # |scale| < 1.5 gives degenerate boxes; draw again
+        sc = 0.0
+        while abs(sc) < 1.5:
+            sc = float(rs.uniform(-2.0, 2.5))
+        s.user_uniforms = {"Scale": sc, "MinRad2": 0.25}
+        extent = 2.0 * (abs(sc) + 1.0) / (abs(sc) - 1.0) if sc > 0 else 2.0
+        r.maxMarchSteps = 512
+    d = 2.6 * extent
+    s.camera.position = [0.55 * d, 0.45 * d, 0.75 * d]
+    s.camera.target = [0.0, 0.0, 0.0]
+    if material == "METAL":
+        m = host.GlslMetal("analytic", host.ANALYTIC_METAL_GLSL)
+        m.roughness = float(rs.uniform(0.02, 0.2))
+        s.materials.metal = m
+    elif material == "DIELECTRIC":
+        dl = host.CauchyDielectric("cauchy", [1.5, 0.0042, -2.0])
+        dl.roughness = float(rs.uniform(0.005, 0.05))
+        dl.absorptionScale = 1.0
+        dl.absorptionColor = [0.9, 0.7, 0.5]
+        s.materials.dielectric = dl
+    else:
+        su = s.materials.surface
+        su.diffuseAlbedo = [float(x) for x in rs.uniform(0.3, 0.9, 3)]
+        su.specAlbedo = [0.2, 0.2, 0.2]
+        su.ior = 1.5
+        su.roughness = float(rs.uniform(0.05, 0.3))
+    s.flops_per_step = flops
+    return s
+
+
+def batch_jobs(n=64, seed=2024, min_side=128, max_side=1024):
+    """The batched workload: n scenes drawn from a fixed seed -- family in
+    BATCH_FAMILIES, material Metal / Dielectric / Surface, side log-uniform in
+    [min_side, max_side], spp in {4, 16, 64}, bounces 4..12."""
+    rs = np.random.RandomState(seed)
+    jobs = []
+    for _ in range(n):
+        family = BATCH_FAMILIES[rs.randint(len(BATCH_FAMILIES))]
+        material = BATCH_MATERIALS[rs.randint(len(BATCH_MATERIALS))]
+        side = int(round(math.exp(rs.uniform(math.log(min_side), math.log(max_side)))))
+        spp = int([4, 16, 64][rs.randint(3)])
+        bounces = int(rs.randint(4, 13))
+        jobs.append(batch_job(family, material, side, spp, bounces, rs))
+    return jobs

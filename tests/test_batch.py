@@ -1,0 +1,109 @@
+"""Batched rendering (snl_render_batch) and the persistent engine's in-lane
+frames.
+
+Per job, a batch must produce exactly what the job produces alone: pixels
+never read each other's state, so neither the other jobs in the batch nor the
+order in which lanes take pixels may change a single bit (the alone-vs-mix
+invariance).  The same holds between one multi-frame launch and one launch per
+frame.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from snelly_amd import host, native, scenes
+
+
+def small_jobs():
+    """a mixed batch of small images: every family, all three materials"""
+    rs = np.random.RandomState(7)
+    jobs = []
+    for k, fam in enumerate(scenes.BATCH_FAMILIES):
+        mat = scenes.BATCH_MATERIALS[k % 3]
+        side = [24, 40, 33, 64, 17, 48, 29][k]
+        jobs.append(scenes.batch_job(fam, mat, side, [2, 3, 4][k % 3], 3 + k % 4, rs))
+    return jobs
+
+
+def test_batch_workload_is_deterministic():
+    a, b = scenes.batch_jobs(), scenes.batch_jobs()
+    assert len(a) == 64
+    for x, y in zip(a, b):
+        assert (x.name, x.width, x.spp, x.renderer.maxBounces, x.user_uniforms) == \
+               (y.name, y.width, y.spp, y.renderer.maxBounces, y.user_uniforms)
+        assert 128 <= x.width <= 1024 and x.spp in (4, 16, 64) and 4 <= x.renderer.maxBounces <= 12
+
+
+def test_batch_rejects_bad_arguments():
+    L = native.lib()
+    st = native.Stats()
+    assert L.snl_render_batch(-1, None, None, None, ctypes.byref(st)) != 0
+    assert L.snl_render_batch(0, None, None, None, ctypes.byref(st)) == 0
+    null = (ctypes.c_void_p * 1)(None)
+    fr = (ctypes.c_int * 1)(1)
+    assert L.snl_render_batch(1, null, null, fr, ctypes.byref(st)) != 0
+
+
+def run_alone(ctx, s, seed, per_frame=False):
+    sess = host.Session(s, s.width, s.height, ctx, seed=seed, flags=native.BUILD_PRECISE)
+    if per_frame:
+        for _ in range(s.spp):
+            sess.render(1)
+    else:
+        sess.render(s.spp)
+    out = sess.radiance().copy(), sess.state.read(1).copy()
+    sess.close()
+    return out
+
+
+@pytest.mark.gpu
+def test_batch_equals_alone(ctx):
+    jobs = small_jobs()
+    sessions = [host.Session(s, s.width, s.height, ctx, seed=100 + j, flags=native.BUILD_PRECISE)
+                for j, s in enumerate(jobs)]
+    st = native.render_batch([(se.prog, se.state, s.spp) for se, s in zip(sessions, jobs)])
+    assert st.paths == sum(s.width * s.height * s.spp for s in jobs)
+    for j, (se, s) in enumerate(zip(sessions, jobs)):
+        rad, rng = se.radiance(), se.state.read(1)
+        alone_rad, alone_rng = run_alone(ctx, s, 100 + j)
+        assert np.array_equal(rad, alone_rad), s.name
+        assert np.array_equal(rng, alone_rng), s.name
+        assert np.all(rad.reshape(-1, 4)[:, 3] == s.spp + 1)
+    for se in sessions:
+        se.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("engine", [native.ENGINE_PERSISTENT, native.ENGINE_WAVEFRONT], ids=["persistent", "wavefront"])
+def test_multi_frame_launch_equals_per_frame(ctx, engine):
+    s = scenes.menger_metal(48, 40, 5, iterations=3)
+    a = host.Session(s, 48, 40, ctx, seed=9, flags=native.BUILD_PRECISE)
+    a.prog.engine(engine)
+    a.render(5)
+    b = host.Session(s, 48, 40, ctx, seed=9, flags=native.BUILD_PRECISE)
+    b.prog.engine(native.ENGINE_PERSISTENT)
+    for _ in range(5):
+        b.render(1)
+    assert np.array_equal(a.radiance(), b.radiance())
+    assert np.array_equal(a.state.read(1), b.state.read(1))
+    a.close()
+    b.close()
+
+
+@pytest.mark.gpu
+def test_batch_with_unit0_sampler_job(ctx):
+    """a tabulated metal (K_METAL on texture unit 0) needs frame order; in a
+    batch it still renders exactly as alone"""
+    t = scenes.sphere_plane(32, 24, 3)
+    t.glsl = scenes.TORUS_GLSL.replace("@MAT@", "METAL")
+    t.materials.metal = host.metal("Gold")
+    other = scenes.batch_job("menger3", "SURFACE", 40, 3, 4, np.random.RandomState(1))
+    sa = host.Session(t, 32, 24, ctx, seed=5, flags=native.BUILD_PRECISE)
+    sb = host.Session(other, 40, 40, ctx, seed=6, flags=native.BUILD_PRECISE)
+    native.render_batch([(sa.prog, sa.state, 3), (sb.prog, sb.state, 3)])
+    ra, rb = sa.radiance(), sb.radiance()
+    sa.close()
+    sb.close()
+    assert np.array_equal(ra, run_alone(ctx, t, 5)[0])
+    assert np.array_equal(rb, run_alone(ctx, other, 6)[0])

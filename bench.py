@@ -23,6 +23,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# hardware queues of this process: the batch runs up to this many jobs'
+# launches at once (set before the HIP runtime starts)
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
 METRIC = "path samples/s (plain vs batched/served) at RMSE≤1%; p50/p99 job latency"
 HBM_PEAK_GBS = 8000.0
@@ -101,7 +104,7 @@ def batched(ctx, steps, warmup):
             "plain_path_samples_per_s": round(samples / plain_s, 1), "plain_s": round(plain_s, 3),
             "batched_path_samples_per_s": round(samples / batch_s, 1), "batched_s": round(batch_s, 3),
             "batched_gpu_ms": round(sum(gpu_ms) / len(gpu_ms), 2), "speedup_vs_plain": round(plain_s / batch_s, 2),
-            "build": "fast", "jobs_by_scene": fams}
+            "build": "fast", "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")), "jobs_by_scene": fams}
 
 
 def main():

@@ -136,8 +136,10 @@ int snl_set_engine(snl_prog prog, int engine, int tiles, int persist_blocks, int
 
 /* Batched rendering: frames[j] frames of every job j = (progs[j], states[j]),
  * each with its program's current uniforms, submitted together -- the jobs'
- * persistent launches are spread over the context's streams, largest first,
- * so that many small images fill the device at once.  The reference renders
+ * persistent launches are spread over the context's batch streams
+ * (SNL_BATCH_STREAMS, default 16), longest first, so that many small images
+ * fill the device at once; they run concurrently up to the process's hardware
+ * queue count (GPU_MAX_HW_QUEUES).  The reference renders
  * one scene per canvas per frame (js/renderer.js:462-644); per job the result
  * is identical to snl_render(progs[j], states[j], frames[j]).  All programs
  * belong to one context; every job has its own state.  Blocks until the batch

@@ -43,6 +43,11 @@ struct snl_ctx_s {
   hipStream_t tileStream[SNL_MAX_TILES] = {};  // default-priority streams for pipelined pixel tiles
   hipEvent_t tileEvent[SNL_MAX_TILES] = {};
   hipEvent_t joinEvent = nullptr;
+  // batch streams (default priority), created on first use; their count is
+  // SNL_BATCH_STREAMS (default 16).  Kernels on them run concurrently up to
+  // the process's hardware queue count (GPU_MAX_HW_QUEUES).
+  std::vector<hipStream_t> batchStream;
+  std::vector<hipEvent_t> batchEvent;
   std::string err;
   int numCU = 256;
 };
@@ -354,6 +359,8 @@ int snl_ctx_destroy(snl_ctx ctx) {
     if (ctx->tileEvent[k]) hipEventDestroy(ctx->tileEvent[k]);
   }
   if (ctx->joinEvent) hipEventDestroy(ctx->joinEvent);
+  for (auto& st : ctx->batchStream) hipStreamDestroy(st);
+  for (auto& ev : ctx->batchEvent) hipEventDestroy(ev);
   delete ctx;
   return SNL_OK;
 }
@@ -839,22 +846,39 @@ int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, 
     z.cnt = states[j]->tileCnt;
     if ((rc = launch(ctx, progs[j]->k_reset, 1, 64, z))) return rc;
   }
-  // largest job first onto the least loaded stream (work ~ pixels * frames)
+  if (ctx->batchStream.empty()) {
+    int n = 16;
+    if (const char* e = getenv("SNL_BATCH_STREAMS")) n = std::max(1, std::min(32, atoi(e)));
+    ctx->batchStream.resize(n);
+    ctx->batchEvent.resize(n);
+    for (int k = 0; k < n; ++k) {
+      HIPCHK(ctx, hipStreamCreateWithFlags(&ctx->batchStream[k], hipStreamNonBlocking));
+      HIPCHK(ctx, hipEventCreateWithFlags(&ctx->batchEvent[k], hipEventDisableTiming));
+    }
+  }
+  // Longest job first onto the least loaded stream.  A job lasts about the
+  // larger of its throughput time (all its path samples at a device-wide rate)
+  // and its latency (its frames run back to back in one lane: frames times a
+  // deep path's time, ~bounces); both scales are coarse and only order jobs.
   std::vector<int> order(njobs);
   for (int j = 0; j < njobs; ++j) order[j] = j;
-  auto work = [&](int j) { return (double)states[j]->npix * frames[j] * progs[j]->spf; };
+  auto work = [&](int j) {
+    double samples = (double)states[j]->npix * frames[j] * progs[j]->spf;
+    double latency = (double)frames[j] * progs[j]->spf * (progs[j]->maxBounces + 1) * 4.0e5;
+    return std::max(samples, latency);
+  };
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
-  const int NS = SNL_MAX_TILES;
-  double load[NS] = {};
+  const int NS = (int)ctx->batchStream.size();
+  std::vector<double> load(NS, 0.0);
   HIPCHK(ctx, hipEventRecord(progs[0]->ev0, ctx->stream));
   HIPCHK(ctx, hipEventRecord(ctx->joinEvent, ctx->stream));
-  for (int k = 0; k < NS; ++k) HIPCHK(ctx, hipStreamWaitEvent(ctx->tileStream[k], ctx->joinEvent, 0));
+  for (int k = 0; k < NS; ++k) HIPCHK(ctx, hipStreamWaitEvent(ctx->batchStream[k], ctx->joinEvent, 0));
   for (int j : order) {
-    int k = (int)(std::min_element(load, load + NS) - load);
+    int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
     load[k] += work(j);
     snl_prog p = progs[j];
     snl_state st = states[j];
-    hipStream_t ts = ctx->tileStream[k];
+    hipStream_t ts = ctx->batchStream[k];
     SnlKArgs a;
     fillArgs(p, st, a);
     a.chunk = p->persistChunk;
@@ -875,8 +899,8 @@ int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, 
     }
   }
   for (int k = 0; k < NS; ++k) {
-    HIPCHK(ctx, hipEventRecord(ctx->tileEvent[k], ctx->tileStream[k]));
-    HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->tileEvent[k], 0));
+    HIPCHK(ctx, hipEventRecord(ctx->batchEvent[k], ctx->batchStream[k]));
+    HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->batchEvent[k], 0));
   }
   HIPCHK(ctx, hipEventRecord(progs[0]->ev1, ctx->stream));
   HIPCHK(ctx, hipEventSynchronize(progs[0]->ev1));

@@ -7,10 +7,12 @@ samples per pixel from a freshly seeded accumulation state, i.e. 67,108,864
 path samples.  Scene data are synthetic (an analytic SDF); nothing is read from
 disk.  Prints one JSON line.
 
-Also measured (the "batched" object): the 64-scene batched workload rendered
-plain (one launch per scene and frame) and as one batch.
+Also measured: the 64-scene batched workload rendered plain (one launch per
+scene and frame) and as one batch ("batched"), and a served run -- jobs
+arriving at 20/s, rendered to a target error by continuous batching, with
+p50/p99 time from arrival to target ("served").
 
-  python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--cpu-seconds 15] [--no-batch]
+  python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--cpu-seconds 15] [--no-batch] [--served-jobs 200]
 """
 import argparse
 import json
@@ -107,6 +109,24 @@ def batched(ctx, steps, warmup):
             "build": "fast", "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")), "jobs_by_scene": fams}
 
 
+def served(ctx, n_jobs):
+    """The served workload (scenes.served_jobs): jobs arriving as a Poisson
+    process at 20/s, each rendered until its estimated relative RMS error
+    reaches its target (5-20%), by the continuous-batching server."""
+    from snelly_amd import scenes, serve
+    arrivals = scenes.served_jobs(n_jobs)
+    srv = serve.Server(ctx)
+    t = time.perf_counter()
+    done, left = srv.run(arrivals, time_limit=90.0)
+    wall = time.perf_counter() - t
+    rep = serve.Server.report(done, wall, srv.samples)
+    rep.update({"workload": "served_poisson20_mixed_sdf", "jobs": len(arrivals), "unfinished": len(left),
+                "arrival_rate_per_s": 20.0, "sides": [64, 512], "target_rrmse": [0.05, 0.2], "wall_s": round(wall, 2),
+                "ticks": srv.ticks, "gpu_busy_frac": round(srv.gpu_ms * 1e-3 / wall, 3),
+                "error_estimate": "two independently seeded accumulations, ||A-B|| / (2 ||(A+B)/2||), on the GPU"})
+    return rep
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +138,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--served-jobs", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the path is per-image data parallel "
@@ -201,6 +222,8 @@ def main():
     sess.close()
     if not args.no_batch:
         out["batched"] = batched(ctx, max(1, min(args.steps, 3)), min(args.warmup, 1))
+    if args.served_jobs > 0:
+        out["served"] = served(ctx, args.served_jobs)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scenes.menger_metal(W, H, spp), W, H, spp, args.cpu_seconds)
     print(json.dumps(out), flush=True)

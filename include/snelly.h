@@ -148,6 +148,22 @@ int snl_set_engine(snl_prog prog, int engine, int tiles, int persist_blocks, int
 int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, const int* frames,
                      snl_stats* stats);
 
+/* Served rendering support (no counterpart in the reference, which renders one
+ * interactive canvas).  A server renders a job into two states seeded
+ * differently and reads its error from them:
+ *   out[0] = sum over pixels and X,Y,Z of (ua - ub)^2,
+ *   out[1] = sum of ((ua + ub) / 2)^2,
+ * where ua, ub are the unbiased means XYZ * N / (N - 1) of the two states.
+ * The relative RMS error of the averaged image is about sqrt(out[0]) / 2 /
+ * sqrt(out[1]).  Computed on the device; only the two sums are copied back. */
+int snl_state_compare(snl_state a, snl_state b, double* out);
+/* free and total device memory of the context's GPU (hipMemGetInfo), for
+ * admission control */
+int snl_ctx_mem_info(snl_ctx ctx, unsigned long long* free_bytes, unsigned long long* total_bytes);
+/* device bytes of a width x height state rendered by the persistent engine or
+ * snl_render_batch (the wavefront engine adds its path buffers on first use) */
+unsigned long long snl_state_bytes(int width, int height);
+
 #ifdef __cplusplus
 }
 #endif

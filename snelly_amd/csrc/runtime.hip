@@ -258,7 +258,7 @@ void fillArgs(snl_prog p, snl_state st, SnlKArgs& a) {
   a.cap = cap;
   a.pf = st->pf;
   a.pi = st->pi;
-  a.qmat = st->q + 4 * cap;
+  a.qmat = st->q ? st->q + 4 * cap : nullptr;
   a.cnt = st->cnt;
   a.radiance = st->radiance;
   a.rngstate = st->rng;
@@ -577,11 +577,11 @@ int snl_state_create(snl_ctx ctx, int w, int h, snl_state* out) {
   s->npix = (long long)w * h;
   long long n = s->npix;
   HIPCHK(ctx, hipSetDevice(ctx->device));
+  // radiance and RNG (32 B per pixel) plus counters; the wavefront engine's
+  // path and queue buffers are allocated on its first use
   struct { void** p; size_t bytes; } allocs[] = {
       {(void**)&s->radiance, (size_t)n * 16}, {(void**)&s->rng, (size_t)n * 16},
-      {(void**)&s->pf, (size_t)n * F_COUNT * 4}, {(void**)&s->pi, (size_t)n * I_COUNT * 4},
-      {(void**)&s->q, (size_t)n * 8 * 4}, {(void**)&s->cnt, (size_t)C_COUNT * 4},
-      {(void**)&s->tileCnt, (size_t)SNL_MAX_TILES * C_COUNT * 4}};
+      {(void**)&s->cnt, (size_t)C_COUNT * 4}, {(void**)&s->tileCnt, (size_t)SNL_MAX_TILES * C_COUNT * 4}};
   for (auto& a : allocs) {
     if (hipMalloc(a.p, a.bytes) != hipSuccess) {
       for (auto& b : allocs) if (*b.p) hipFree(*b.p);
@@ -590,7 +590,6 @@ int snl_state_create(snl_ctx ctx, int w, int h, snl_state* out) {
   }
   HIPCHK(ctx, hipMemsetAsync(s->cnt, 0, C_COUNT * 4, ctx->stream));
   HIPCHK(ctx, hipMemsetAsync(s->tileCnt, 0, SNL_MAX_TILES * C_COUNT * 4, ctx->stream));
-  HIPCHK(ctx, hipMemsetAsync(s->pi, 0, (size_t)n * I_COUNT * 4, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   *out = s.release();
   return snl_state_seed(*out, 0);
@@ -605,6 +604,40 @@ int snl_state_destroy(snl_state s) {
     if (p) hipFree(p);
   delete s;
   return SNL_OK;
+}
+
+// Sums over pixels of two accumulations a, b of the same image: the unbiased
+// means u = XYZ * N / (N - 1) (the count starts at 1, js/renderer.js:735-745),
+// out[0] += sum |ua - ub|^2, out[1] += sum |(ua + ub) / 2|^2.  One partial per
+// block, summed in a second pass so the result does not depend on block order.
+__global__ void compare_kernel(const float4* a, const float4* b, long long npix, double* partial) {
+  __shared__ double s0[256], s1[256];
+  double d2 = 0.0, m2 = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += (long long)gridDim.x * blockDim.x) {
+    float4 x = a[i], y = b[i];
+    double fx = x.w > 1.0f ? (double)x.w / (x.w - 1.0) : 0.0, fy = y.w > 1.0f ? (double)y.w / (y.w - 1.0) : 0.0;
+    double ux[3] = {x.x * fx, x.y * fx, x.z * fx}, uy[3] = {y.x * fy, y.y * fy, y.z * fy};
+    for (int c = 0; c < 3; ++c) {
+      double d = ux[c] - uy[c], m = 0.5 * (ux[c] + uy[c]);
+      d2 += d * d;
+      m2 += m * m;
+    }
+  }
+  s0[threadIdx.x] = d2;
+  s1[threadIdx.x] = m2;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) { s0[threadIdx.x] += s0[threadIdx.x + k]; s1[threadIdx.x] += s1[threadIdx.x + k]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { partial[2 * blockIdx.x] = s0[0]; partial[2 * blockIdx.x + 1] = s1[0]; }
+}
+__global__ void compare_sum_kernel(const double* partial, int n, double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double d2 = 0.0, m2 = 0.0;
+  for (int k = 0; k < n; ++k) { d2 += partial[2 * k]; m2 += partial[2 * k + 1]; }
+  out[0] = d2;
+  out[1] = m2;
 }
 
 static int seedMode(snl_state s, unsigned seed, int mode) {
@@ -665,9 +698,27 @@ static int copyUnit0(snl_ctx ctx, snl_state st, hipStream_t s) {
   return SNL_OK;
 }
 
+static int wavefrontBuffers(snl_ctx ctx, snl_state st) {
+  if (st->pf) return SNL_OK;
+  long long n = st->npix;
+  struct { void** p; size_t bytes; } allocs[] = {
+      {(void**)&st->pf, (size_t)n * F_COUNT * 4}, {(void**)&st->pi, (size_t)n * I_COUNT * 4},
+      {(void**)&st->q, (size_t)n * 8 * 4}};
+  for (auto& a : allocs) {
+    if (hipMalloc(a.p, a.bytes) != hipSuccess) {
+      for (auto& b : allocs) if (*b.p) { hipFree(*b.p); *b.p = nullptr; }
+      return fail(ctx, SNL_E_NOMEM, "wavefront buffers of %zu bytes: allocation failed", a.bytes);
+    }
+  }
+  HIPCHK(ctx, hipMemsetAsync(st->pi, 0, (size_t)n * I_COUNT * 4, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return SNL_OK;
+}
+
 static int renderWavefront(snl_prog p, snl_state st, int frames) {
   snl_ctx ctx = p->ctx;
   if (st->ctx != ctx) return fail(ctx, SNL_E_ARG, "program and state belong to different contexts");
+  if (int rc0 = wavefrontBuffers(ctx, st)) return rc0;
   bool unit0 = false;
   int rc = prepareUnit0(p, st, unit0);
   if (rc) return rc;
@@ -814,6 +865,38 @@ static int renderPersistent(snl_prog p, snl_state st, int frames) {
   p->stats.paths = (long long)frames * p->spf * st->npix;
   p->stats.trace_launches = unit0 ? frames * K : K;
   return SNL_OK;
+}
+
+int snl_state_compare(snl_state a, snl_state b, double* out) {
+  if (!a || !b || !out) return fail(a ? a->ctx : nullptr, SNL_E_ARG, "NULL argument");
+  if (a->ctx != b->ctx || a->npix != b->npix) return fail(a->ctx, SNL_E_ARG, "states differ in context or size");
+  snl_ctx ctx = a->ctx;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const int blocks = 256;
+  double* d = nullptr;
+  HIPCHK(ctx, hipMallocAsync((void**)&d, (2 * blocks + 2) * sizeof(double), ctx->stream));
+  hipLaunchKernelGGL(compare_kernel, dim3(blocks), dim3(256), 0, ctx->stream, reinterpret_cast<const float4*>(a->radiance),
+                     reinterpret_cast<const float4*>(b->radiance), a->npix, d);
+  hipLaunchKernelGGL(compare_sum_kernel, dim3(1), dim3(64), 0, ctx->stream, d, blocks, d + 2 * blocks);
+  HIPCHK(ctx, hipGetLastError());
+  HIPCHK(ctx, hipMemcpyAsync(out, d + 2 * blocks, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipFreeAsync(d, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return SNL_OK;
+}
+
+int snl_ctx_mem_info(snl_ctx ctx, unsigned long long* free_bytes, unsigned long long* total_bytes) {
+  if (!ctx) return fail(nullptr, SNL_E_ARG, "ctx is NULL");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  size_t f = 0, t = 0;
+  HIPCHK(ctx, hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return SNL_OK;
+}
+
+unsigned long long snl_state_bytes(int width, int height) {
+  return (unsigned long long)width * height * 32ull + (unsigned long long)(SNL_MAX_TILES + 1) * C_COUNT * 4ull;
 }
 
 int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, const int* frames,

@@ -23,6 +23,7 @@ EXPORTS = [
     "snl_state_create", "snl_state_destroy", "snl_state_seed", "snl_state_clear", "snl_state_read",
     "snl_state_write", "snl_render", "snl_render_normals", "snl_pick", "snl_tonemap", "snl_sync",
     "snl_get_stats", "snl_set_tuning", "snl_set_profiling", "snl_set_engine", "snl_render_batch",
+    "snl_state_compare", "snl_ctx_mem_info", "snl_state_bytes",
 ]
 ENGINE_WAVEFRONT, ENGINE_PERSISTENT = 0, 1
 
@@ -78,6 +79,9 @@ def lib():
         "snl_set_engine": (ip, [vp, ip, ip, ip, ip, ip]),
         "snl_render_batch": (ip, [ip, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(ip),
                                   ctypes.POINTER(Stats)]),
+        "snl_state_compare": (ip, [vp, vp, ctypes.POINTER(ctypes.c_double)]),
+        "snl_ctx_mem_info": (ip, [vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_ulonglong)]),
+        "snl_state_bytes": (ctypes.c_ulonglong, [ip, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -95,6 +99,23 @@ def check(rc, ctx=None):
     if rc != 0:
         msg = lib().snl_last_error(ctx).decode(errors="replace")
         raise SnellyError(f"snelly error {rc}: {msg}")
+
+
+def compare_states(a, b):
+    """(sum |ua - ub|^2, sum |(ua + ub)/2|^2) over the two states' unbiased means"""
+    out = (ctypes.c_double * 2)()
+    a.ctx.check(lib().snl_state_compare(a.h, b.h, out))
+    return out[0], out[1]
+
+
+def mem_info(ctx):
+    f, t = ctypes.c_ulonglong(), ctypes.c_ulonglong()
+    ctx.check(lib().snl_ctx_mem_info(ctx.h, ctypes.byref(f), ctypes.byref(t)))
+    return f.value, t.value
+
+
+def state_bytes(width, height):
+    return int(lib().snl_state_bytes(width, height))
 
 
 def render_batch(jobs):

@@ -331,3 +331,21 @@ def batch_jobs(n=64, seed=2024, min_side=128, max_side=1024):
         bounces = int(rs.randint(4, 13))
         jobs.append(batch_job(family, material, side, spp, bounces, rs))
     return jobs
+
+
+def served_jobs(n=200, seed=7, rate=20.0, min_side=64, max_side=512, target=(0.05, 0.20)):
+    """The served workload: n jobs arriving as a Poisson process of `rate` per
+    second, scenes drawn like batch_jobs (family, material, bounces), side
+    log-uniform in [min_side, max_side], target relative RMS error uniform in
+    `target`.  Returns [(arrival_s, scene, target)]."""
+    rs = np.random.RandomState(seed)
+    t, out = 0.0, []
+    for _ in range(n):
+        t += float(rs.exponential(1.0 / rate))
+        family = BATCH_FAMILIES[rs.randint(len(BATCH_FAMILIES))]
+        material = BATCH_MATERIALS[rs.randint(len(BATCH_MATERIALS))]
+        side = int(round(math.exp(rs.uniform(math.log(min_side), math.log(max_side)))))
+        bounces = int(rs.randint(4, 13))
+        s = batch_job(family, material, side, 0, bounces, rs)
+        out.append((t, s, float(rs.uniform(*target))))
+    return out

@@ -1,0 +1,137 @@
+"""Served rendering: continuous batching of render jobs that arrive over time,
+each rendered until its estimated error reaches the job's target.
+
+A job is a scene at some resolution with a target relative RMS error.  It is
+rendered into two accumulation states seeded differently (A and B); their
+difference, reduced on the GPU (snl_state_compare), estimates the error of the
+averaged image, ||(A+B)/2 - I|| / ||I|| ~ ||A - B|| / (2 ||(A+B)/2||).
+Each tick the server
+  1. admits waiting jobs in arrival order while the device has room for their
+     states (snl_ctx_mem_info, snl_state_bytes) and the active set is not full;
+  2. renders one quantum of frames for every active job in one
+     snl_render_batch call -- a job's quantum is sized from its error trend,
+     frames_needed ~ frames_done * (error / target)^2;
+  3. retires the jobs whose estimate is at or below their target, recording
+     the time from arrival to completion.
+Pixels never read each other's state, so a job's images do not depend on
+which other jobs shared its batches (tests/test_serve.py checks this).
+"""
+import math
+import time
+
+import numpy as np
+
+from . import host, native
+
+
+class Job:
+    def __init__(self, jid, scene, target, arrival):
+        self.id, self.scene, self.target, self.arrival = jid, scene, target, arrival
+        self.frames = 0
+        self.error = math.inf
+        self.admitted = self.finished = None
+        self.a = self.b = None
+
+    @property
+    def pixels(self):
+        return self.scene.width * self.scene.height
+
+    def open(self, ctx, flags, seed):
+        s = self.scene
+        self.a = host.Session(s, s.width, s.height, ctx, seed=seed, flags=flags)
+        self.b = native.State(ctx, s.width, s.height, seed ^ 0x5EED5EED)
+
+    def close(self):
+        if self.b is not None:
+            self.b.close()
+        if self.a is not None:
+            self.a.close()
+        self.a = self.b = None
+
+
+class Server:
+    """min_frames: frames per state before an error estimate counts;
+    quantum: (smallest, largest) frames per state per tick;
+    reserve: device bytes kept free; max_active: cap on concurrent jobs."""
+
+    def __init__(self, ctx, flags=native.BUILD_FAST, min_frames=4, quantum=(2, 64), reserve=8 << 30,
+                 max_active=256, clock=time.perf_counter, keep_images=False):
+        self.ctx, self.flags, self.keep_images = ctx, flags, keep_images
+        self.min_frames, self.quantum, self.reserve, self.max_active = min_frames, quantum, reserve, max_active
+        self.clock = clock
+        self.ticks = 0
+        self.samples = 0
+        self.gpu_ms = 0.0
+
+    def _quantum(self, job):
+        lo, hi = self.quantum
+        if job.frames < self.min_frames:
+            return max(lo, self.min_frames - job.frames)
+        need = job.frames * (job.error / job.target) ** 2 if job.error < math.inf else 2 * job.frames
+        return int(max(lo, min(hi, math.ceil(need - job.frames))))
+
+    def run(self, arrivals, time_limit=None):
+        """arrivals: [(t, scene, target)] with t in seconds from the start,
+        ascending.  Returns the finished jobs (with .arrival, .admitted,
+        .finished in seconds from the start) and the ones left unfinished."""
+        t0 = self.clock()
+        waiting = [Job(i, s, tg, t) for i, (t, s, tg) in enumerate(arrivals)]
+        waiting.reverse()  # pop() takes the earliest
+        queue, active, done = [], [], []
+        while waiting or queue or active:
+            now = self.clock() - t0
+            if time_limit is not None and now > time_limit:
+                break
+            while waiting and waiting[-1].arrival <= now:
+                queue.append(waiting.pop())
+            # admission by free device memory
+            free, _ = native.mem_info(self.ctx)
+            while queue and len(active) < self.max_active:
+                job = queue[0]
+                need = 2 * native.state_bytes(job.scene.width, job.scene.height)
+                if need + self.reserve > free:
+                    break
+                queue.pop(0)
+                job.open(self.ctx, self.flags, 1000003 * (job.id + 1))
+                job.admitted = self.clock() - t0
+                free -= need
+                active.append(job)
+            if not active:
+                if waiting:
+                    time.sleep(max(0.0, min(waiting[-1].arrival - (self.clock() - t0), 0.05)))
+                continue
+            batch, counts = [], []
+            for job in active:
+                q = self._quantum(job)
+                counts.append(q)
+                batch.append((job.a.prog, job.a.state, q))
+                batch.append((job.a.prog, job.b, q))
+            st = native.render_batch(batch)
+            self.ticks += 1
+            self.gpu_ms += st.ms_total
+            still = []
+            for job, q in zip(active, counts):
+                job.frames += q
+                self.samples += 2 * q * job.pixels
+                if job.frames >= self.min_frames:
+                    d2, m2 = native.compare_states(job.a.state, job.b)
+                    job.error = math.sqrt(d2) / 2.0 / math.sqrt(m2) if m2 > 0 else math.inf
+                if job.error <= job.target:
+                    job.finished = self.clock() - t0
+                    if self.keep_images:
+                        job.images = (job.a.radiance().copy(), job.b.read(0).copy())
+                    job.close()
+                    done.append(job)
+                else:
+                    still.append(job)
+            active = still
+        for job in active:
+            job.close()
+        return done, active + queue + waiting
+
+    @staticmethod
+    def report(done, wall_s, samples):
+        lat = np.array([j.finished - j.arrival for j in done]) if done else np.array([math.nan])
+        return {"jobs_done": len(done), "p50_s": round(float(np.percentile(lat, 50)), 4),
+                "p99_s": round(float(np.percentile(lat, 99)), 4), "mean_s": round(float(lat.mean()), 4),
+                "path_samples_per_s": round(samples / wall_s, 1) if wall_s > 0 else None}

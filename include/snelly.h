@@ -149,20 +149,26 @@ int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, 
                      snl_stats* stats);
 
 /* Served rendering support (no counterpart in the reference, which renders one
- * interactive canvas).  A server renders a job into two states seeded
- * differently and reads its error from them:
- *   out[0] = sum over pixels and X,Y,Z of (ua - ub)^2,
- *   out[1] = sum of ((ua + ub) / 2)^2,
- * where ua, ub are the unbiased means XYZ * N / (N - 1) of the two states.
- * The relative RMS error of the averaged image is about sqrt(out[0]) / 2 /
- * sqrt(out[1]).  Computed on the device; only the two sums are copied back. */
-int snl_state_compare(snl_state a, snl_state b, double* out);
+ * interactive canvas).  A state of K chains holds K independent accumulations
+ * of one width x height image: pixel p of chain k is state pixel k*w*h + p,
+ * rendered like pixel p of the image from its own seed (PCG32 of its state
+ * index).  One launch renders all chains, so small images still fill the
+ * device, and the chains' spread gives the error of their mean:
+ *   out[0] = sum over pixels and X,Y,Z of var_k(u_k) / K,
+ *   out[1] = sum of m^2,
+ * where u_k are the chains' unbiased means XYZ * N / (N - 1) and m their mean;
+ * the relative RMS error of the mean image is about sqrt(out[0] / out[1]).
+ * Computed on the device; only the two sums are copied back.  snl_state_read
+ * returns all chains, chain after chain. */
+int snl_state_create_chains(snl_ctx ctx, int width, int height, int chains, snl_state* out);
+int snl_state_error(snl_state st, double* out);
 /* free and total device memory of the context's GPU (hipMemGetInfo), for
  * admission control */
 int snl_ctx_mem_info(snl_ctx ctx, unsigned long long* free_bytes, unsigned long long* total_bytes);
-/* device bytes of a width x height state rendered by the persistent engine or
- * snl_render_batch (the wavefront engine adds its path buffers on first use) */
-unsigned long long snl_state_bytes(int width, int height);
+/* device bytes of a state of `chains` width x height images rendered by the
+ * persistent engine or snl_render_batch (the wavefront engine adds its path
+ * buffers on first use) */
+unsigned long long snl_state_bytes(int width, int height, int chains);
 
 #ifdef __cplusplus
 }

@@ -60,8 +60,8 @@ struct snl_tex_s {
 
 struct snl_state_s {
   snl_ctx ctx;
-  int w = 0, h = 0;
-  long long npix = 0;
+  int w = 0, h = 0, chains = 1;
+  long long npix = 0;  // w * h * chains
   float* radiance = nullptr;
   float* rng = nullptr;
   float* pf = nullptr;
@@ -265,6 +265,7 @@ void fillArgs(snl_prog p, snl_state st, SnlKArgs& a) {
   a.W = st->w;
   a.H = st->h;
   a.npix = (int)st->npix;
+  a.imagePix = st->w * st->h;
   a.spf = p->spf;
   a.chunk = p->traceChunk;
   a.frames = 1;
@@ -569,12 +570,15 @@ int snl_texture_destroy(snl_tex t) {
   return SNL_OK;
 }
 
-int snl_state_create(snl_ctx ctx, int w, int h, snl_state* out) {
-  if (!ctx || !out || w <= 0 || h <= 0) return fail(ctx, SNL_E_ARG, "bad state size");
+int snl_state_create(snl_ctx ctx, int w, int h, snl_state* out) { return snl_state_create_chains(ctx, w, h, 1, out); }
+
+int snl_state_create_chains(snl_ctx ctx, int w, int h, int chains, snl_state* out) {
+  if (!ctx || !out || w <= 0 || h <= 0 || chains <= 0) return fail(ctx, SNL_E_ARG, "bad state size");
+  if ((long long)w * h * chains > 0x7fffffffLL) return fail(ctx, SNL_E_ARG, "state too large");
   std::unique_ptr<snl_state_s> s(new snl_state_s);
   s->ctx = ctx;
-  s->w = w; s->h = h;
-  s->npix = (long long)w * h;
+  s->w = w; s->h = h; s->chains = chains;
+  s->npix = (long long)w * h * chains;
   long long n = s->npix;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   // radiance and RNG (32 B per pixel) plus counters; the wavefront engine's
@@ -606,24 +610,32 @@ int snl_state_destroy(snl_state s) {
   return SNL_OK;
 }
 
-// Sums over pixels of two accumulations a, b of the same image: the unbiased
-// means u = XYZ * N / (N - 1) (the count starts at 1, js/renderer.js:735-745),
-// out[0] += sum |ua - ub|^2, out[1] += sum |(ua + ub) / 2|^2.  One partial per
-// block, summed in a second pass so the result does not depend on block order.
-__global__ void compare_kernel(const float4* a, const float4* b, long long npix, double* partial) {
+// Error estimate of a state of K >= 2 chains of one image.  Per pixel and
+// channel, with u_k the unbiased mean XYZ * N / (N - 1) of chain k (the count
+// starts at 1, js/renderer.js:735-745) and m their mean over chains:
+//   out[0] += sum_k (u_k - m)^2 / (K - 1) / K   (variance of m)
+//   out[1] += m^2
+// One partial per block, summed in a second pass in block order.
+__global__ void chain_error_kernel(const float4* rad, long long imagePix, int K, double* partial) {
   __shared__ double s0[256], s1[256];
-  double d2 = 0.0, m2 = 0.0;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += (long long)gridDim.x * blockDim.x) {
-    float4 x = a[i], y = b[i];
-    double fx = x.w > 1.0f ? (double)x.w / (x.w - 1.0) : 0.0, fy = y.w > 1.0f ? (double)y.w / (y.w - 1.0) : 0.0;
-    double ux[3] = {x.x * fx, x.y * fx, x.z * fx}, uy[3] = {y.x * fy, y.y * fy, y.z * fy};
+  double v2 = 0.0, m2 = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < imagePix;
+       i += (long long)gridDim.x * blockDim.x) {
+    double sum[3] = {0, 0, 0}, sq[3] = {0, 0, 0};
+    for (int k = 0; k < K; ++k) {
+      float4 x = rad[k * imagePix + i];
+      double f = x.w > 1.0f ? (double)x.w / (x.w - 1.0) : 0.0;
+      double u[3] = {x.x * f, x.y * f, x.z * f};
+      for (int c = 0; c < 3; ++c) { sum[c] += u[c]; sq[c] += u[c] * u[c]; }
+    }
     for (int c = 0; c < 3; ++c) {
-      double d = ux[c] - uy[c], m = 0.5 * (ux[c] + uy[c]);
-      d2 += d * d;
+      double m = sum[c] / K;
+      double var = (sq[c] - K * m * m) / (K - 1);
+      v2 += (var > 0.0 ? var : 0.0) / K;
       m2 += m * m;
     }
   }
-  s0[threadIdx.x] = d2;
+  s0[threadIdx.x] = v2;
   s1[threadIdx.x] = m2;
   __syncthreads();
   for (int k = 128; k > 0; k >>= 1) {
@@ -632,11 +644,11 @@ __global__ void compare_kernel(const float4* a, const float4* b, long long npix,
   }
   if (threadIdx.x == 0) { partial[2 * blockIdx.x] = s0[0]; partial[2 * blockIdx.x + 1] = s1[0]; }
 }
-__global__ void compare_sum_kernel(const double* partial, int n, double* out) {
+__global__ void error_sum_kernel(const double* partial, int n, double* out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double d2 = 0.0, m2 = 0.0;
-  for (int k = 0; k < n; ++k) { d2 += partial[2 * k]; m2 += partial[2 * k + 1]; }
-  out[0] = d2;
+  double v2 = 0.0, m2 = 0.0;
+  for (int k = 0; k < n; ++k) { v2 += partial[2 * k]; m2 += partial[2 * k + 1]; }
+  out[0] = v2;
   out[1] = m2;
 }
 
@@ -867,17 +879,17 @@ static int renderPersistent(snl_prog p, snl_state st, int frames) {
   return SNL_OK;
 }
 
-int snl_state_compare(snl_state a, snl_state b, double* out) {
-  if (!a || !b || !out) return fail(a ? a->ctx : nullptr, SNL_E_ARG, "NULL argument");
-  if (a->ctx != b->ctx || a->npix != b->npix) return fail(a->ctx, SNL_E_ARG, "states differ in context or size");
-  snl_ctx ctx = a->ctx;
+int snl_state_error(snl_state st, double* out) {
+  if (!st || !out) return fail(st ? st->ctx : nullptr, SNL_E_ARG, "NULL argument");
+  if (st->chains < 2) return fail(st->ctx, SNL_E_ARG, "the error estimate needs a state of two or more chains");
+  snl_ctx ctx = st->ctx;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const int blocks = 256;
   double* d = nullptr;
   HIPCHK(ctx, hipMallocAsync((void**)&d, (2 * blocks + 2) * sizeof(double), ctx->stream));
-  hipLaunchKernelGGL(compare_kernel, dim3(blocks), dim3(256), 0, ctx->stream, reinterpret_cast<const float4*>(a->radiance),
-                     reinterpret_cast<const float4*>(b->radiance), a->npix, d);
-  hipLaunchKernelGGL(compare_sum_kernel, dim3(1), dim3(64), 0, ctx->stream, d, blocks, d + 2 * blocks);
+  hipLaunchKernelGGL(chain_error_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
+                     reinterpret_cast<const float4*>(st->radiance), (long long)st->w * st->h, st->chains, d);
+  hipLaunchKernelGGL(error_sum_kernel, dim3(1), dim3(64), 0, ctx->stream, d, blocks, d + 2 * blocks);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipMemcpyAsync(out, d + 2 * blocks, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipFreeAsync(d, ctx->stream));
@@ -895,8 +907,9 @@ int snl_ctx_mem_info(snl_ctx ctx, unsigned long long* free_bytes, unsigned long 
   return SNL_OK;
 }
 
-unsigned long long snl_state_bytes(int width, int height) {
-  return (unsigned long long)width * height * 32ull + (unsigned long long)(SNL_MAX_TILES + 1) * C_COUNT * 4ull;
+unsigned long long snl_state_bytes(int width, int height, int chains) {
+  return (unsigned long long)width * height * (unsigned long long)(chains > 0 ? chains : 1) * 32ull +
+         (unsigned long long)(SNL_MAX_TILES + 1) * C_COUNT * 4ull;
 }
 
 int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, const int* frames,

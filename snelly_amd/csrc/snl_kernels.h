@@ -85,7 +85,7 @@ __device__ inline void camera_body(const SnlKArgs& a) {
       flags = SNL_PI(I_FLAGS, i) & PF_SKIP;
     }
     if (!(flags & PF_SKIP)) {
-      vec2 pixel(float(i % a.W) + 0.5f, float(i / a.W) + 0.5f);
+      vec2 pixel(float(i % a.W) + 0.5f, float((i % a.imagePix) / a.W) + 0.5f);
       float jx = 0.5f * P.filterRadius * sample_jitter(snl_rand(rnd));
       float jy = 0.5f * P.filterRadius * sample_jitter(snl_rand(rnd));
       vec2 pixelj = pixel + vec2(jx, jy);
@@ -621,7 +621,7 @@ __device__ inline void normals_body(const SnlKArgs& a) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.npix) return;
   vec4 rnd = ld4(a.rngstate, i);
-  vec2 pixel(float(i % a.W) + 0.5f, float(i / a.W) + 0.5f);
+  vec2 pixel(float(i % a.W) + 0.5f, float((i % a.imagePix) / a.W) + 0.5f);
   vec3 colorXYZ(0.0f);
   for (int n = 0; n < a.spf; ++n) {
     float r1 = snl_rand(rnd);

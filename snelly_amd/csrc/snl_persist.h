@@ -369,7 +369,7 @@ __device__ inline void persist_body(const SnlKArgs& a) {
           st = PS_IDLE;
         }
       } else {
-        vec2 pixel(float(pix % a.W) + 0.5f, float(pix / a.W) + 0.5f);
+        vec2 pixel(float(pix % a.W) + 0.5f, float((pix % a.imagePix) / a.W) + 0.5f);
         float jx = 0.5f * P.filterRadius * sample_jitter(snl_rand(rnd));
         float jy = 0.5f * P.filterRadius * sample_jitter(snl_rand(rnd));
         vec2 pixelj = pixel + vec2(jx, jy);

@@ -49,6 +49,7 @@ struct SnlKArgs {
   int shadeThresh;        // persistent kernel: lanes waiting before a wave shades
   int frames;             // persistent kernel: frames each pixel runs before the lane moves on
   int maxBounces;         // __MAX_BOUNCES__: the last vertex index a path may reach
+  int imagePix;           // W * H; a state of several chains holds npix = chains * imagePix
 };
 
 #define SNL_PF(k, i) a.pf[(long long)(k) * a.cap + (i)]

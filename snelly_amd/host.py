@@ -592,7 +592,7 @@ INT_UNIFORMS = {"maxSSSSteps", "maxStepsIsMiss", "wavelengthSamples", "envMapVis
 class Session:
     """One scene on one GPU: program + accumulation state + LUT textures."""
 
-    def __init__(self, scene, width, height, ctx=None, seed=1234, flags=0):
+    def __init__(self, scene, width, height, ctx=None, seed=1234, flags=0, chains=1):
         self.scene = scene
         self.ctx = ctx or native.Context(0)
         self.width, self.height = width, height
@@ -601,7 +601,7 @@ class Session:
         self.prog = native.Program(self.ctx, self.shader, self.ior, self.defines, int(round(r.maxMarchSteps)),
                                    int(round(r.maxBounces)), int(round(r.maxAtmosphereScatters)),
                                    int(round(r.maxSamplesPerFrame)), flags)
-        self.state = native.State(self.ctx, width, height, seed)
+        self.state = native.State(self.ctx, width, height, seed, chains)
         self.textures = {}
         self.textures["WavelengthToXYZ"] = native.Texture(self.ctx, xyz_lut(), 1024, 1, 1, 4, True, True)
         icdf = inverse_cdf(lambda wl: blackbody(wl, r.skyTemperature))

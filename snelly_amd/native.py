@@ -23,7 +23,7 @@ EXPORTS = [
     "snl_state_create", "snl_state_destroy", "snl_state_seed", "snl_state_clear", "snl_state_read",
     "snl_state_write", "snl_render", "snl_render_normals", "snl_pick", "snl_tonemap", "snl_sync",
     "snl_get_stats", "snl_set_tuning", "snl_set_profiling", "snl_set_engine", "snl_render_batch",
-    "snl_state_compare", "snl_ctx_mem_info", "snl_state_bytes",
+    "snl_state_create_chains", "snl_state_error", "snl_ctx_mem_info", "snl_state_bytes",
 ]
 ENGINE_WAVEFRONT, ENGINE_PERSISTENT = 0, 1
 
@@ -79,9 +79,10 @@ def lib():
         "snl_set_engine": (ip, [vp, ip, ip, ip, ip, ip]),
         "snl_render_batch": (ip, [ip, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(ip),
                                   ctypes.POINTER(Stats)]),
-        "snl_state_compare": (ip, [vp, vp, ctypes.POINTER(ctypes.c_double)]),
+        "snl_state_create_chains": (ip, [vp, ip, ip, ip, ctypes.POINTER(vp)]),
+        "snl_state_error": (ip, [vp, ctypes.POINTER(ctypes.c_double)]),
         "snl_ctx_mem_info": (ip, [vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_ulonglong)]),
-        "snl_state_bytes": (ctypes.c_ulonglong, [ip, ip]),
+        "snl_state_bytes": (ctypes.c_ulonglong, [ip, ip, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -101,10 +102,11 @@ def check(rc, ctx=None):
         raise SnellyError(f"snelly error {rc}: {msg}")
 
 
-def compare_states(a, b):
-    """(sum |ua - ub|^2, sum |(ua + ub)/2|^2) over the two states' unbiased means"""
+def state_error(st):
+    """(sum of the variance of the chains' mean, sum of its square) for a state
+    of two or more chains"""
     out = (ctypes.c_double * 2)()
-    a.ctx.check(lib().snl_state_compare(a.h, b.h, out))
+    st.ctx.check(lib().snl_state_error(st.h, out))
     return out[0], out[1]
 
 
@@ -114,8 +116,8 @@ def mem_info(ctx):
     return f.value, t.value
 
 
-def state_bytes(width, height):
-    return int(lib().snl_state_bytes(width, height))
+def state_bytes(width, height, chains=1):
+    return int(lib().snl_state_bytes(width, height, chains))
 
 
 def render_batch(jobs):
@@ -186,10 +188,10 @@ class Texture:
 
 
 class State:
-    def __init__(self, ctx, width, height, seed=1234):
-        self.ctx, self.width, self.height = ctx, width, height
+    def __init__(self, ctx, width, height, seed=1234, chains=1):
+        self.ctx, self.width, self.height, self.chains = ctx, width, height, chains
         h = ctypes.c_void_p()
-        ctx.check(lib().snl_state_create(ctx.h, width, height, ctypes.byref(h)))
+        ctx.check(lib().snl_state_create_chains(ctx.h, width, height, chains, ctypes.byref(h)))
         self.h = h
         self.seed(seed)
 
@@ -200,12 +202,14 @@ class State:
         self.ctx.check(lib().snl_state_clear(self.h))
 
     def read(self, which=0):
-        out = np.empty((self.height, self.width, 4), np.float32)
+        """(height, width, 4) float32, or (chains, height, width, 4) for several chains"""
+        shape = (self.height, self.width, 4) if self.chains == 1 else (self.chains, self.height, self.width, 4)
+        out = np.empty(shape, np.float32)
         self.ctx.check(lib().snl_state_read(self.h, which, _fptr(out)))
         return out
 
     def write(self, which, data):
-        arr = np.ascontiguousarray(data, dtype=np.float32).reshape(self.height, self.width, 4)
+        arr = np.ascontiguousarray(data, dtype=np.float32).reshape(self.chains * self.height, self.width, 4)
         self.ctx.check(lib().snl_state_write(self.h, which, _fptr(arr)))
 
     def close(self):

@@ -2,9 +2,10 @@
 each rendered until its estimated error reaches the job's target.
 
 A job is a scene at some resolution with a target relative RMS error.  It is
-rendered into two accumulation states seeded differently (A and B); their
-difference, reduced on the GPU (snl_state_compare), estimates the error of the
-averaged image, ||(A+B)/2 - I|| / ||I|| ~ ||A - B|| / (2 ||(A+B)/2||).
+rendered into K independently seeded chains of its image held in one state
+(snl_state_create_chains), K = 2..32 chosen so that even small images give
+one launch tens of thousands of lanes; the chains' spread, reduced on the GPU
+(snl_state_error), estimates the relative RMS error of their mean image.
 Each tick the server
   1. admits waiting jobs in arrival order while the device has room for their
      states (snl_ctx_mem_info, snl_state_bytes) and the active set is not full;
@@ -30,23 +31,29 @@ class Job:
         self.frames = 0
         self.error = math.inf
         self.admitted = self.finished = None
-        self.a = self.b = None
+        self.chains = 2
+        self.a = None
 
     @property
     def pixels(self):
         return self.scene.width * self.scene.height
 
+    @staticmethod
+    def chains_for(pixels, lanes=1 << 16):
+        k = 2
+        while k < 32 and k * pixels < lanes:
+            k *= 2
+        return k
+
     def open(self, ctx, flags, seed):
         s = self.scene
-        self.a = host.Session(s, s.width, s.height, ctx, seed=seed, flags=flags)
-        self.b = native.State(ctx, s.width, s.height, seed ^ 0x5EED5EED)
+        self.chains = self.chains_for(self.pixels)
+        self.a = host.Session(s, s.width, s.height, ctx, seed=seed, flags=flags, chains=self.chains)
 
     def close(self):
-        if self.b is not None:
-            self.b.close()
         if self.a is not None:
             self.a.close()
-        self.a = self.b = None
+        self.a = None
 
 
 class Server:
@@ -88,7 +95,7 @@ class Server:
             free, _ = native.mem_info(self.ctx)
             while queue and len(active) < self.max_active:
                 job = queue[0]
-                need = 2 * native.state_bytes(job.scene.width, job.scene.height)
+                need = native.state_bytes(job.scene.width, job.scene.height, Job.chains_for(job.pixels))
                 if need + self.reserve > free:
                     break
                 queue.pop(0)
@@ -105,21 +112,20 @@ class Server:
                 q = self._quantum(job)
                 counts.append(q)
                 batch.append((job.a.prog, job.a.state, q))
-                batch.append((job.a.prog, job.b, q))
             st = native.render_batch(batch)
             self.ticks += 1
             self.gpu_ms += st.ms_total
             still = []
             for job, q in zip(active, counts):
                 job.frames += q
-                self.samples += 2 * q * job.pixels
+                self.samples += q * job.pixels * job.chains
                 if job.frames >= self.min_frames:
-                    d2, m2 = native.compare_states(job.a.state, job.b)
-                    job.error = math.sqrt(d2) / 2.0 / math.sqrt(m2) if m2 > 0 else math.inf
+                    v2, m2 = native.state_error(job.a.state)
+                    job.error = math.sqrt(v2 / m2) if m2 > 0 else math.inf
                 if job.error <= job.target:
                     job.finished = self.clock() - t0
                     if self.keep_images:
-                        job.images = (job.a.radiance().copy(), job.b.read(0).copy())
+                        job.image = job.a.radiance().copy()
                     job.close()
                     done.append(job)
                 else:

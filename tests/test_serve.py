@@ -39,22 +39,43 @@ def test_report_percentiles():
     assert r["path_samples_per_s"] == 1e8
 
 
+def test_chains_for_small_images():
+    assert serve.Job.chains_for(64 * 64) == 16
+    assert serve.Job.chains_for(512 * 512) == 2
+    assert serve.Job.chains_for(8 * 8) == 32
+
+
 @pytest.mark.gpu
-def test_state_compare_sums(ctx):
-    W, H = 19, 7
+def test_state_error_sums(ctx):
+    W, H, K = 19, 7, 3
     rs = np.random.RandomState(0)
-    a, b = native.State(ctx, W, H, 1), native.State(ctx, W, H, 2)
-    ra = rs.uniform(0, 2, (W * H, 4)).astype(np.float32)
-    rb = rs.uniform(0, 2, (W * H, 4)).astype(np.float32)
-    ra[:, 3] = rs.randint(2, 9, W * H)
-    rb[:, 3] = rs.randint(2, 9, W * H)
-    a.write(0, ra)
-    b.write(0, rb)
-    d2, m2 = native.compare_states(a, b)
-    ua = ra[:, :3].astype(np.float64) * (ra[:, 3:] / (ra[:, 3:] - 1.0))
-    ub = rb[:, :3].astype(np.float64) * (rb[:, 3:] / (rb[:, 3:] - 1.0))
-    assert d2 == pytest.approx(((ua - ub) ** 2).sum(), rel=1e-12)
-    assert m2 == pytest.approx((((ua + ub) / 2) ** 2).sum(), rel=1e-12)
+    st = native.State(ctx, W, H, 1, chains=K)
+    r = rs.uniform(0, 2, (K, H, W, 4)).astype(np.float32)
+    r[..., 3] = rs.randint(2, 9, (K, H, W))
+    st.write(0, r)
+    assert np.array_equal(st.read(0), r)
+    v2, m2 = native.state_error(st)
+    f = r.astype(np.float64)
+    u = f[..., :3] * (f[..., 3:] / (f[..., 3:] - 1.0))
+    m = u.mean(axis=0)
+    assert v2 == pytest.approx((u.var(axis=0, ddof=1) / K).sum(), rel=1e-9)
+    assert m2 == pytest.approx((m ** 2).sum(), rel=1e-12)
+    st.close()
+
+
+@pytest.mark.gpu
+def test_chain_zero_is_the_single_chain_render(ctx):
+    """chain 0 of a multi-chain state starts from the same seeds as a plain
+    state and renders the same image: bit for bit"""
+    s = scenes.batch_job("menger3", "METAL", 40, 3, 5, np.random.RandomState(2))
+    a = host.Session(s, 40, 40, ctx, seed=77, flags=native.BUILD_PRECISE, chains=4)
+    b = host.Session(s, 40, 40, ctx, seed=77, flags=native.BUILD_PRECISE)
+    a.render(3)
+    b.render(3)
+    ra = a.state.read(0)
+    assert ra.shape == (4, 40, 40, 4)
+    assert np.array_equal(ra[0], b.state.read(0))
+    assert not np.array_equal(ra[1], ra[0])
     a.close()
     b.close()
 
@@ -70,12 +91,8 @@ def test_served_jobs_finish_and_match_alone(ctx):
     for job in done:
         assert job.error <= job.target and job.frames >= srv.min_frames
         s = job.scene
-        seed = 1000003 * (job.id + 1)
-        alone = host.Session(s, s.width, s.height, ctx, seed=seed, flags=native.BUILD_PRECISE)
+        alone = host.Session(s, s.width, s.height, ctx, seed=1000003 * (job.id + 1), flags=native.BUILD_PRECISE,
+                             chains=job.chains)
         alone.render(job.frames)
-        b = native.State(ctx, s.width, s.height, seed ^ 0x5EED5EED)
-        alone.prog.render(b, job.frames)
-        assert np.array_equal(alone.radiance(), job.images[0]), s.name
-        assert np.array_equal(b.read(0), job.images[1]), s.name
-        b.close()
+        assert np.array_equal(alone.radiance(), job.image), s.name
         alone.close()

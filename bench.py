@@ -123,7 +123,7 @@ def served(ctx, n_jobs):
     rep.update({"workload": "served_poisson20_mixed_sdf", "jobs": len(arrivals), "unfinished": len(left),
                 "arrival_rate_per_s": 20.0, "sides": [64, 512], "target_rrmse": [0.10, 0.25], "wall_s": round(wall, 2),
                 "ticks": srv.ticks, "gpu_busy_frac": round(srv.gpu_ms * 1e-3 / wall, 3),
-                "error_estimate": "two independently seeded accumulations, ||A-B|| / (2 ||(A+B)/2||), on the GPU"})
+                "error_estimate": "spread of 2-32 independently seeded chains per job, reduced on the GPU"})
     return rep
 
 

@@ -171,6 +171,7 @@ def main():
     images = []
     trace_ms = 0.0
     trace_steps = 0
+    normal_evals = 0
     launches = 0
     step_ms = []
     elapsed = 0.0
@@ -186,22 +187,26 @@ def main():
         st = sess.prog.stats()
         trace_ms += st.ms_trace
         trace_steps += st.trace_steps
+        normal_evals += st.normal_evals
         launches += st.trace_launches
         if k < 2:  # read back outside the timed region
             images.append(unbiased(sess.radiance()))
     samples = W * H * spp * args.steps
     value = samples / elapsed
 
-    # snl_persist is the whole frame; its launches on the tile streams overlap,
-    # so the rate is total algorithmic FLOPs over the GPU time of the renders
-    flops = trace_steps * scene.flops_per_step
+    # snl_persist is the whole render; its launches on the tile streams overlap,
+    # so the rate is total algorithmic FLOPs over the GPU time of the renders.
+    # Counted: every SDF march step (SDF + 8 for the step itself) and every
+    # central-difference normal (6 SDF evaluations); shading is not counted.
+    flops = trace_steps * scene.flops_per_step + normal_evals * 6 * scene.sdf_flops
     achieved = flops / (trace_ms * 1e-3) / 1e12
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
                 "kernel": "snl_persist", "launches": launches, "gpu_ms": round(trace_ms, 2),
                 "flops_per_launch": flops / launches, "flops_per_step": scene.flops_per_step,
-                "steps_per_launch": trace_steps / launches,
-                "flops_counted": "SDF march steps only (shading excluded)"}
+                "steps_per_launch": trace_steps / launches, "normal_evals_per_launch": normal_evals / launches,
+                "sdf_flops": scene.sdf_flops,
+                "flops_counted": "SDF march steps (SDF + 8) and normals (6 SDF); shading excluded"}
     rr = None
     if len(images) == 2:
         a, b = images

@@ -115,6 +115,7 @@ typedef struct snl_stats {
   long long trace_steps;   /* SDF march steps executed (all rays) */
   long long paths;         /* camera paths started */
   int trace_launches;
+  long long normal_evals;  /* surface normals computed (6 SDF evaluations each) */
 } snl_stats;
 int snl_get_stats(snl_prog prog, snl_stats* out);
 /* launch-shape knobs: trace grid (blocks), jobs fetched per wave, shade grid;

@@ -571,9 +571,14 @@ SNL_HD int snl_wrap(int i, int n, bool clampEdge) {
   int m = i % n;
   return m < 0 ? m + n : m;
 }
+// channels a texture lacks read as (0, 0, 0, 1); the component writes are
+// spelled out so that no vector is indexed by a runtime value (that would put
+// it in scratch memory)
 SNL_HD vec4 snl_texel(const float* p, int ch) {
-  vec4 r(0.0f, 0.0f, 0.0f, 1.0f);
-  for (int c = 0; c < ch; ++c) r.e[c] = p[c];
+  vec4 r(p[0], 0.0f, 0.0f, 1.0f);
+  if (ch > 1) r.y = p[1];
+  if (ch > 2) r.z = p[2];
+  if (ch > 3) r.w = p[3];
   return r;
 }
 SNL_HD vec4 texture(const sampler2D& s, const vec2& uv) {

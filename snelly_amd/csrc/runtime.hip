@@ -798,9 +798,11 @@ static int renderWavefront(snl_prog p, snl_state st, int frames) {
       traceMs += t;
     }
   }
-  long long steps = 0;
+  long long steps = 0, normals = 0;
   HIPCHK(ctx, hipMemcpyAsync(&steps, st->cnt + C_TRACE_STEPS, 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(ctx, hipMemcpyAsync(&normals, st->cnt + C_NORMALS, 8, hipMemcpyDeviceToHost, s));
   HIPCHK(ctx, hipStreamSynchronize(s));
+  p->stats.normal_evals = normals;
   p->stats.ms_total = ms;
   p->stats.ms_trace = traceMs;
   p->stats.trace_steps = steps;
@@ -864,13 +866,16 @@ static int renderPersistent(snl_prog p, snl_state st, int frames) {
   HIPCHK(ctx, hipEventSynchronize(p->ev1));
   float ms = 0.0f;
   HIPCHK(ctx, hipEventElapsedTime(&ms, p->ev0, p->ev1));
-  long long steps = 0;
+  long long steps = 0, normals = 0;
   for (int k = 0; k < K; ++k) {
-    long long sk = 0;
+    long long sk = 0, nk = 0;
     HIPCHK(ctx, hipMemcpyAsync(&sk, st->tileCnt + k * C_COUNT + C_TRACE_STEPS, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(&nk, st->tileCnt + k * C_COUNT + C_NORMALS, 8, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     steps += sk;
+    normals += nk;
   }
+  p->stats.normal_evals = normals;
   p->stats.ms_total = ms;
   p->stats.ms_trace = ms;  // the persistent kernel is the whole frame
   p->stats.trace_steps = steps;
@@ -1006,10 +1011,12 @@ int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, 
     stats->ms_total = ms;
     stats->ms_trace = ms;
     for (int j = 0; j < njobs; ++j) {
-      long long sk = 0;
+      long long sk = 0, nk = 0;
       HIPCHK(ctx, hipMemcpyAsync(&sk, states[j]->tileCnt + C_TRACE_STEPS, 8, hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(ctx, hipMemcpyAsync(&nk, states[j]->tileCnt + C_NORMALS, 8, hipMemcpyDeviceToHost, ctx->stream));
       HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
       stats->trace_steps += sk;
+      stats->normal_evals += nk;
       stats->paths += (long long)frames[j] * progs[j]->spf * states[j]->npix;
       stats->trace_launches += unit0[j] ? frames[j] : 1;
     }

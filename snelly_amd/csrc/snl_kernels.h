@@ -34,7 +34,8 @@ __device__ inline void stRng(const SnlKArgs& a, int i, const vec4& r) {
 // vertex == -2 also clears the 64-bit step counter kept across a render call
 extern "C" __global__ void snl_counters_reset(SnlKArgs a) {
   int k = threadIdx.x;
-  if (k < C_COUNT && (a.vertex == -2 || (k != C_TRACE_STEPS && k != C_TRACE_STEPS + 1))) a.cnt[k] = 0;
+  bool stat = k == C_TRACE_STEPS || k == C_TRACE_STEPS + 1 || k == C_NORMALS || k == C_NORMALS + 1;
+  if (k < C_COUNT && (a.vertex == -2 || !stat)) a.cnt[k] = 0;
 }
 // the output queues of one stage become the input queues of the next
 extern "C" __global__ void snl_counters_rotate(SnlKArgs a) {
@@ -356,6 +357,11 @@ __device__ inline void shade_hit(const SnlKArgs& a) {
       }
       pW = pW_next;
       vec3 nW = pt.normal(pW, hitMaterial, true);
+      {
+        unsigned long long nb = __ballot(1);
+        if (snl_lane() == __ffsll((long long)nb) - 1)
+          atomicAdd(reinterpret_cast<unsigned long long*>(&a.cnt[C_NORMALS]), (unsigned long long)__popcll(nb));
+      }
       vec3 ngW = nW;
       Basis basis = makeBasis(nW);
       if constexpr (SC::kHasNormalMap) {

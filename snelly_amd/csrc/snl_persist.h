@@ -57,7 +57,7 @@ __device__ inline void persist_body(const SnlKArgs& a) {
   RT nf0 = rt1(0.0f), nf1 = rt1(0.0f), nf2 = rt1(0.0f), nl0 = rt1(0.0f), nl1 = rt1(0.0f), nl2 = rt1(0.0f);
   float nden0 = 1.0f, nden1 = 1.0f, nden2 = 1.0f, ncos0 = 0.0f, ncos1 = 0.0f, ncos2 = 0.0f;
   float nmis0 = 0.0f, nmis1 = 0.0f, nmis2 = 0.0f;
-  long long stepTotal = 0;
+  long long stepTotal = 0, normalTotal = 0;
 
   auto startRay = [&](const vec3& dir, float len, bool isShadow) {
     d = dir;
@@ -211,6 +211,7 @@ __device__ inline void persist_body(const SnlKArgs& a) {
         }
         pW = pW_next;
         vec3 nW = pt.normal(pW, hitMaterial, true);
+        ++normalTotal;
         vec3 ngW = nW;
         Basis basis = makeBasis(nW);
         if constexpr (SC::kHasNormalMap) {
@@ -389,9 +390,13 @@ __device__ inline void persist_body(const SnlKArgs& a) {
       }
     }
   }
-  unsigned long long tot = stepTotal;
-  for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
+  unsigned long long tot = stepTotal, ntot = normalTotal;
+  for (int off = 32; off > 0; off >>= 1) {
+    tot += __shfl_xor(tot, off);
+    ntot += __shfl_xor(ntot, off);
+  }
   if (lane == 0 && tot) atomicAdd(reinterpret_cast<unsigned long long*>(&a.cnt[C_TRACE_STEPS]), tot);
+  if (lane == 0 && ntot) atomicAdd(reinterpret_cast<unsigned long long*>(&a.cnt[C_NORMALS]), ntot);
 }
 
 extern "C" __global__ void __launch_bounds__(256) snl_persist(SnlKArgs a) { persist_body<SnlScene>(a); }

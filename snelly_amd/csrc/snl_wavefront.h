@@ -25,8 +25,10 @@ enum { I_FLAGS, I_HMAT, I_NEE, I_COUNT };
 #define PF_ASHIFT 16
 
 // counters
-// C_TRACE_STEPS is a 64-bit counter (8-byte aligned slot)
-enum { C_TRACE_IN, C_NEE_IN, C_TRACE_OUT, C_NEE_OUT, C_MAT, C_FETCH = C_MAT + 4, C_TRACE_STEPS = 10, C_COUNT = 16 };
+// C_TRACE_STEPS (SDF march steps) and C_NORMALS (normal evaluations, 6 SDF
+// evaluations each) are 64-bit counters in 8-byte aligned slots
+enum { C_TRACE_IN, C_NEE_IN, C_TRACE_OUT, C_NEE_OUT, C_MAT, C_FETCH = C_MAT + 4, C_TRACE_STEPS = 10, C_NORMALS = 12,
+       C_COUNT = 16 };
 
 struct SnlKArgs {
   const SnlParams* P;

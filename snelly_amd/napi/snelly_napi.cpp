@@ -297,6 +297,7 @@ napi_value Stats(napi_env env, napi_callback_info info) {
   napi_set_named_property(env, o, "ms_total", num(env, s.ms_total));
   napi_set_named_property(env, o, "trace_steps", num(env, (double)s.trace_steps));
   napi_set_named_property(env, o, "paths", num(env, (double)s.paths));
+  napi_set_named_property(env, o, "normal_evals", num(env, (double)s.normal_evals));
   return o;
 }
 napi_value SetEngine(napi_env env, napi_callback_info info) {

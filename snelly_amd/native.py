@@ -34,7 +34,7 @@ class SnellyError(RuntimeError):
 
 class Stats(ctypes.Structure):
     _fields_ = [("ms_total", ctypes.c_double), ("ms_trace", ctypes.c_double), ("trace_steps", ctypes.c_longlong),
-                ("paths", ctypes.c_longlong), ("trace_launches", ctypes.c_int)]
+                ("paths", ctypes.c_longlong), ("trace_launches", ctypes.c_int), ("normal_evals", ctypes.c_longlong)]
 
 
 _lib = None

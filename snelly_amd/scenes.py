@@ -123,6 +123,7 @@ class Scene:
         # materials + 8 for p = o + t*d and t += sign*sdf), from the counting
         # build of the oracle (pinned by tests/test_flops.py); None = varies
         self.flops_per_step = None
+        self.sdf_flops = None  # one SDF evaluation; a normal costs 6
         r = self.renderer
         r.interactive = False
         r.sunLongitude = 0.0
@@ -157,6 +158,7 @@ def sphere_plane(width=256, height=256, spp=16):
     s.oracle_sdfs = [(2, 2, 0, 0, 0, 0)]
     s.oracle_albedo_hook = 1
     s.flops_per_step = 8 + SPHERE_PLANE_SDF_FLOPS
+    s.sdf_flops = SPHERE_PLANE_SDF_FLOPS
     return s
 
 
@@ -202,6 +204,7 @@ def menger_metal(width=1024, height=1024, spp=64, iterations=4):
     s.user_uniforms = {"size": 1.0, "scale": 1.0}
     s.oracle_sdfs = [(1, 3, 1.0, 1.0, float(iterations), 0)]
     s.flops_per_step = 8 + MENGER_SDF_FLOPS[iterations]
+    s.sdf_flops = MENGER_SDF_FLOPS[iterations]
     return s
 
 

@@ -134,6 +134,9 @@ int snl_set_profiling(snl_prog prog, int on);
 #define SNL_ENGINE_WAVEFRONT 0
 #define SNL_ENGINE_PERSISTENT 1
 int snl_set_engine(snl_prog prog, int engine, int tiles, int persist_blocks, int shade_thresh, int chunk);
+/* persistent engine: march steps a tracing lane takes between two checks of
+ * the shading gate (1..64, default 4); results do not depend on it */
+int snl_set_march_burst(snl_prog prog, int steps);
 
 /* Batched rendering: frames[j] frames of every job j = (progs[j], states[j]),
  * each with its program's current uniforms, submitted together -- the jobs'

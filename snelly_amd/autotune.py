@@ -2,8 +2,9 @@
 
 Knobs: pixel tiles pipelined on separate streams, resident blocks per launch
 (waves per CU = 4 x blocks / CUs), the shading threshold (lanes of a wave that
-must wait before the wave runs the shading code) and the number of pixels a
-wave takes from its tile's queue per atomic.  The tuner times short renders of
+must wait before the wave runs the shading code), the number of pixels a
+wave takes from its tile's queue per atomic and the march steps a lane takes
+between two checks of the shading gate.  The tuner times short renders of
 the scene being rendered (the state is re-seeded afterwards by the caller) and
 keeps the fastest shape; coordinate descent from the defaults keeps the number
 of trial renders small.
@@ -29,11 +30,13 @@ def tune(sess, frames=4, cu=256, verbose=False):
         "blocks": [cu * 2, cu * 3, cu * 4, cu * 6, cu * 8],
         "thresh": [8, 16, 24, 32, 48],
         "chunk": [32, 64, 128, 256],
+        "burst": [1, 2, 4, 8, 16],
     }
-    best = {"tiles": 2, "blocks": cu * 4, "thresh": 32, "chunk": 64}
+    best = {"tiles": 1, "blocks": cu * 6, "thresh": 32, "chunk": 32, "burst": 4}
 
     def run(cfg):
         prog.engine(native.ENGINE_PERSISTENT, cfg["tiles"], cfg["blocks"], cfg["thresh"], cfg["chunk"])
+        prog.march_burst(cfg["burst"])
         return time_render(sess, frames)
 
     run(best)  # warm the code object and caches
@@ -52,4 +55,5 @@ def tune(sess, frames=4, cu=256, verbose=False):
                 if t < best_t:
                     best, best_t = cfg, t
     prog.engine(native.ENGINE_PERSISTENT, best["tiles"], best["blocks"], best["thresh"], best["chunk"])
+    prog.march_burst(best["burst"])
     return best, best_t, trials

@@ -78,7 +78,7 @@ struct snl_prog_s {
   hipFunction_t k_reset, k_rotate, k_camera, k_trace, k_bin, k_miss, k_surf, k_metal, k_diele, k_accum, k_normals,
       k_pick, k_tonemap, k_persist, k_persist_reset;
   int engine = SNL_ENGINE_PERSISTENT;
-  int tiles = 2, persistBlocks = 0, shadeThresh = 32, persistChunk = 64;
+  int tiles = 2, persistBlocks = 0, shadeThresh = 32, persistChunk = 64, marchBurst = 4;
   bool hasSurface = false, hasMetal = false, hasDiele = false;
   int maxBounces = 3, maxMarch = 256, spf = 1;
   SnlParams hostP;
@@ -270,6 +270,7 @@ void fillArgs(snl_prog p, snl_state st, SnlKArgs& a) {
   a.chunk = p->traceChunk;
   a.frames = 1;
   a.maxBounces = p->maxBounces;
+  a.burst = p->marchBurst;
 }
 
 int launch(snl_ctx ctx, hipFunction_t f, int grid, int block, SnlKArgs& a) {
@@ -1116,6 +1117,13 @@ int snl_set_tuning(snl_prog p, int traceBlocks, int traceChunk, int shadeBlocks)
   if (traceBlocks > 0) p->traceBlocks = traceBlocks;
   if (traceChunk > 0) p->traceChunk = traceChunk;
   if (shadeBlocks > 0) p->shadeBlocks = shadeBlocks;
+  return SNL_OK;
+}
+
+int snl_set_march_burst(snl_prog p, int steps) {
+  if (!p) return fail(nullptr, SNL_E_ARG, "prog is NULL");
+  if (steps < 1 || steps > 64) return fail(p->ctx, SNL_E_ARG, "march burst must be 1..64");
+  p->marchBurst = steps;
   return SNL_OK;
 }
 

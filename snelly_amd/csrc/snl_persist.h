@@ -109,48 +109,53 @@ __device__ inline void persist_body(const SnlKArgs& a) {
       continue;
     }
 
-    // ---- one march step for every tracing lane (traceRay, 881-920)
-    if (st == PS_TRACE) {
-      vec3 p = init ? o : o + t * d;
-      float sdf;
-      int m = pt.sdfStep(p, !init, sdf);
-      ++stepTotal;
-      bool done = false, hit = false;
-      if (init) {
-        init = false;
-        sgn = sign(sdf);
-        t = sgn * sdf;
-        if (abs(t) >= tmax) done = true;
-      } else if (m != MAT_INVAL) {
-        done = true;
-        hit = true;
-      } else {
-        t += sgn * sdf;
-        ++steps;
-        if (abs(t) >= tmax) done = true;
-        else if (steps >= SNL_MAX_MARCH_STEPS) { done = true; hit = !P.maxStepsIsMiss; m = -2; p = o + t * d; }
-      }
-      if (done) {
-        if (!shadow) {
-          hm = hit ? m : MAT_INVAL;
-          hp = p;
-          st = PS_SHADE;
+    // ---- up to a.burst march steps for every tracing lane (traceRay, 881-920);
+    // the shading gate is checked once per burst
+    for (int burst = 0; burst < a.burst; ++burst) {
+      if (st == PS_TRACE) {
+        vec3 p = init ? o : o + t * d;
+        float sdf;
+        int m = pt.sdfStep(p, !init, sdf);
+        ++stepTotal;
+        bool done = false, hit = false;
+        if (init) {
+          init = false;
+          sgn = sign(sdf);
+          t = sgn * sdf;
+          if (abs(t) >= tmax) done = true;
+        } else if (m != MAT_INVAL) {
+          done = true;
+          hit = true;
         } else {
-          // fold one shadow ray into the vertex's direct light (1236-1245)
-          RT Tr = rt1(0.0f);
-          if (!hit) Tr = pt.transmittanceOverFreeSegment(o, d, tmax, rgb);
-          RT f = light == 0 ? nf0 : (light == 1 ? nf1 : nf2);
-          RT Li = light == 0 ? nl0 : (light == 1 ? nl1 : nl2);
-          float den = light == 0 ? nden0 : (light == 1 ? nden1 : nden2);
-          float cs = light == 0 ? ncos0 : (light == 1 ? ncos1 : ncos2);
-          float w = light == 0 ? nmis0 : (light == 1 ? nmis1 : nmis2);
-          Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
-          if (averageComponent(Li) > RADIANCE_EPSILON) acc += f * Li / den * cs * w;
-          neeMask &= ~(1 << light);
-          if (neeMask) startLight(__ffs(neeMask) - 1);
-          else finishNee();
+          t += sgn * sdf;
+          ++steps;
+          if (abs(t) >= tmax) done = true;
+          else if (steps >= SNL_MAX_MARCH_STEPS) { done = true; hit = !P.maxStepsIsMiss; m = -2; p = o + t * d; }
+        }
+        if (done) {
+          if (!shadow) {
+            hm = hit ? m : MAT_INVAL;
+            hp = p;
+            st = PS_SHADE;
+          } else {
+            // fold one shadow ray into the vertex's direct light (1236-1245)
+            RT Tr = rt1(0.0f);
+            if (!hit) Tr = pt.transmittanceOverFreeSegment(o, d, tmax, rgb);
+            RT f = light == 0 ? nf0 : (light == 1 ? nf1 : nf2);
+            RT Li = light == 0 ? nl0 : (light == 1 ? nl1 : nl2);
+            float den = light == 0 ? nden0 : (light == 1 ? nden1 : nden2);
+            float cs = light == 0 ? ncos0 : (light == 1 ? ncos1 : ncos2);
+            float w = light == 0 ? nmis0 : (light == 1 ? nmis1 : nmis2);
+            Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
+            if (averageComponent(Li) > RADIANCE_EPSILON) acc += f * Li / den * cs * w;
+            neeMask &= ~(1 << light);
+            if (neeMask) startLight(__ffs(neeMask) - 1);
+            else finishNee();
+          }
         }
       }
+      if (__ballot(st == PS_TRACE) == 0ull) break;
+      if ((int)__popcll(__ballot(st == PS_CAMERA || st == PS_SHADE || st == PS_DONE)) >= a.shadeThresh) break;
     }
 
     // ---- shading, when enough lanes wait for it

@@ -52,6 +52,7 @@ struct SnlKArgs {
   int frames;             // persistent kernel: frames each pixel runs before the lane moves on
   int maxBounces;         // __MAX_BOUNCES__: the last vertex index a path may reach
   int imagePix;           // W * H; a state of several chains holds npix = chains * imagePix
+  int burst;              // persistent kernel: march steps per lane between shading checks
 };
 
 #define SNL_PF(k, i) a.pf[(long long)(k) * a.cap + (i)]

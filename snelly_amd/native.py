@@ -23,7 +23,7 @@ EXPORTS = [
     "snl_state_create", "snl_state_destroy", "snl_state_seed", "snl_state_clear", "snl_state_read",
     "snl_state_write", "snl_render", "snl_render_normals", "snl_pick", "snl_tonemap", "snl_sync",
     "snl_get_stats", "snl_set_tuning", "snl_set_profiling", "snl_set_engine", "snl_render_batch",
-    "snl_state_create_chains", "snl_state_error", "snl_ctx_mem_info", "snl_state_bytes",
+    "snl_state_create_chains", "snl_state_error", "snl_ctx_mem_info", "snl_state_bytes", "snl_set_march_burst",
 ]
 ENGINE_WAVEFRONT, ENGINE_PERSISTENT = 0, 1
 
@@ -83,6 +83,7 @@ def lib():
         "snl_state_error": (ip, [vp, ctypes.POINTER(ctypes.c_double)]),
         "snl_ctx_mem_info": (ip, [vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_ulonglong)]),
         "snl_state_bytes": (ctypes.c_ulonglong, [ip, ip, ip]),
+        "snl_set_march_burst": (ip, [vp, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -280,6 +281,9 @@ class Program:
 
     def tune(self, trace_blocks=0, trace_chunk=0, shade_blocks=0):
         self.ctx.check(lib().snl_set_tuning(self.h, trace_blocks, trace_chunk, shade_blocks))
+
+    def march_burst(self, steps):
+        self.ctx.check(lib().snl_set_march_burst(self.h, steps))
 
     def engine(self, engine, tiles=0, persist_blocks=0, shade_thresh=0, chunk=0):
         self.ctx.check(lib().snl_set_engine(self.h, engine, tiles, persist_blocks, shade_thresh, chunk))

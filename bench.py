@@ -32,6 +32,9 @@ os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("SNL_BENCH_HW_QUEUES", "16")
 METRIC = "path samples/s (plain vs batched/served) at RMSE≤1%; p50/p99 job latency"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+# HBM bytes per pixel per snl_persist launch, from the FETCH_SIZE / WRITE_SIZE
+# counter runs scaled by the copy yardstick (profiles/snl_persist_menger4_1024_traffic_valu.md)
+PERSIST_TRAFFIC_B_PER_PX = 67.1
 
 
 def unbiased(rad):
@@ -200,8 +203,11 @@ def main():
     # central-difference normal (6 SDF evaluations); shading is not counted.
     flops = trace_steps * scene.flops_per_step + normal_evals * 6 * scene.sdf_flops
     achieved = flops / (trace_ms * 1e-3) / 1e12
+    pixels_per_launch = W * H * args.steps / launches
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                "traffic": round(PERSIST_TRAFFIC_B_PER_PX * pixels_per_launch),
+                "traffic_algorithmic": 64 * pixels_per_launch,
                 "kernel": "snl_persist", "launches": launches, "gpu_ms": round(trace_ms, 2),
                 "flops_per_launch": flops / launches, "flops_per_step": scene.flops_per_step,
                 "steps_per_launch": trace_steps / launches, "normal_evals_per_launch": normal_evals / launches,

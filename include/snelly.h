@@ -128,11 +128,15 @@ int snl_set_profiling(snl_prog prog, int on);
  * shade launch sequence over SoA path queues.  PERSISTENT: one launch per
  * pixel tile in which every lane runs whole paths from registers (no
  * per-bounce barrier) and a pixel's frames back to back; tiles run on separate
- * streams.  Both draw the same random numbers per path.  Programs whose
- * surface has sub-surface scattering always render with the wavefront engine.
- * 0 keeps a knob. */
+ * streams.  MEGAKERNEL: the same launch shape, every lane running cameraPath
+ * as written, shadow and probe rays traced in place.  All draw the same random
+ * numbers per path.  Programs with the homogeneous atmosphere (HAS_ATMOSPHERE)
+ * always render with the megakernel; those whose surface has sub-surface
+ * scattering use the wavefront engine unless the megakernel is chosen (and the
+ * megakernel in snl_render_batch).  0 keeps a knob. */
 #define SNL_ENGINE_WAVEFRONT 0
 #define SNL_ENGINE_PERSISTENT 1
+#define SNL_ENGINE_MEGAKERNEL 2
 int snl_set_engine(snl_prog prog, int engine, int tiles, int persist_blocks, int shade_thresh, int chunk);
 /* persistent engine: march steps a tracing lane takes between two checks of
  * the shading gate (1..64, default 4); results do not depend on it */

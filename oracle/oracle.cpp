@@ -6,14 +6,16 @@
 // loop iteration, cameraPath with inline traceRay and shadow rays):
 //   shaders/pathtracer-fragment-shader.glsl  (rand 119-130, BSDFs 379-852,
 //     traceRay 881-920, normal 922-940, lights 1051-1212,
-//     directSurfaceLighting 1221-1295, randomwalk_SSS 1468-1593,
+//     transmittance 1003-1050, directSurfaceLighting 1221-1295,
+//     atmosphere 1299-1457, randomwalk_SSS 1468-1593,
 //     constructPrimaryRay 1602-1622, cameraPath 1625-1926, pathtrace 1934-1989)
 //   shaders/normals-fragment-shader.glsl     (traceDistance 70-113, main 293-345)
 //   shaders/pick-fragment-shader.glsl        (main 133-152)
 //   shaders/tonemapper-fragment-shader.glsl  (main 36-97)
 // Scenes are hand-written C++ SDFs (no GLSL translation), so the oracle is
-// independent of the product's scene compiler.  The homogeneous atmosphere
-// (HAS_ATMOSPHERE) is not restated here.
+// independent of the product's scene compiler.  Scene hooks keep their
+// defaults (constant subsurface albedo, ...) except where a scene selects one
+// of the hand-written variants (surface albedo, volume emission field).
 //
 // Parity status of each part is recorded in the tests that pin it; see
 // tests/test_oracle.py (closed forms, invariants, brute force).
@@ -53,6 +55,10 @@ struct Params {
   float dieleRoughness, dieleAbsorptionRGB[3], dieleSpecAlbedoRGB[3];
   float surfaceDiffuseAlbedoRGB[3], surfaceSpecAlbedoRGB[3], surfaceRoughness, surfaceIor, subsurface;
   float subsurfaceAlbedoRGB[3], subsurfaceMFP, subsurfaceAnisotropy, subsurfaceDiffuseWeight, maxSSSSteps;
+  float atmosphereBoundsMin[3], atmosphereBoundsMax[3], atmosphereExtinctionCoeffRGB[3];
+  float atmosphereScatteringCoeffRGB[3], atmosphereMFP, atmosphereAnisotropy;
+  float fogEnable, fogMaxOpticalDepth, fogMFP, fogEmission[3], fogTint[3];
+  float volumeEmission, volumeEmissionColorRGB[3];
   float exposure, invGamma, contrast, saturation, hueShift;
 };
 Params P;
@@ -85,13 +91,21 @@ std::vector<Named> table() {
       {"subsurfaceAnisotropy", &P.subsurfaceAnisotropy, 1},
       {"subsurfaceDiffuseWeight", &P.subsurfaceDiffuseWeight, 1}, {"maxSSSSteps", &P.maxSSSSteps, 1},
       {"exposure", &P.exposure, 1}, {"invGamma", &P.invGamma, 1}, {"contrast", &P.contrast, 1},
-      {"saturation", &P.saturation, 1}, {"hueShift", &P.hueShift, 1}};
+      {"saturation", &P.saturation, 1}, {"hueShift", &P.hueShift, 1},
+      {"atmosphereBoundsMin", P.atmosphereBoundsMin, 3}, {"atmosphereBoundsMax", P.atmosphereBoundsMax, 3},
+      {"atmosphereExtinctionCoeffRGB", P.atmosphereExtinctionCoeffRGB, 3},
+      {"atmosphereScatteringCoeffRGB", P.atmosphereScatteringCoeffRGB, 3}, {"atmosphereMFP", &P.atmosphereMFP, 1},
+      {"atmosphereAnisotropy", &P.atmosphereAnisotropy, 1}, {"fogEnable", &P.fogEnable, 1},
+      {"fogMaxOpticalDepth", &P.fogMaxOpticalDepth, 1}, {"fogMFP", &P.fogMFP, 1},
+      {"fogEmission", P.fogEmission, 3}, {"fogTint", P.fogTint, 3},
+      {"volumeEmission", &P.volumeEmission, 1}, {"volumeEmissionColorRGB", P.volumeEmissionColorRGB, 3}};
 }
 inline V3 A3(const float* a) { return v3(a[0], a[1], a[2]); }
 
 // ---------------------------------------------------------------- build-time switches (the #defines)
 struct Config {
   int dispersive = 0, interactive = 0, maxBounces = 3, maxMarch = 256, spf = 1, sphereLight = 0;
+  int atmosphere = 0, fog = 0, maxAtmScatters = 1;  // HAS_ATMOSPHERE, HAS_FOG, __MAX_ATMOSPHERE_SCATTERS__
 } C;
 
 // ---------------------------------------------------------------- scene (hand-written SDFs)
@@ -99,6 +113,7 @@ enum { SDF_NONE, SDF_SPHERE, SDF_SPHERE_PLANE, SDF_MENGER, SDF_MANDELBOX, SDF_TO
 struct SdfSpec { int kind = SDF_NONE; float a = 0, b = 0, c = 0, d = 0; };
 SdfSpec sdfs[3];        // indexed by material id (DIELE, METAL, SURFA)
 int albedoHook = 0;     // 1: sphere/plane albedo (0.8 above y=-1+1e-3, else 0.5)
+int emissionHook = 0;   // HAS_VOLUME_EMISSION; 1: VOLUME_EMISSION = emission where X.y > 0, else emission/4
 
 // SDFs are templates on the scalar type: float for rendering, Counted for the
 // algorithmic FLOP count of one evaluation (+ - * / and sqrt count 1 each;
@@ -320,6 +335,8 @@ inline float minR(float a, float b) { return fmin_(a, b); }
 inline V3 minR(V3 a, V3 b) { return vmin(a, b); }
 inline float expR(float a) { return std::exp(a); }
 inline V3 expR(V3 a) { return vexp(a); }
+inline float lenR(float a) { return std::fabs(a); }
+inline float lenR(V3 a) { return length(a); }
 inline float absR(float a) { return std::fabs(a); }
 inline V3 absR(V3 a) { return vabs(a); }
 inline float mixR(float x, float y, float a) { return x * (1.0f - a) + y * a; }
@@ -361,6 +378,15 @@ V3 rgbToXyz(V3 RGB) {
 }
 
 // ---------------------------------------------------------------- sampling (300-372)
+V3 sampleSphereUniformly(float* rnd) {
+  float z = 1.0f - 2.0f * randf(rnd);
+  float phi = 2.0f * PI_F * randf(rnd);
+  return v3(std::cos(phi), std::sin(phi), z);  // as the reference: x, y not scaled by sqrt(1 - z^2)
+}
+float phaseFunction(float mu, float g) {
+  float gSqr = g * g;
+  return (1.0f / (4.0f * PI_F)) * (1.0f - gSqr) / std::pow(1.0f - 2.0f * g * mu + gSqr, 1.5f);
+}
 V3 sampleHemisphereUniformly(float* rnd) {
   float xi = randf(rnd);
   float r = std::sqrt(1.0f - xi * xi);
@@ -722,9 +748,61 @@ struct Tracer {
     pdfDir = pdfArea * dHit2 / fmax_(DENOM_TOLERANCE, costheta);
     return A(P.sphereLightPower * A3(P.sphereLightColor));
   }
-  R shadowFactor(V3 o, V3 d, float len) const {
+
+  // homogeneous atmosphere and fog (856-868, 975-1050)
+  R VOLUME_SCATTERING_EVAL() const { return A(A3(P.atmosphereScatteringCoeffRGB) / (P.atmosphereMFP * P.lengthScale)); }
+  R VOLUME_EXTINCTION_EVAL() const { return A(A3(P.atmosphereExtinctionCoeffRGB) / (P.atmosphereMFP * P.lengthScale)); }
+  R VOLUME_EMISSION_EVAL(V3 X) const {
+    V3 e = A3(P.volumeEmissionColorRGB) * P.volumeEmission;
+    return A(X.y > 0.0f ? e : e * 0.25f);
+  }
+  bool atmosphereHit(V3 pos, V3 dir, float& t0, float& t1) const {
+    V3 dL = v3(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
+    V3 lo = (A3(P.atmosphereBoundsMin) * P.lengthScale - pos) * dL;
+    V3 hi = (A3(P.atmosphereBoundsMax) * P.lengthScale - pos) * dL;
+    V3 tmp = vmin(lo, hi);
+    hi = lo + hi - tmp;
+    lo = tmp;
+    bool hit = !(lo.x > hi.y || lo.y > hi.x || lo.x > hi.z || lo.z > hi.x || lo.y > hi.z || lo.z > hi.y);
+    t0 = fmax_(fmax_(lo.x, lo.y), lo.z);
+    t1 = fmin_(fmin_(hi.x, hi.y), hi.z);
+    return hit;
+  }
+  bool atmosphereSegment(V3 pW, V3 dir, float len, float& t0, float& t1) const {
+    if (!atmosphereHit(pW, dir, t0, t1)) return false;
+    t0 = fmax_(0.0f, t0);
+    if (t1 < 0.0f || len < t0) return false;
+    t1 = fmin_(len, t1);
+    return true;
+  }
+  R transmittanceOverFreeSegment(V3 pW, V3 dir, float len) const {
+    R Tr = one();
+    if (C.atmosphere) {
+      R ext = VOLUME_EXTINCTION_EVAL();
+      if (lenR(ext) == 0.0f) return one();
+      float t0 = 0.0f, t1 = 0.0f;
+      if (!atmosphereSegment(pW, dir, len, t0, t1)) return one();
+      Tr = expR(-((t1 - t0) * ext));
+    }
+    if (C.fog && P.fogEnable != 0.0f) {
+      V3 fogAbsorption = v3(1.0f) - A3(P.fogTint);
+      V3 depth = P.fogMaxOpticalDepth * (v3(1.0f) - vexp(-(len * fogAbsorption / P.fogMFP)));
+      Tr = Tr * expR(-A(depth));
+    }
+    return Tr;
+  }
+  R transmittanceOverSegment(V3 pW, V3 dir, float len) const {
     V3 h; int m;
-    R Tr = traceRay(o, d, h, m, len) ? zero() : one();
+    if (traceRay(pW, dir, h, m, len)) return zero();
+    return transmittanceOverFreeSegment(pW, dir, len);
+  }
+  float atmosphericScatteringProbability(R Tr) const {
+    if (!C.atmosphere) return 0.0f;
+    R ext = VOLUME_EXTINCTION_EVAL(), sca = VOLUME_SCATTERING_EVAL();
+    return avgC(sca / ext) * (1.0f - avgC(Tr));
+  }
+  R shadowFactor(V3 o, V3 d, float len) const {
+    R Tr = transmittanceOverSegment(o, d, len);
     return absR(one() - P.shadowStrength * (one() - Tr));
   }
 
@@ -786,13 +864,182 @@ struct Tracer {
     return minR(rt(P.radianceClamp, R()), Ld);
   }
 
-  // cameraPath (1625-1926) without the atmosphere and sub-surface walk
+  // directVolumeLighting (1299-1363)
+  R directVolumeLighting(V3 pW, V3 rayDir, float* rnd) const {
+    R Ld = zero();
+    if (P.skyPower > RADIANCE_EPSILON) {
+      V3 wW = v3(0.0f);
+      float skyPdf = 0.0f;
+      R Li = zero();
+      if (!(P.skyPower < RADIANCE_EPSILON)) {
+        wW = sampleSphereUniformly(rnd);
+        skyPdf = 1.0f / (4.0f * PI_F);
+        Li = environmentRadiance(wW);
+      }
+      if (avgC(Li) > RADIANCE_EPSILON) {
+        Li = Li * shadowFactor(pW, wW, P.maxLengthScale);
+        if (avgC(Li) > RADIANCE_EPSILON)
+          Ld = Ld + phaseFunction(dot(-rayDir, -wW), P.atmosphereAnisotropy) * Li / fmax_(PDF_EPSILON, skyPdf);
+      }
+    }
+    if (P.sunPower > RADIANCE_EPSILON) {
+      float sunPdf = 0.0f;
+      V3 wW = sampleSunDir(rnd, sunPdf);
+      R Li = sunRadiance(wW);
+      if (avgC(Li) > RADIANCE_EPSILON) {
+        Li = Li * shadowFactor(pW, wW, P.maxLengthScale);
+        if (avgC(Li) > RADIANCE_EPSILON)
+          Ld = Ld + phaseFunction(dot(-rayDir, -wW), P.atmosphereAnisotropy) * Li / fmax_(PDF_EPSILON, sunPdf);
+      }
+    }
+    if (C.sphereLight && P.sphereLightPower > RADIANCE_EPSILON) {
+      float sphPdf = 0.0f;
+      V3 pHit;
+      R Li = sampleSphereLight(rnd, pW, pHit, sphPdf);
+      if (avgC(Li) > RADIANCE_EPSILON) {
+        float hitDist = fmax_(0.0f, length(pHit - pW) - 3.0f * P.minLengthScale);
+        V3 wW = normalize(pHit - pW);
+        Li = Li * shadowFactor(pW, wW, hitDist);
+        if (avgC(Li) > RADIANCE_EPSILON)
+          Ld = Ld + phaseFunction(dot(-rayDir, -wW), P.atmosphereAnisotropy) * Li / fmax_(PDF_EPSILON, sphPdf);
+      }
+    }
+    return minR(rt(P.radianceClamp, R()), Ld);
+  }
+
+  // a scatter distance in the atmosphere part [t0, t1] of the segment, drawn
+  // with density proportional to the transmittance (1392-1398, 1430-1436)
+  bool scatterPoint(V3 pW, V3 rayDir, float len, float* rnd, V3& pS, R& TrS, float& invPdf) const {
+    R ext = VOLUME_EXTINCTION_EVAL();
+    float en = avgC(ext);
+    float t0 = 0.0f, t1 = 0.0f;
+    if (!atmosphereSegment(pW, rayDir, len, t0, t1)) return false;
+    float T01 = std::exp(-(t1 - t0) * en);
+    float ts = t0 - std::log(1.0f - randf(rnd) * (1.0f - T01)) / en;
+    pS = pW + ts * rayDir;
+    R od = (ts - t0) * ext;
+    TrS = expR(-od);
+    invPdf = (1.0f - T01) * std::exp(avgC(od)) / en;
+    return true;
+  }
+
+  // atmosphericInscatteringRadiance (1365-1417)
+  R atmosphericInscatteringRadiance(V3 pW, V3 rayDir, float len, float* rnd) const {
+    R Ls = zero();
+    if (C.fog && P.fogEnable != 0.0f) Ls = (one() - transmittanceOverFreeSegment(pW, rayDir, len)) * A(A3(P.fogEmission));
+    R ext = VOLUME_EXTINCTION_EVAL(), sca = VOLUME_SCATTERING_EVAL();
+    if (avgC(ext) < RADIANCE_EPSILON || avgC(sca) < RADIANCE_EPSILON) return Ls;
+    V3 pS;
+    R TrS;
+    float invPdf;
+    if (!scatterPoint(pW, rayDir, len, rnd, pS, TrS, invPdf)) return Ls;
+    R Li = directVolumeLighting(pS, rayDir, rnd);
+    Ls = Ls + TrS * sca * Li * invPdf;
+    if (emissionHook) {
+      V3 wo = samplePhaseFunction(rayDir, P.atmosphereAnisotropy, rnd);
+      V3 h;
+      int m;
+      if (traceRay(pS, wo, h, m, P.maxLengthScale)) Ls = Ls + TrS * sca * VOLUME_EMISSION_EVAL(h) * invPdf;
+    }
+    return Ls;
+  }
+
+  // atmosphericScatterSample (1419-1442)
+  R atmosphericScatterSample(V3 pW, V3 rayDir, float len, float* rnd, V3& pS, V3& wo) const {
+    R TrS;
+    float invPdf;
+    if (!scatterPoint(pW, rayDir, len, rnd, pS, TrS, invPdf)) return zero();
+    wo = samplePhaseFunction(rayDir, P.atmosphereAnisotropy, rnd);
+    return VOLUME_SCATTERING_EVAL() * TrS * invPdf;
+  }
+
+  // randomwalk_SSS (1468-1518)
+  bool randomwalkSSS(V3 pW, const Basis& b, float* rnd, R albedoSSS, R& walkT, V3& pExit) const {
+    V3 pWalk = pW - 3.0f * P.minLengthScale * b.nW;
+    float pdfDir;
+    V3 dL = sampleHemisphereCosineWeighted(rnd, pdfDir);
+    V3 dW = -localToWorld(dL, b);
+    walkT = one();
+    for (int n = 0; n < 2048; ++n) {
+      if ((float)n > P.maxSSSSteps) break;
+      float step = -std::log(randf(rnd)) * P.subsurfaceMFP * P.lengthScale;
+      V3 h;
+      int m;
+      if (traceRay(pWalk, dW, h, m, step)) {
+        pExit = h;
+        return true;
+      }
+      if (n > 4) {
+        float cont = fclamp(10.0f * maxC(walkT), 0.0f, 1.0f);
+        if (randf(rnd) < 1.0f - cont) break;
+        walkT = walkT / cont;
+      }
+      dW = samplePhaseFunction(dW, P.subsurfaceAnisotropy, rnd);
+      pWalk = pWalk + step * dW;
+      walkT = walkT * albedoSSS;
+    }
+    return false;
+  }
+
+  // SSS_exit_radiance (1521-1593): a Lambertian lobe at the exit point
+  R exitRadianceSSS(V3 pW, const Basis& b, float* rnd, R albedoExit, float& skyPdf, float& sunPdf, float& sphPdf) const {
+    V3 dPw = 3.0f * P.minLengthScale * b.nW;
+    R Ld = zero();
+    R f = mixR(one(), albedoExit, P.subsurfaceDiffuseWeight) / PI_F;
+    if (P.skyPower > RADIANCE_EPSILON) {
+      V3 wL = v3(0.0f), wW = v3(0.0f);
+      R Li = zero();
+      if (!(P.skyPower < RADIANCE_EPSILON)) {
+        wL = sampleHemisphereCosineWeighted(rnd, skyPdf);
+        wW = localToWorld(wL, b);
+        Li = environmentRadiance(wW);
+      }
+      if (avgC(Li) > RADIANCE_EPSILON) {
+        Li = Li * shadowFactor(pW + dPw, wW, P.maxLengthScale);
+        if (avgC(Li) > RADIANCE_EPSILON) {
+          float w = powerHeuristic(skyPdf, pdfHemisphereCosineWeighted(wL));
+          Ld = Ld + f * Li / fmax_(PDF_EPSILON, skyPdf) * std::fabs(dot(wW, b.nW)) * w;
+        }
+      }
+    }
+    if (P.sunPower > RADIANCE_EPSILON) {
+      V3 wW = sampleSunDir(rnd, sunPdf);
+      V3 wL = worldToLocal(wW, b);
+      R Li = wL.z < 0.0f ? zero() : sunRadiance(wW);
+      if (avgC(Li) > RADIANCE_EPSILON) {
+        Li = Li * shadowFactor(pW + dPw, wW, P.maxLengthScale);
+        if (avgC(Li) > RADIANCE_EPSILON) {
+          float w = powerHeuristic(sunPdf, pdfHemisphereCosineWeighted(wL));
+          Ld = Ld + f * Li / fmax_(PDF_EPSILON, sunPdf) * std::fabs(dot(wW, b.nW)) * w;
+        }
+      }
+    }
+    if (C.sphereLight && (P.sphereLightPower > RADIANCE_EPSILON) &&
+        (dot(pW - A3(P.sphereLightPosition), b.nW) <= P.sphereLightRadius)) {
+      V3 pHit;
+      R Li = sampleSphereLight(rnd, pW, pHit, sphPdf);
+      if (avgC(Li) > RADIANCE_EPSILON) {
+        float hitDist = fmax_(0.0f, length(pHit - pW) - 3.0f * P.minLengthScale);
+        V3 wW = normalize(pHit - pW);
+        V3 wL = worldToLocal(wW, b);
+        Li = Li * shadowFactor(pW + dPw, wW, hitDist);
+        if (avgC(Li) > RADIANCE_EPSILON) {
+          float w = powerHeuristic(sphPdf, pdfHemisphereCosineWeighted(wL));
+          Ld = Ld + f * Li / fmax_(PDF_EPSILON, sphPdf) * std::fabs(dot(wW, b.nW)) * w;
+        }
+      }
+    }
+    return minR(rt(P.radianceClamp, R()), Ld);
+  }
+
+  // cameraPath (1625-1926)
   R cameraPath(V3 start, V3 dir, float* rnd) const {
     R L = zero();
     float misSky = 1.0f, misSun = 1.0f, misSph = 1.0f;
     V3 pW = start, rayDir = dir;
     bool inDielectric = hasMat(MAT_DIELE) && evalSdf(sdfs[MAT_DIELE], start) < 0.0f;
     R throughput = one();
+    int scatters = 0;
     for (int vertex = 0; vertex <= C.maxBounces; ++vertex) {
       if (maxC(throughput) < THROUGHPUT_EPSILON) break;
       V3 pNext;
@@ -801,62 +1048,148 @@ struct Tracer {
       float rayLength = P.maxLengthScale;
       if (hit) rayLength = length(pNext - pW);
       else {
+        float segLen = P.maxLengthScale;
+        R Tr = one();
         bool lit = false;
         if (C.sphereLight) {
           V3 pl;
           R LiSph = (P.sphereLightPower < RADIANCE_EPSILON) ? zero() : sphereLightRadiance(pW, rayDir, pl);
           if (avgC(LiSph) > RADIANCE_EPSILON) {
             lit = true;
-            L = L + throughput * one() * misSph * LiSph / (1.0f - 0.0f);
+            segLen = length(pl - pW);
+            Tr = transmittanceOverFreeSegment(pW, rayDir, segLen);
+            float sp = atmosphericScatteringProbability(Tr);
+            L = L + throughput * Tr * misSph * LiSph / (1.0f - sp);
           }
         }
         if (!lit) {
-          R Tr = one();
+          segLen = P.maxLengthScale;
+          Tr = transmittanceOverFreeSegment(pW, rayDir, segLen);
+          float sp = atmosphericScatteringProbability(Tr);
           if (avgC(Tr) > RADIANCE_EPSILON) {
             if (!(vertex == 0 && P.envMapVisible == 0.0f) && misSky > 0.0f)
-              L = L + throughput * Tr * misSky * environmentRadiance(rayDir) / (1.0f - 0.0f);
+              L = L + throughput * Tr * misSky * environmentRadiance(rayDir) / (1.0f - sp);
             if (!(vertex == 0 && P.sunVisibleDirectly == 0.0f) && misSun > 0.0f)
-              L = L + throughput * Tr * misSun * sunRadiance(rayDir) / (1.0f - 0.0f);
+              L = L + throughput * Tr * misSun * sunRadiance(rayDir) / (1.0f - sp);
+          }
+        }
+        if (C.atmosphere) {
+          if (vertex < 2) L = L + throughput * atmosphericInscatteringRadiance(pW, rayDir, segLen, rnd);
+          if (scatters + 1 < C.maxAtmScatters) {
+            float sp = atmosphericScatteringProbability(Tr);
+            if (sp > randf(rnd)) {
+              V3 pS = v3(0.0f), wo = v3(0.0f);
+              throughput = throughput * atmosphericScatterSample(pW, rayDir, segLen, rnd, pS, wo) / fmax_(DENOM_TOLERANCE, sp);
+              pW = pS;
+              rayDir = wo;
+              scatters++;
+              continue;
+            }
           }
         }
         break;
       }
       if (hm == MAT_INVAL) break;  // max-step "hit" without a material: undefined in the reference; ends the path
+      R Tr = one();
+      bool hitSphereLight = false;
       if (C.sphereLight) {
         V3 pl;
         R LiSph = (P.sphereLightPower < RADIANCE_EPSILON) ? zero() : sphereLightRadiance(pW, rayDir, pl);
         if (avgC(LiSph) > RADIANCE_EPSILON) {
           float lsl = length(pl - pW);
-          if (lsl < rayLength) L = L + throughput * one() * misSph * LiSph;
+          if (lsl < rayLength) {
+            hitSphereLight = true;
+            Tr = transmittanceOverFreeSegment(pW, rayDir, lsl);
+            L = L + throughput * Tr * misSph * LiSph;
+            if (C.atmosphere && vertex < 2) L = L + throughput * atmosphericInscatteringRadiance(pW, rayDir, lsl, rnd);
+          }
         }
       }
       if (hasMat(MAT_DIELE) && inDielectric) {
         R absorption = A(A3(P.dieleAbsorptionRGB));
         throughput = throughput * expR(-rayLength * absorption);
       }
+      if (C.atmosphere) {
+        if (vertex < 2) L = L + throughput * atmosphericInscatteringRadiance(pW, rayDir, rayLength, rnd);
+        if (!hitSphereLight) Tr = transmittanceOverFreeSegment(pW, rayDir, rayLength);
+        if (scatters + 1 < C.maxAtmScatters) {
+          float sp = atmosphericScatteringProbability(Tr);
+          if (sp > randf(rnd)) {
+            V3 pS = v3(0.0f), wo = v3(0.0f);
+            throughput = throughput * atmosphericScatterSample(pW, rayDir, rayLength, rnd, pS, wo) / fmax_(DENOM_TOLERANCE, sp);
+            pW = pS;
+            rayDir = wo;
+            scatters++;
+            continue;
+          }
+          throughput = throughput / fmax_(DENOM_TOLERANCE, 1.0f - sp);
+        }
+        if (hitSphereLight) break;
+        throughput = throughput * transmittanceOverFreeSegment(pW, rayDir, rayLength);
+      }
       pW = pNext;
       V3 nW = normal(pW, hm);
       V3 ngW = nW;
       Basis b = makeBasis(nW);
-      V3 winW = -rayDir;
-      V3 wi = worldToLocal(winW, b);
-      // WebGL zero-initialises uninitialised locals (woutputL, bsdfPdf)
-      V3 wo = v3(0.0f);
-      float bsdfPdf = 0.0f;
-      R f = sampleBsdf(pW, b, wi, hm, wo, bsdfPdf, rnd);
-      V3 woW = localToWorld(wo, b);
-      if (hm == MAT_DIELE && dot(winW, ngW) * dot(woW, ngW) < 0.0f) inDielectric = !inDielectric;
-      rayDir = woW;
-      pW = pW + ngW * fsign(dot(rayDir, ngW)) * 3.0f * P.minLengthScale;
-      float skyPdf = 0.0f, sunPdf = 0.0f, sphPdf = 0.0f;
-      if (!(hasMat(MAT_DIELE) && inDielectric))
-        L = L + throughput * directSurfaceLighting(pW, b, winW, hm, rnd, skyPdf, sunPdf, sphPdf);
-      R fOverPdf = minR(rt(P.radianceClamp, R()), f / fmax_(PDF_EPSILON, bsdfPdf));
-      R st = fOverPdf * std::fabs(dot(woW, nW)) / fmax_(PDF_EPSILON, 1.0f - 0.0f);
-      throughput = throughput * st;
-      misSky = powerHeuristic(bsdfPdf, skyPdf);
-      misSun = powerHeuristic(bsdfPdf, sunPdf);
-      misSph = powerHeuristic(bsdfPdf, sphPdf);
+      bool sss = false;
+      float probSSS = 0.0f;
+      R albedoSSS = zero();
+      if (hasMat(MAT_SURFA) && P.subsurfaceMFP > 0.0f) {
+        albedoSSS = A(A3(P.subsurfaceAlbedoRGB));
+        float diffW = (1.0f - P.subsurface) * avgC(diffuseRefl(pW));
+        float specW = avgC(A(A3(P.surfaceSpecAlbedoRGB)));
+        float sssW = P.subsurface * avgC(albedoSSS) * 3.0f;
+        float total = diffW + specW + sssW;
+        probSSS = sssW / (total + DENOM_TOLERANCE);
+        probSSS = fclamp(probSSS, 0.0f, 1.0f - PDF_EPSILON);
+        sss = randf(rnd) < probSSS;
+      }
+      if (!sss) {
+        V3 winW = -rayDir;
+        V3 wi = worldToLocal(winW, b);
+        // WebGL zero-initialises uninitialised locals (woutputL, bsdfPdf)
+        V3 wo = v3(0.0f);
+        float bsdfPdf = 0.0f;
+        R f = sampleBsdf(pW, b, wi, hm, wo, bsdfPdf, rnd);
+        V3 woW = localToWorld(wo, b);
+        if (hm == MAT_DIELE && dot(winW, ngW) * dot(woW, ngW) < 0.0f) inDielectric = !inDielectric;
+        if (emissionHook) L = L + throughput * VOLUME_EMISSION_EVAL(pW);
+        rayDir = woW;
+        pW = pW + ngW * fsign(dot(rayDir, ngW)) * 3.0f * P.minLengthScale;
+        float skyPdf = 0.0f, sunPdf = 0.0f, sphPdf = 0.0f;
+        if (!(hasMat(MAT_DIELE) && inDielectric))
+          L = L + throughput * directSurfaceLighting(pW, b, winW, hm, rnd, skyPdf, sunPdf, sphPdf);
+        R fOverPdf = minR(rt(P.radianceClamp, R()), f / fmax_(PDF_EPSILON, bsdfPdf));
+        R st = fOverPdf * std::fabs(dot(woW, nW)) / fmax_(PDF_EPSILON, 1.0f - probSSS);
+        throughput = throughput * st;
+        misSky = powerHeuristic(bsdfPdf, skyPdf);
+        misSun = powerHeuristic(bsdfPdf, sunPdf);
+        misSph = powerHeuristic(bsdfPdf, sphPdf);
+      } else {
+        R walkT = one();
+        V3 pExit = v3(0.0f);
+        b.nW = ngW;  // the entry uses the geometric normal
+        if (!randomwalkSSS(pW, b, rnd, albedoSSS, walkT, pExit)) break;
+        pW = pExit;
+        nW = normal(pExit, MAT_SURFA);
+        Basis be = makeBasis(nW);
+        float bsdfPdf;
+        V3 dL = sampleHemisphereCosineWeighted(rnd, bsdfPdf);
+        V3 woW = localToWorld(dL, be);
+        rayDir = woW;
+        throughput = throughput * walkT;
+        float skyPdf = 0.0f, sunPdf = 0.0f, sphPdf = 0.0f;
+        R albedoExit = diffuseRefl(pW);
+        if (!(hasMat(MAT_DIELE) && inDielectric))
+          L = L + throughput * exitRadianceSSS(pExit, be, rnd, albedoExit, skyPdf, sunPdf, sphPdf);
+        misSky = powerHeuristic(bsdfPdf, skyPdf);
+        misSun = powerHeuristic(bsdfPdf, sunPdf);
+        misSph = powerHeuristic(bsdfPdf, sphPdf);
+        R f = mixR(one(), albedoExit, P.subsurfaceDiffuseWeight) / PI_F;
+        R fOverPdf = minR(rt(P.radianceClamp, R()), f / fmax_(PDF_EPSILON, bsdfPdf));
+        throughput = throughput * (fOverPdf * std::fabs(dot(woW, nW)) / fmax_(PDF_EPSILON, probSSS));
+        pW = pW + nW * fsign(dot(rayDir, nW)) * 3.0f * P.minLengthScale;
+      }
       if (maxC(throughput) < THROUGHPUT_EPSILON) break;
     }
     return L;
@@ -981,6 +1314,7 @@ void oracle_reset() {
   C = Config();
   for (auto& s : sdfs) s = SdfSpec();
   albedoHook = 0;
+  emissionHook = 0;
   dieleModel = 1;
   dieleC.clear();
   metalModel = 0;
@@ -1006,6 +1340,11 @@ void oracle_sdf(int material, int kind, float a, float b, float c, float d) {
   sdfs[material].kind = kind; sdfs[material].a = a; sdfs[material].b = b; sdfs[material].c = c; sdfs[material].d = d;
 }
 void oracle_albedo_hook(int kind) { albedoHook = kind; }
+void oracle_emission_hook(int kind) { emissionHook = kind; }
+// HAS_ATMOSPHERE, HAS_FOG and __MAX_ATMOSPHERE_SCATTERS__
+void oracle_volume(int atmosphere, int fog, int maxAtmScatters) {
+  C.atmosphere = atmosphere; C.fog = fog; C.maxAtmScatters = maxAtmScatters;
+}
 void oracle_dielectric(int model, const float* coeffs, int n) { dieleModel = model; dieleC.assign(coeffs, coeffs + n); }
 void oracle_metal(int model, const float* n64, const float* k64) {
   metalModel = model;
@@ -1118,7 +1457,6 @@ int oracle_normals(float* radiance, float* rng, int W, int H, int frames, int th
 
 // pick program (pick-fragment-shader.glsl:116-152)
 int oracle_pick(const float* xy, int n, float* out) {
-  Tracer<V3> T;
   for (int k = 0; k < n; ++k) {
     float dummy[4] = {1, 1, 1, 1};
     V3 s, d;

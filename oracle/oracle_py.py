@@ -32,6 +32,8 @@ def lib():
         L.oracle_config.argtypes = [ctypes.c_int] * 6
         L.oracle_sdf.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 4
         L.oracle_albedo_hook.argtypes = [ctypes.c_int]
+        L.oracle_volume.argtypes = [ctypes.c_int] * 3
+        L.oracle_emission_hook.argtypes = [ctypes.c_int]
         L.oracle_dielectric.argtypes = [ctypes.c_int, fp, ctypes.c_int]
         L.oracle_metal.argtypes = [ctypes.c_int, fp, fp]
         L.oracle_luts.argtypes = [fp, fp]
@@ -86,9 +88,13 @@ class Oracle:
         sphere_light = 1 if r.sphereLightPower > -7.0 else 0
         L.oracle_config(int(r.dispersive), int(r.interactive), int(round(r.maxBounces)),
                         int(round(r.maxMarchSteps)), int(round(r.maxSamplesPerFrame)), sphere_light)
+        v = mats.volume
+        has_atm = any(c > 0.0 for c in v.scatteringColor) or any(c > 0.0 for c in v.absorptionColor)
+        L.oracle_volume(int(has_atm), int(bool(v.fogEnable)), int(round(r.maxAtmosphereScatters)))
         for (m, kind, a, b, c, d) in scene.oracle_sdfs:
             L.oracle_sdf(m, kind, a, b, c, d)
         L.oracle_albedo_hook(scene.oracle_albedo_hook)
+        L.oracle_emission_hook(getattr(scene, "oracle_emission_hook", 0))
         diel = mats.dielectric
         model = DIELECTRIC_MODELS[type(diel).__name__]
         coeffs = f3([diel.iorVal, diel.abbe] if model == 0 else diel.coeffs)

@@ -76,10 +76,11 @@ struct snl_prog_s {
   snl_ctx ctx;
   hipModule_t mod = nullptr;
   hipFunction_t k_reset, k_rotate, k_camera, k_trace, k_bin, k_miss, k_surf, k_metal, k_diele, k_accum, k_normals,
-      k_pick, k_tonemap, k_persist, k_persist_reset;
+      k_pick, k_tonemap, k_persist, k_persist_reset, k_mega;
   int engine = SNL_ENGINE_PERSISTENT;
   int tiles = 2, persistBlocks = 0, shadeThresh = 32, persistChunk = 64, marchBurst = 4;
-  bool hasSurface = false, hasMetal = false, hasDiele = false;
+  bool hasSurface = false, hasMetal = false, hasDiele = false, hasAtmosphere = false;
+  int megaBlocks = 0;
   int maxBounces = 3, maxMarch = 256, spf = 1;
   SnlParams hostP;
   std::vector<char> hostU;
@@ -191,7 +192,7 @@ int compileProgram(snl_ctx ctx, const char* shader, const char* ior, const char*
   src << "#include \"snl_prelude.h\"\n#include \"snl_wavefront.h\"\n";
   src << (defines ? defines : "") << "\n";
   src << "namespace glsl {\n" << out.tr.code << "\n}  // namespace glsl\n";
-  src << "#line 1 \"snl_pt.h\"\n#include \"snl_pt.h\"\n#include \"snl_kernels.h\"\n#include \"snl_persist.h\"\n";
+  src << "#line 1 \"snl_pt.h\"\n#include \"snl_pt.h\"\n#include \"snl_kernels.h\"\n#include \"snl_persist.h\"\n#include \"snl_mega.h\"\n";
   std::string source = src.str();
   if (const char* dump = getenv("SNL_DUMP_SOURCE")) {  // for offline ISA inspection
     std::ofstream f(dump);
@@ -386,8 +387,6 @@ int snl_program_build(snl_ctx ctx, const char* shader, const char* ior, const ch
   Compiled c;
   int rc = compileProgram(ctx, shader, ior, defines, maxMarch, maxBounces, maxAtm, spf, flags, c, log, loglen);
   if (rc) return rc;
-  if (c.defines.count("HAS_ATMOSPHERE"))
-    return fail(ctx, SNL_E_UNSUPPORTED, "the homogeneous atmosphere (HAS_ATMOSPHERE) is not implemented yet");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   std::unique_ptr<snl_prog_s> p(new snl_prog_s);
   p->ctx = ctx;
@@ -397,11 +396,13 @@ int snl_program_build(snl_ctx ctx, const char* shader, const char* ior, const ch
       {&p->k_trace, "snl_trace"}, {&p->k_bin, "snl_bin"}, {&p->k_miss, "snl_shade_miss"},
       {&p->k_surf, "snl_shade_surface"}, {&p->k_metal, "snl_shade_metal"}, {&p->k_diele, "snl_shade_dielectric"},
       {&p->k_accum, "snl_accumulate"}, {&p->k_normals, "snl_normals"}, {&p->k_pick, "snl_pick"},
-      {&p->k_tonemap, "snl_tonemap"}, {&p->k_persist, "snl_persist"}, {&p->k_persist_reset, "snl_persist_reset"}};
+      {&p->k_tonemap, "snl_tonemap"}, {&p->k_persist, "snl_persist"}, {&p->k_persist_reset, "snl_persist_reset"},
+      {&p->k_mega, "snl_mega"}};
   for (auto& f : fns) HIPCHK(ctx, hipModuleGetFunction(f.f, p->mod, f.n));
   p->hasSurface = c.defines.count("HAS_SURFACE") > 0;
   p->hasMetal = c.defines.count("HAS_METAL") > 0;
   p->hasDiele = c.defines.count("HAS_DIELECTRIC") > 0;
+  p->hasAtmosphere = c.defines.count("HAS_ATMOSPHERE") > 0;
   p->maxBounces = maxBounces;
   p->maxMarch = maxMarch;
   p->spf = spf;
@@ -433,6 +434,9 @@ int snl_program_build(snl_ctx ctx, const char* shader, const char* ior, const ch
   int occ = 0;
   if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, p->k_persist, 256, 0) != hipSuccess || occ < 1) occ = 2;
   p->persistBlocks = ctx->numCU * occ;
+  occ = 0;
+  if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, p->k_mega, 256, 0) != hipSuccess || occ < 1) occ = 2;
+  p->megaBlocks = ctx->numCU * occ;
   HIPCHK(ctx, hipEventCreate(&p->ev0));
   HIPCHK(ctx, hipEventCreate(&p->ev1));
   HIPCHK(ctx, hipEventCreate(&p->et0));
@@ -814,7 +818,8 @@ static int renderWavefront(snl_prog p, snl_state st, int frames) {
 
 // Persistent engine: per frame and pixel tile one launch of snl_persist; the
 // tiles run on their own streams so one tile's tail overlaps another's work.
-static int renderPersistent(snl_prog p, snl_state st, int frames) {
+// The megakernel (snl_mega) launches the same way.
+static int renderPersistent(snl_prog p, snl_state st, int frames, bool mega) {
   snl_ctx ctx = p->ctx;
   bool unit0 = false;
   int rc = prepareUnit0(p, st, unit0);
@@ -838,7 +843,8 @@ static int renderPersistent(snl_prog p, snl_state st, int frames) {
   }
   HIPCHK(ctx, hipEventRecord(p->ev0, ctx->stream));
   HIPCHK(ctx, hipEventRecord(ctx->joinEvent, ctx->stream));
-  int blocks = std::max(1, p->persistBlocks / K);
+  int blocks = std::max(1, (mega ? p->megaBlocks : p->persistBlocks) / K);
+  hipFunction_t kern = mega ? p->k_mega : p->k_persist;
   for (int k = 0; k < K; ++k) {
     hipStream_t ts = ctx->tileStream[k];
     HIPCHK(ctx, hipStreamWaitEvent(ts, ctx->joinEvent, 0));
@@ -852,13 +858,13 @@ static int renderPersistent(snl_prog p, snl_state st, int frames) {
       for (int f = 0; f < frames; ++f) {
         if ((rc = copyUnit0(ctx, st, ts))) return rc;
         HIPCHK(ctx, hipModuleLaunchKernel(p->k_persist_reset, 1, 1, 1, 64, 1, 1, 0, ts, args, nullptr));
-        HIPCHK(ctx, hipModuleLaunchKernel(p->k_persist, blocks, 1, 1, 256, 1, 1, 0, ts, args, nullptr));
+        HIPCHK(ctx, hipModuleLaunchKernel(kern, blocks, 1, 1, 256, 1, 1, 0, ts, args, nullptr));
       }
     } else if (frames > 0) {
       // every pixel runs all its frames in one lane: one launch per tile
       a.frames = frames;
       HIPCHK(ctx, hipModuleLaunchKernel(p->k_persist_reset, 1, 1, 1, 64, 1, 1, 0, ts, args, nullptr));
-      HIPCHK(ctx, hipModuleLaunchKernel(p->k_persist, blocks, 1, 1, 256, 1, 1, 0, ts, args, nullptr));
+      HIPCHK(ctx, hipModuleLaunchKernel(kern, blocks, 1, 1, 256, 1, 1, 0, ts, args, nullptr));
     }
     HIPCHK(ctx, hipEventRecord(ctx->tileEvent[k], ts));
     HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->tileEvent[k], 0));
@@ -918,6 +924,12 @@ unsigned long long snl_state_bytes(int width, int height, int chains) {
          (unsigned long long)(SNL_MAX_TILES + 1) * C_COUNT * 4ull;
 }
 
+// batched jobs run the persistent kernel, or the megakernel where that one
+// lacks a feature (the sub-surface walk, the atmosphere) or it is asked for
+static bool useMega(snl_prog p) {
+  return p->hasAtmosphere || p->engine == SNL_ENGINE_MEGAKERNEL || p->hostP.subsurfaceMFP > 0.0f;
+}
+
 int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, const int* frames,
                      snl_stats* stats) {
   if (njobs < 0 || (njobs > 0 && (!progs || !states || !frames))) return fail(nullptr, SNL_E_ARG, "bad batch");
@@ -930,8 +942,6 @@ int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, 
     if (!progs[j] || !states[j] || frames[j] < 0) return fail(ctx, SNL_E_ARG, "bad job %d", j);
     if (progs[j]->ctx != ctx || states[j]->ctx != ctx) return fail(ctx, SNL_E_ARG, "job %d: other context", j);
     if (!seen.insert(states[j]).second) return fail(ctx, SNL_E_ARG, "job %d: state used twice", j);
-    if (progs[j]->hostP.subsurfaceMFP > 0.0f)
-      return fail(ctx, SNL_E_UNSUPPORTED, "job %d: sub-surface scattering renders only with snl_render", j);
   }
   HIPCHK(ctx, hipSetDevice(ctx->device));
   // parameters up, per-job counters zeroed, all on the main stream
@@ -988,16 +998,19 @@ int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, 
     a.cnt = st->tileCnt;
     a.itemBase = 0;
     a.itemEnd = (int)st->npix;
-    // one lane per pixel up to the program's full persistent grid
+    // one lane per pixel up to the program's full persistent grid; the
+    // megakernel takes sub-surface and atmosphere programs
+    bool mega = useMega(p);
+    hipFunction_t kern = mega ? p->k_mega : p->k_persist;
     long long want = (st->npix + 255) / 256;
-    int blocks = (int)std::max(1LL, std::min<long long>(p->persistBlocks, want));
+    int blocks = (int)std::max(1LL, std::min<long long>(mega ? p->megaBlocks : p->persistBlocks, want));
     void* args[] = {&a};
     int launches = unit0[j] ? frames[j] : (frames[j] > 0 ? 1 : 0);
     a.frames = unit0[j] ? 1 : frames[j];
     for (int f = 0; f < launches; ++f) {
       if (unit0[j] && (rc = copyUnit0(ctx, st, ts))) return rc;
       HIPCHK(ctx, hipModuleLaunchKernel(p->k_persist_reset, 1, 1, 1, 64, 1, 1, 0, ts, args, nullptr));
-      HIPCHK(ctx, hipModuleLaunchKernel(p->k_persist, blocks, 1, 1, 256, 1, 1, 0, ts, args, nullptr));
+      HIPCHK(ctx, hipModuleLaunchKernel(kern, blocks, 1, 1, 256, 1, 1, 0, ts, args, nullptr));
     }
   }
   for (int k = 0; k < NS; ++k) {
@@ -1028,15 +1041,19 @@ int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, 
 int snl_render(snl_prog p, snl_state st, int frames) {
   if (!p || !st || frames < 0) return fail(p ? p->ctx : nullptr, SNL_E_ARG, "bad argument");
   if (st->ctx != p->ctx) return fail(p->ctx, SNL_E_ARG, "program and state belong to different contexts");
-  // the persistent kernel does not run the sub-surface walk: such programs use the wavefront
+  // only the megakernel renders the atmosphere; the persistent kernel does not
+  // run the sub-surface walk, such programs use the wavefront unless the
+  // megakernel is asked for
+  if (p->hasAtmosphere || p->engine == SNL_ENGINE_MEGAKERNEL) return renderPersistent(p, st, frames, true);
   bool needWavefront = p->hostP.subsurfaceMFP > 0.0f;
-  if (p->engine == SNL_ENGINE_PERSISTENT && !needWavefront) return renderPersistent(p, st, frames);
+  if (p->engine == SNL_ENGINE_PERSISTENT && !needWavefront) return renderPersistent(p, st, frames, false);
   return renderWavefront(p, st, frames);
 }
 
 int snl_set_engine(snl_prog p, int engine, int tiles, int persist_blocks, int shade_thresh, int chunk) {
   if (!p) return fail(nullptr, SNL_E_ARG, "prog is NULL");
-  if (engine != SNL_ENGINE_WAVEFRONT && engine != SNL_ENGINE_PERSISTENT) return fail(p->ctx, SNL_E_ARG, "bad engine");
+  if (engine != SNL_ENGINE_WAVEFRONT && engine != SNL_ENGINE_PERSISTENT && engine != SNL_ENGINE_MEGAKERNEL)
+    return fail(p->ctx, SNL_E_ARG, "bad engine");
   p->engine = engine;
   if (tiles > 0) p->tiles = std::min(tiles, SNL_MAX_TILES);
   if (persist_blocks > 0) p->persistBlocks = persist_blocks;

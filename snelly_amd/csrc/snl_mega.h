@@ -1,0 +1,523 @@
+// snl_mega.h -- the path tracer as one loop per lane, shadow and probe rays
+// traced where the reference traces them.
+//
+// This engine renders programs with the homogeneous atmosphere
+// (HAS_ATMOSPHERE): there a vertex may trace up to five extra rays -- the
+// in-scattering estimate's light samples and its emission probe -- between
+// random draws, which the queue-based engines do not stage.  It follows
+// cameraPath (pathtracer-fragment-shader.glsl:1625-1926) with
+// directSurfaceLighting (1221-1295), the atmosphere routines (1297-1457) and
+// the sub-surface walk (1468-1593).  A wave takes 64 pixels at a time from the
+// launch's queue; each lane runs all of its pixel's frames.
+#pragma once
+
+namespace glsl {
+
+template <class SC>
+struct Mega {
+  PT<SC>& pt;
+  const SnlParams& P;
+  SNL_HD Mega(PT<SC>& t, const SnlParams& p) : pt(t), P(p) {}
+
+  // transmittanceOverSegment (1042-1050)
+  SNL_HD RT transmittanceOverSegment(const vec3& pW, const vec3& rayDir, float segmentLength, const vec3& rgb) {
+    vec3 pHit;
+    int hm;
+    if (pt.traceRay(pW, rayDir, pHit, hm, segmentLength)) return rt1(0.0f);
+    return pt.transmittanceOverFreeSegment(pW, rayDir, segmentLength, rgb);
+  }
+
+  // directSurfaceLighting (1221-1295)
+  SNL_HD RT directSurfaceLighting(const vec3& pW, const Basis& basis, const vec3& winputW, int material, float wl,
+                                  const vec3& rgb, vec4& rnd, float& skyPdf, float& sunPdf, float& sphPdf) {
+    vec3 winputL = worldToLocal(winputW, basis);
+    RT Ldirect = rt1(0.0f);
+    vec3 woutputL(0.0f), woutputW(0.0f);
+    if (P.skyPower > RADIANCE_EPSILON) {
+      RT Li = pt.sampleSkyAtSurface(basis, rgb, rnd, woutputL, woutputW, skyPdf);
+      if (averageComponent(Li) > RADIANCE_EPSILON) {
+        RT Tr = transmittanceOverSegment(pW, woutputW, P.maxLengthScale, rgb);
+        Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
+        if (averageComponent(Li) > RADIANCE_EPSILON) {
+          float bsdfPdf = max(PDF_EPSILON, pt.pdfBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true));
+          RT f = pt.evaluateBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true);
+          float misWeight = powerHeuristic(skyPdf, bsdfPdf);
+          Ldirect += f * Li / max(PDF_EPSILON, skyPdf) * abs(dot(woutputW, basis.nW)) * misWeight;
+        }
+      }
+    }
+    if (P.sunPower > RADIANCE_EPSILON) {
+      RT Li = pt.sampleSunAtSurface(basis, rgb, rnd, woutputL, woutputW, sunPdf);
+      if (averageComponent(Li) > RADIANCE_EPSILON) {
+        RT Tr = transmittanceOverSegment(pW, woutputW, P.maxLengthScale, rgb);
+        Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
+        if (averageComponent(Li) > RADIANCE_EPSILON) {
+          float bsdfPdf = max(PDF_EPSILON, pt.pdfBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true));
+          RT f = pt.evaluateBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true);
+          float misWeight = powerHeuristic(sunPdf, bsdfPdf);
+          Ldirect += f * Li / max(PDF_EPSILON, sunPdf) * abs(dot(woutputW, basis.nW)) * misWeight;
+        }
+      }
+    }
+#ifdef HAS_SPHERE_LIGHT
+    if ((P.sphereLightPower > RADIANCE_EPSILON) && (dot(pW - P.sphereLightPosition, basis.nW) <= P.sphereLightRadius)) {
+      vec3 pHit;
+      RT Li = pt.sampleSphereLight(rgb, rnd, pW, pHit, sphPdf);
+      if (averageComponent(Li) > RADIANCE_EPSILON) {
+        float hitDist = max(0.0f, length(pHit - pW) - 3.0f * P.minLengthScale);
+        woutputW = normalize(pHit - pW);
+        woutputL = worldToLocal(woutputW, basis);
+        RT Tr = transmittanceOverSegment(pW, woutputW, hitDist, rgb);
+        Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
+        if (averageComponent(Li) > RADIANCE_EPSILON) {
+          float bsdfPdf = max(PDF_EPSILON, pt.pdfBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true));
+          RT f = pt.evaluateBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true);
+          float misWeight = powerHeuristic(sphPdf, bsdfPdf);
+          Ldirect += f * Li / max(PDF_EPSILON, sphPdf) * abs(dot(woutputW, basis.nW)) * misWeight;
+        }
+      }
+    }
+#endif
+    return min(rt1(P.radianceClamp), Ldirect);
+  }
+
+  // directVolumeLighting (1299-1363)
+  SNL_HD RT directVolumeLighting(const vec3& pW, const vec3& rayDir, const vec3& rgb, vec4& rnd) {
+    RT Ldirect = rt1(0.0f);
+    if (P.skyPower > RADIANCE_EPSILON) {
+      vec3 woutputW(0.0f);
+      float skyPdf = 0.0f;
+      RT Li = pt.sampleSkyInVolume(rgb, rnd, woutputW, skyPdf);
+      if (averageComponent(Li) > RADIANCE_EPSILON) {
+        RT Tr = transmittanceOverSegment(pW, woutputW, P.maxLengthScale, rgb);
+        Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
+        if (averageComponent(Li) > RADIANCE_EPSILON) {
+          float PF = phaseFunction(dot(-rayDir, -woutputW), P.atmosphereAnisotropy);
+          Ldirect += PF * Li / max(PDF_EPSILON, skyPdf);
+        }
+      }
+    }
+    if (P.sunPower > RADIANCE_EPSILON) {
+      vec3 woutputW(0.0f);
+      float sunPdf = 0.0f;
+      RT Li = pt.sampleSunInVolume(rgb, rnd, woutputW, sunPdf);
+      if (averageComponent(Li) > RADIANCE_EPSILON) {
+        RT Tr = transmittanceOverSegment(pW, woutputW, P.maxLengthScale, rgb);
+        Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
+        if (averageComponent(Li) > RADIANCE_EPSILON) {
+          float PF = phaseFunction(dot(-rayDir, -woutputW), P.atmosphereAnisotropy);
+          Ldirect += PF * Li / max(PDF_EPSILON, sunPdf);
+        }
+      }
+    }
+#ifdef HAS_SPHERE_LIGHT
+    if (P.sphereLightPower > RADIANCE_EPSILON) {
+      float sphPdf = 0.0f;
+      vec3 pHit;
+      RT Li = pt.sampleSphereLight(rgb, rnd, pW, pHit, sphPdf);
+      if (averageComponent(Li) > RADIANCE_EPSILON) {
+        float hitDist = max(0.0f, length(pHit - pW) - 3.0f * P.minLengthScale);
+        vec3 woutputW = normalize(pHit - pW);
+        RT Tr = transmittanceOverSegment(pW, woutputW, hitDist, rgb);
+        Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
+        if (averageComponent(Li) > RADIANCE_EPSILON) {
+          float PF = phaseFunction(dot(-rayDir, -woutputW), P.atmosphereAnisotropy);
+          Ldirect += PF * Li / max(PDF_EPSILON, sphPdf);
+        }
+      }
+    }
+#endif
+    return min(rt1(P.radianceClamp), Ldirect);
+  }
+
+  // atmosphericInscatteringRadiance (1365-1417)
+  SNL_HD RT atmosphericInscatteringRadiance(const vec3& pW, const vec3& rayDir, float segmentLength, const vec3& rgb,
+                                            vec4& rnd) {
+    RT Ls = rt1(0.0f);
+    if constexpr (SC::kHasFog) {
+      if (P.fogEnable) {
+        RT TrFog = pt.transmittanceOverFreeSegment(pW, rayDir, segmentLength, rgb);
+        Ls = (rt1(1.0f) - TrFog) * rgbToAlbedo(P.fogEmission, rgb);
+      }
+    }
+    RT extinction = pt.VOLUME_EXTINCTION_EVAL(rgb);
+    RT scattering = pt.VOLUME_SCATTERING_EVAL(rgb);
+    float extinction_norm = averageComponent(extinction);
+    float scattering_norm = averageComponent(scattering);
+    if (extinction_norm < RADIANCE_EPSILON || scattering_norm < RADIANCE_EPSILON) return Ls;
+    float t0 = 0.0f, t1 = 0.0f;
+    if (!pt.atmosphereSegment(pW, rayDir, segmentLength, t0, t1)) return Ls;
+    float T01 = exp(-(t1 - t0) * extinction_norm);
+    float t_scatter = t0 - log(1.0f - snl_rand(rnd) * (1.0f - T01)) / extinction_norm;
+    vec3 pW_scatter = pW + t_scatter * rayDir;
+    RT opticalDepth_scatter = (t_scatter - t0) * extinction;
+    RT Tr_pW = exp(-opticalDepth_scatter);
+    float invDistancePdf = (1.0f - T01) * exp(averageComponent(opticalDepth_scatter)) / extinction_norm;
+    RT Li = directVolumeLighting(pW_scatter, rayDir, rgb, rnd);
+    Ls += Tr_pW * scattering * Li * invDistancePdf;
+    if constexpr (SC::kHasVolumeEmission) {
+      vec3 woW = samplePhaseFunction(rayDir, P.atmosphereAnisotropy, rnd);
+      vec3 pW_hit;
+      int hm;
+      if (pt.traceRay(pW_scatter, woW, pW_hit, hm, P.maxLengthScale))
+        Ls += Tr_pW * scattering * pt.VOLUME_EMISSION_EVAL(pW_hit, rgb) * invDistancePdf;
+    }
+    return Ls;
+  }
+
+  // atmosphericScatterSample (1419-1442)
+  SNL_HD RT atmosphericScatterSample(const vec3& pW, const vec3& rayDir, float segmentLength, const vec3& rgb,
+                                     vec4& rnd, vec3& pW_scatter, vec3& woutputW) {
+    RT extinction = pt.VOLUME_EXTINCTION_EVAL(rgb);
+    RT scattering = pt.VOLUME_SCATTERING_EVAL(rgb);
+    float extinction_norm = averageComponent(extinction);
+    float t0 = 0.0f, t1 = 0.0f;
+    if (!pt.atmosphereSegment(pW, rayDir, segmentLength, t0, t1)) return rt1(0.0f);
+    float T01 = exp(-(t1 - t0) * extinction_norm);
+    float t_scatter = t0 - log(1.0f - snl_rand(rnd) * (1.0f - T01)) / extinction_norm;
+    pW_scatter = pW + t_scatter * rayDir;
+    RT opticalDepth_scatter = (t_scatter - t0) * extinction;
+    RT Tr_pW = exp(-opticalDepth_scatter);
+    float invDistancePdf = (1.0f - T01) * exp(averageComponent(opticalDepth_scatter)) / extinction_norm;
+    woutputW = samplePhaseFunction(rayDir, P.atmosphereAnisotropy, rnd);
+    return scattering * Tr_pW * invDistancePdf;
+  }
+
+  // randomwalk_SSS (1468-1518)
+  SNL_HD bool randomwalk_SSS(const vec3& pW, const Basis& basis, vec4& rnd, float MFP, const RT& subsurfaceAlbedo,
+                             RT& walk_throughput, vec3& pExit) {
+    vec3 pWalk = pW - 3.0f * P.minLengthScale * basis.nW;
+    float pdfDir;
+    vec3 dirwalkL = sampleHemisphereCosineWeighted(rnd, pdfDir);
+    vec3 dirwalkW = -localToWorld(dirwalkL, basis);
+    walk_throughput = rt1(1.0f);
+    for (int n = 0; n < 2048; ++n) {
+      if (n > P.maxSSSSteps) break;
+      float walk_step = -log(snl_rand(rnd)) * MFP * P.lengthScale;
+      vec3 pHit;
+      int hm;
+      if (pt.traceRay(pWalk, dirwalkW, pHit, hm, walk_step)) {
+        pExit = pHit;
+        return true;
+      }
+      if (n > 4) {
+        float continuation_prob = clamp(10.0f * maxComponent(walk_throughput), 0.0f, 1.0f);
+        if (snl_rand(rnd) < 1.0f - continuation_prob) break;
+        walk_throughput /= continuation_prob;
+      }
+      dirwalkW = samplePhaseFunction(dirwalkW, P.subsurfaceAnisotropy, rnd);
+      pWalk += walk_step * dirwalkW;
+      walk_throughput *= subsurfaceAlbedo;
+    }
+    return false;
+  }
+
+  // SSS_exit_radiance (1521-1593)
+  SNL_HD RT SSS_exit_radiance(const vec3& pW, const Basis& basis, const vec3& rgb, vec4& rnd,
+                              const RT& diffuseAlbedoExit, float& skyPdf, float& sunPdf, float& sphPdf) {
+    vec3 dPw = 3.0f * P.minLengthScale * basis.nW;
+    RT Ldirect = rt1(0.0f);
+    vec3 woutputL(0.0f), woutputW(0.0f);
+    RT f = mix(rt1(1.0f), diffuseAlbedoExit, P.subsurfaceDiffuseWeight) / float(M_PI);
+    if (P.skyPower > RADIANCE_EPSILON) {
+      RT Li = pt.sampleSkyAtSurface(basis, rgb, rnd, woutputL, woutputW, skyPdf);
+      if (averageComponent(Li) > RADIANCE_EPSILON) {
+        RT Tr = transmittanceOverSegment(pW + dPw, woutputW, P.maxLengthScale, rgb);
+        Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
+        if (averageComponent(Li) > RADIANCE_EPSILON) {
+          float misWeight = powerHeuristic(skyPdf, pdfHemisphereCosineWeighted(woutputL));
+          Ldirect += f * Li / max(PDF_EPSILON, skyPdf) * abs(dot(woutputW, basis.nW)) * misWeight;
+        }
+      }
+    }
+    if (P.sunPower > RADIANCE_EPSILON) {
+      RT Li = pt.sampleSunAtSurface(basis, rgb, rnd, woutputL, woutputW, sunPdf);
+      if (averageComponent(Li) > RADIANCE_EPSILON) {
+        RT Tr = transmittanceOverSegment(pW + dPw, woutputW, P.maxLengthScale, rgb);
+        Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
+        if (averageComponent(Li) > RADIANCE_EPSILON) {
+          float misWeight = powerHeuristic(sunPdf, pdfHemisphereCosineWeighted(woutputL));
+          Ldirect += f * Li / max(PDF_EPSILON, sunPdf) * abs(dot(woutputW, basis.nW)) * misWeight;
+        }
+      }
+    }
+#ifdef HAS_SPHERE_LIGHT
+    if ((P.sphereLightPower > RADIANCE_EPSILON) && (dot(pW - P.sphereLightPosition, basis.nW) <= P.sphereLightRadius)) {
+      vec3 pHit;
+      RT Li = pt.sampleSphereLight(rgb, rnd, pW, pHit, sphPdf);
+      if (averageComponent(Li) > RADIANCE_EPSILON) {
+        float hitDist = max(0.0f, length(pHit - pW) - 3.0f * P.minLengthScale);
+        woutputW = normalize(pHit - pW);
+        woutputL = worldToLocal(woutputW, basis);
+        RT Tr = transmittanceOverSegment(pW + dPw, woutputW, hitDist, rgb);
+        Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
+        if (averageComponent(Li) > RADIANCE_EPSILON) {
+          float misWeight = powerHeuristic(sphPdf, pdfHemisphereCosineWeighted(woutputL));
+          Ldirect += f * Li / max(PDF_EPSILON, sphPdf) * abs(dot(woutputW, basis.nW)) * misWeight;
+        }
+      }
+    }
+#endif
+    return min(rt1(P.radianceClamp), Ldirect);
+  }
+
+  // cameraPath (1625-1926)
+  SNL_HD RT cameraPath(const vec3& primaryStart, const vec3& primaryDir, float wl, const vec3& rgb, vec4& rnd,
+                       int maxBounces, long long& normals) {
+    RT L = rt1(0.0f);
+    float misWeightSky = 1.0f, misWeightSun = 1.0f, misWeightSph = 1.0f;
+    vec3 pW = primaryStart, rayDir = primaryDir;
+    bool inDielectric = false;
+    if constexpr (SC::kHasDielectric) inDielectric = pt.S.SDF_DIELECTRIC(primaryStart) < 0.0f;
+    RT throughput = rt1(1.0f);
+    int atmosphere_scatters = 0;
+    (void)misWeightSph;
+    (void)atmosphere_scatters;
+    for (int vertex = 0; vertex <= maxBounces; ++vertex) {
+      if (maxComponent(throughput) < THROUGHPUT_EPSILON) break;
+      vec3 pW_next;
+      int hitMaterial;
+      bool hit = pt.traceRay(pW, rayDir, pW_next, hitMaterial, P.maxLengthScale);
+      float rayLength = P.maxLengthScale;
+      if (hit) {
+        rayLength = length(pW_next - pW);
+      } else {
+        float lightSegmentLength = P.maxLengthScale;
+        RT Tr = rt1(1.0f);
+        bool litSph = false;
+#ifdef HAS_SPHERE_LIGHT
+        {
+          vec3 pLightHit;
+          RT LiSph = (P.sphereLightPower < RADIANCE_EPSILON) ? rt1(0.0f) : pt.sphereLightRadiance(pW, rayDir, rgb, pLightHit);
+          if (averageComponent(LiSph) > RADIANCE_EPSILON) {
+            litSph = true;
+            lightSegmentLength = length(pLightHit - pW);
+            Tr = pt.transmittanceOverFreeSegment(pW, rayDir, lightSegmentLength, rgb);
+            float scatter_prob = pt.atmosphericScatteringProbability(Tr, rgb);
+            L += throughput * Tr * misWeightSph * LiSph / (1.0f - scatter_prob);
+          }
+        }
+#endif
+        if (!litSph) {
+          lightSegmentLength = P.maxLengthScale;
+          Tr = pt.transmittanceOverFreeSegment(pW, rayDir, lightSegmentLength, rgb);
+          float scatter_prob = pt.atmosphericScatteringProbability(Tr, rgb);
+          if (averageComponent(Tr) > RADIANCE_EPSILON) {
+            if (!(vertex == 0 && !P.envMapVisible) && misWeightSky > 0.0f)
+              L += throughput * Tr * misWeightSky * pt.environmentRadiance(rayDir, rgb) / (1.0f - scatter_prob);
+            if (!(vertex == 0 && !P.sunVisibleDirectly) && misWeightSun > 0.0f)
+              L += throughput * Tr * misWeightSun * pt.sunRadiance(rayDir, rgb) / (1.0f - scatter_prob);
+          }
+        }
+        if constexpr (SC::kHasAtmosphere) {
+          if (vertex < 2) L += throughput * atmosphericInscatteringRadiance(pW, rayDir, lightSegmentLength, rgb, rnd);
+          if (atmosphere_scatters + 1 < SNL_MAX_ATMOSPHERE_SCATTERS) {
+            float scatter_prob = pt.atmosphericScatteringProbability(Tr, rgb);
+            if (scatter_prob > snl_rand(rnd)) {
+              vec3 pW_scatter(0.0f), woutputW(0.0f);
+              throughput *= atmosphericScatterSample(pW, rayDir, lightSegmentLength, rgb, rnd, pW_scatter, woutputW) /
+                            max(DENOM_TOLERANCE, scatter_prob);
+              pW = pW_scatter;
+              rayDir = woutputW;
+              atmosphere_scatters++;
+              continue;
+            }
+          }
+        }
+        break;
+      }
+      if (hitMaterial < 0) break;  // a max-step "hit" has no material: the path ends
+
+      RT Tr = rt1(1.0f);
+      bool hitSphereLight = false;
+#ifdef HAS_SPHERE_LIGHT
+      {
+        vec3 pLightHit;
+        RT LiSph = (P.sphereLightPower < RADIANCE_EPSILON) ? rt1(0.0f) : pt.sphereLightRadiance(pW, rayDir, rgb, pLightHit);
+        if (averageComponent(LiSph) > RADIANCE_EPSILON) {
+          float lightSegmentLength = length(pLightHit - pW);
+          if (lightSegmentLength < rayLength) {
+            hitSphereLight = true;
+            Tr = pt.transmittanceOverFreeSegment(pW, rayDir, lightSegmentLength, rgb);
+            L += throughput * Tr * misWeightSph * LiSph;
+            if constexpr (SC::kHasAtmosphere) {
+              if (vertex < 2) L += throughput * atmosphericInscatteringRadiance(pW, rayDir, lightSegmentLength, rgb, rnd);
+            }
+          }
+        }
+      }
+#endif
+      if constexpr (SC::kHasDielectric) {
+        if (inDielectric) throughput *= exp(-rayLength * pt.DIELECTRIC_ABSORPTION_EVAL(pW, rgb));
+      }
+      if constexpr (SC::kHasAtmosphere) {
+        if (vertex < 2) L += throughput * atmosphericInscatteringRadiance(pW, rayDir, rayLength, rgb, rnd);
+        if (!hitSphereLight) Tr = pt.transmittanceOverFreeSegment(pW, rayDir, rayLength, rgb);
+        if (atmosphere_scatters + 1 < SNL_MAX_ATMOSPHERE_SCATTERS) {
+          float scatter_prob = pt.atmosphericScatteringProbability(Tr, rgb);
+          if (scatter_prob > snl_rand(rnd)) {
+            vec3 pW_scatter(0.0f), woutputW(0.0f);
+            throughput *= atmosphericScatterSample(pW, rayDir, rayLength, rgb, rnd, pW_scatter, woutputW) /
+                          max(DENOM_TOLERANCE, scatter_prob);
+            pW = pW_scatter;
+            rayDir = woutputW;
+            atmosphere_scatters++;
+            continue;
+          } else {
+            throughput /= max(DENOM_TOLERANCE, 1.0f - scatter_prob);
+          }
+        }
+        if (hitSphereLight) break;
+        throughput *= pt.transmittanceOverFreeSegment(pW, rayDir, rayLength, rgb);
+      }
+      (void)Tr;
+      (void)hitSphereLight;
+
+      // surface interaction
+      pW = pW_next;
+      vec3 nW = pt.normal(pW, hitMaterial, true);
+      ++normals;
+      vec3 ngW = nW;
+      Basis basis = makeBasis(nW);
+      if constexpr (SC::kHasNormalMap) {
+        nW = pt.perturbNormal(pW, basis, hitMaterial);
+        if ((dot(nW, rayDir) > 0.0f) && (hitMaterial != MAT_DIELE)) nW = 2.0f * ngW * dot(ngW, nW) - nW;
+        basis = makeBasis(nW);
+      }
+      bool do_subsurface_walk = false;
+      float prob_sss = 0.0f;
+      RT subsurfaceAlbedo = rt1(0.0f);
+      if constexpr (SC::kHasSurface) {
+        if (P.subsurfaceMFP > 0.0f) {
+          subsurfaceAlbedo = pt.SUBSURFACE_ALBEDO_EVAL(pW, basis.nW, rgb);
+          float diffuse_weight = (1.0f - P.subsurface) * averageComponent(pt.SURFACE_DIFFUSE_REFL_EVAL(pW, basis.nW, -rayDir, rgb));
+          float spec_weight = averageComponent(pt.SURFACE_SPEC_REFL_EVAL(pW, basis.nW, -rayDir, rgb));
+          float subsurface_weight = P.subsurface * averageComponent(subsurfaceAlbedo) * 3.0f;
+          float total_weight = diffuse_weight + spec_weight + subsurface_weight;
+          prob_sss = subsurface_weight / (total_weight + DENOM_TOLERANCE);
+          prob_sss = clamp(prob_sss, 0.0f, 1.0f - PDF_EPSILON);
+          do_subsurface_walk = (snl_rand(rnd) < prob_sss);
+        }
+      }
+      if (!do_subsurface_walk) {
+        vec3 winputW = -rayDir;
+        vec3 winputL = worldToLocal(winputW, basis);
+        vec3 woutputL(0.0f);  // WebGL zero-initialises these locals
+        float bsdfPdf = 0.0f;
+        RT f = pt.sampleBsdf(pW, basis, winputL, hitMaterial, wl, rgb, true, woutputL, bsdfPdf, rnd);
+        vec3 woutputW = localToWorld(woutputL, basis);
+        if constexpr (SC::kHasDielectric) {
+          if (hitMaterial == MAT_DIELE && dot(winputW, ngW) * dot(woutputW, ngW) < 0.0f) inDielectric = !inDielectric;
+        }
+        if constexpr (SC::kHasVolumeEmission) L += throughput * pt.VOLUME_EMISSION_EVAL(pW, rgb);
+        rayDir = woutputW;
+        pW += ngW * sign(dot(rayDir, ngW)) * 3.0f * P.minLengthScale;
+        float skyPdf = 0.0f, sunPdf = 0.0f, sphPdf = 0.0f;
+        if (!inDielectric)
+          L += throughput * directSurfaceLighting(pW, basis, winputW, hitMaterial, wl, rgb, rnd, skyPdf, sunPdf, sphPdf);
+        RT fOverPdf = min(rt1(P.radianceClamp), f / max(PDF_EPSILON, bsdfPdf));
+        RT surface_throughput = fOverPdf * abs(dot(woutputW, nW)) / max(PDF_EPSILON, 1.0f - prob_sss);
+        throughput *= surface_throughput;
+        misWeightSky = powerHeuristic(bsdfPdf, skyPdf);
+        misWeightSun = powerHeuristic(bsdfPdf, sunPdf);
+        misWeightSph = powerHeuristic(bsdfPdf, sphPdf);
+      } else {
+        if constexpr (SC::kHasSurface) {
+          RT diffuseAlbedoEntry = pt.SURFACE_DIFFUSE_REFL_EVAL(pW, basis.nW, -rayDir, rgb);
+          (void)diffuseAlbedoEntry;  // weighs the entry lobe, which the walk leaves at 1 (1486-1489)
+          RT walk_throughput = rt1(1.0f);
+          vec3 pExit;
+          basis.nW = ngW;
+          if (!randomwalk_SSS(pW, basis, rnd, P.subsurfaceMFP, subsurfaceAlbedo, walk_throughput, pExit)) break;
+          pW = pExit;
+          nW = pt.normal(pExit, MAT_SURFA, true);
+          ++normals;
+          Basis basis_exit = makeBasis(nW);
+          float bsdfPdf;
+          vec3 dirExitL = sampleHemisphereCosineWeighted(rnd, bsdfPdf);
+          vec3 woutputW = localToWorld(dirExitL, basis_exit);
+          rayDir = woutputW;
+          throughput *= walk_throughput;
+          float skyPdf = 0.0f, sunPdf = 0.0f, sphPdf = 0.0f;
+          RT diffuseAlbedoExit = pt.SURFACE_DIFFUSE_REFL_EVAL(pW, nW, -rayDir, rgb);
+          if (!inDielectric)
+            L += throughput * SSS_exit_radiance(pExit, basis_exit, rgb, rnd, diffuseAlbedoExit, skyPdf, sunPdf, sphPdf);
+          misWeightSky = powerHeuristic(bsdfPdf, skyPdf);
+          misWeightSun = powerHeuristic(bsdfPdf, sunPdf);
+          misWeightSph = powerHeuristic(bsdfPdf, sphPdf);
+          RT f = mix(rt1(1.0f), diffuseAlbedoExit, P.subsurfaceDiffuseWeight) / float(M_PI);
+          RT fOverPdf = min(rt1(P.radianceClamp), f / max(PDF_EPSILON, bsdfPdf));
+          RT surface_exit_throughput = fOverPdf * abs(dot(woutputW, nW));
+          throughput *= surface_exit_throughput / max(PDF_EPSILON, prob_sss);
+          pW += nW * sign(dot(rayDir, nW)) * 3.0f * P.minLengthScale;
+        }
+      }
+      if (maxComponent(throughput) < THROUGHPUT_EPSILON) break;
+    }
+    return L;
+  }
+};
+
+template <class SC>
+__device__ inline void mega_body(const SnlKArgs& a) {
+  SNL_SCENE_SETUP
+  Mega<SC> mg(pt, P);
+  const int lane = snl_lane();
+  const int nItems = a.itemEnd - a.itemBase;
+  long long normals = 0;
+  while (true) {
+    int base = 0;
+    if (lane == 0) base = atomicAdd(&a.cnt[C_FETCH], 64);
+    base = __shfl(base, 0);
+    if (base >= nItems) break;
+    int item = base + lane;
+    if (item >= nItems) continue;
+    int pix = a.itemBase + item;
+    vec4 rnd = ld4(a.rngstate, pix);
+    vec4 accum = ld4(a.radiance, pix);
+    vec2 pixel(float(pix % a.W) + 0.5f, float((pix % a.imagePix) / a.W) + 0.5f);
+    for (int frame = 0; frame < a.frames; ++frame) {
+#ifdef INTERACTIVE_MODE
+      if (snl_rand(rnd) < P.skipProbability) continue;
+#endif
+      // pathtrace (1934-1989)
+#ifdef DISPERSION_ENABLED
+      float xi = 0.1f + 0.8f * (0.5f + floor(float(P.wavelengthSamples) * snl_rand(rnd))) / float(P.wavelengthSamples);
+      float w = texture(P.ICDF, vec2(xi, 0.5f)).r;
+      float wl = 390.0f + (750.0f - 390.0f) * w;
+      vec3 xyz = vec3(texture(P.WavelengthToXYZ, vec2(w, 0.5f)));
+      vec3 rgb = xyzToRgb(xyz);
+#else
+      float wl = 390.0f + (750.0f - 390.0f) * 0.5f;
+      vec3 rgb(0.0f);
+#endif
+      RT L = rt1(0.0f);
+      for (int n = 0; n < a.spf; ++n) {
+        float jx = 0.5f * P.filterRadius * sample_jitter(snl_rand(rnd));
+        float jy = 0.5f * P.filterRadius * sample_jitter(snl_rand(rnd));
+        vec2 pixelj = pixel + vec2(jx, jy);
+        vec3 primaryStart, primaryDir;
+        if constexpr (SC::kHasCustomCamera) S.CONSTRUCT_PRIMARY_RAY(pixelj, rnd, primaryStart, primaryDir);
+        else pt.constructPrimaryRay(pixelj, rnd, primaryStart, primaryDir, true);
+        L += mg.cameraPath(primaryStart, primaryDir, wl, rgb, rnd, a.maxBounces, normals);
+      }
+      L /= float(a.spf);
+#ifdef DISPERSION_ENABLED
+      vec3 colorXYZ = xyz * L;
+#else
+      vec3 colorXYZ = rgbToXyz(L);
+#endif
+      float oldN = accum.w, newN = oldN + 1.0f;
+      accum = vec4((oldN * vec3(accum) + colorXYZ) / newN, newN);
+    }
+    st4(a.radiance, pix, accum);
+    st4(a.rngstate, pix, rnd);
+  }
+  unsigned long long ntot = normals;
+  for (int off = 32; off > 0; off >>= 1) ntot += __shfl_xor(ntot, off);
+  if (lane == 0 && ntot) atomicAdd(reinterpret_cast<unsigned long long*>(&a.cnt[C_NORMALS]), ntot);
+}
+
+extern "C" __global__ void __launch_bounds__(256) snl_mega(SnlKArgs a) { mega_body<SnlScene>(a); }
+
+}  // namespace glsl

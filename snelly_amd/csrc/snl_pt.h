@@ -103,7 +103,8 @@ SNL_HD vec3 sampleSphereUniformly(vec4& rnd) {
   float z = 1.0f - 2.0f * snl_rand(rnd);
   float r = sqrt(max(0.0f, 1.0f - z * z));
   float phi = 2.0f * float(M_PI) * snl_rand(rnd);
-  return vec3(r * cos(phi), r * sin(phi), z);
+  (void)r;  // the reference leaves x and y unscaled by r: the direction is not unit length
+  return vec3(cos(phi), sin(phi), z);
 }
 SNL_HD vec3 sampleHemisphereUniformly(vec4& rnd) {
   float xi = snl_rand(rnd);

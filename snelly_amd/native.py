@@ -25,7 +25,7 @@ EXPORTS = [
     "snl_get_stats", "snl_set_tuning", "snl_set_profiling", "snl_set_engine", "snl_render_batch",
     "snl_state_create_chains", "snl_state_error", "snl_ctx_mem_info", "snl_state_bytes", "snl_set_march_burst",
 ]
-ENGINE_WAVEFRONT, ENGINE_PERSISTENT = 0, 1
+ENGINE_WAVEFRONT, ENGINE_PERSISTENT, ENGINE_MEGAKERNEL = 0, 1, 2
 
 
 class SnellyError(RuntimeError):

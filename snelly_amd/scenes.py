@@ -105,6 +105,12 @@ MENGER_SDF_FLOPS = {n: 10 + 24 * n for n in range(1, 8)}
 SPHERE_PLANE_SDF_FLOPS = 8
 
 
+# a volume emission field (the VOLUME_EMISSION hook, pathtracer-fragment-shader.glsl:863-869)
+VOLUME_EMISSION_GLSL = """
+vec3 VOLUME_EMISSION(vec3 emission, vec3 X) { return X.y > 0.0 ? emission : 0.25 * emission; }
+"""
+
+
 class Scene:
     def __init__(self, name, glsl, width, height, spp):
         self.name = name
@@ -119,6 +125,7 @@ class Scene:
         # (material id 0 dielectric / 1 metal / 2 surface, kind, a, b, c, d)
         self.oracle_sdfs = []
         self.oracle_albedo_hook = 0
+        self.oracle_emission_hook = 0  # 1: the VOLUME_EMISSION of VOLUME_EMISSION_GLSL
         # algorithmic FLOPs of one march step (SDF evaluations of the enabled
         # materials + 8 for p = o + t*d and t += sign*sdf), from the counting
         # build of the oracle (pinned by tests/test_flops.py); None = varies

@@ -5,11 +5,14 @@ reference_cases.py, from integer RNG seeds.
 
 Development-container tooling: it reads the reference's shader sources from
 /root/reference and drives reference_gles/_build/gles_render, which dlopens the
-SwiftShader GLES libraries installed with the kaleido package.  Two steps
+SwiftShader GLES libraries installed with the kaleido package.  Three steps
 make the run follow the reference's arithmetic rather than the driver's
-liberties: uninitialised locals are zero-initialised, as under WebGL, and the
-transcendental builtins are replaced by accurate ones
-(reference_gles/precise_math.glsl, checked before any fixture is written).  The fixtures it writes are
+liberties: uninitialised locals are zero-initialised, as under WebGL; the
+transcendental builtins are replaced by accurate ones, and min/max/clamp by
+the GLSL definitions (reference_gles/precise_math.glsl); for atmosphere
+programs cameraPath's two `continue` statements are restated as a skip flag
+(without_continue).  Each step is checked under the GLES implementation before
+any fixture is written.  The fixtures it writes are
 data (initial state is implied by the seed; outputs are the
 accumulated radiance and the final RNG state).
 
@@ -77,6 +80,40 @@ def zero_init_locals(src):
     return "\n".join(out)
 
 
+def without_continue(src):
+    """Restate cameraPath's two `continue` statements (an atmosphere scatter,
+    pathtracer-fragment-shader.glsl:1706 and 1767) as a flag that skips the
+    rest of the loop body.  Each `continue` is the last statement of blocks
+    that end right after it, so `skip = true` there, `if (!skip) break` for the
+    miss branch's closing `break` and `if (!skip) { ... }` around the code that
+    follows are the same control flow.  SwiftShader 4.1 mis-executes a
+    `continue` in a loop whose body has a `break` after it (the loop ends
+    early, or runs on with wrong state); check_loop_control shows it on a
+    minimal loop and checks the flag form before any fixture is written."""
+    import re
+    head = re.search(r"for \(int vertex=0; vertex<=\d+; \+\+vertex\)\n    \{\n", src)
+    assert head, "cameraPath loop not found"
+    body = src[head.end():]
+    pieces = [
+        ("                    continue;\n", "                    snl_skip = true;\n"),
+        ("            // Ray escapes to infinity\n            break;\n",
+         "            // Ray escapes to infinity\n            if (!snl_skip) break;\n"),
+        ("        RadianceType Tr = RadianceType(1.0);", "        if (!snl_skip) {\n        RadianceType Tr = RadianceType(1.0);"),
+        ("                continue;\n", "                snl_skip = true;\n"),
+        ("        if (hitSphereLight)\n            break;", "        if (!snl_skip) {\n        if (hitSphereLight)\n            break;"),
+        ("        if (maxComponent(throughput) < THROUGHPUT_EPSILON)\n            break;\n    }\n    return L;",
+         "        if (maxComponent(throughput) < THROUGHPUT_EPSILON)\n            break;\n        }}\n    }\n    return L;"),
+    ]
+    out, rest = [], body
+    for old, new in pieces:
+        at = rest.find(old)
+        assert at >= 0, f"anchor not found: {old!r}"
+        out.append(rest[:at] + new)
+        rest = rest[at + len(old):]
+    assert "continue;" not in "".join(out)
+    return src[:head.end()] + "        bool snl_skip = false;\n" + "".join(out) + rest
+
+
 def program_source(scene, program="pathtracer"):
     """Renderer.compileShaders' replacements (js/renderer.js:286-316), applied in
     the order the replacements object lists them, to the vertex and fragment
@@ -92,6 +129,8 @@ def program_source(scene, program="pathtracer"):
     v, f = shader_source(program + "-vertex-shader"), shader_source(program + "-fragment-shader")
     for k, val in rep:
         v, f = v.replace(k, val), f.replace(k, val)
+    if program == "pathtracer" and "#define HAS_ATMOSPHERE" in defines:
+        f = without_continue(f)
     return v, with_precise_math(zero_init_locals(f))
 
 
@@ -158,6 +197,132 @@ void main(){
     if bad:
         raise RuntimeError(f"precise_math.glsl inaccurate under this GLES implementation: {bad}")
     return errs
+
+
+LOOP_CONTINUE = """
+    for (int v = 0; v <= 3; ++v) {
+      count += 1;
+      if (!(r.x > 0.5)) {
+        acc += 0.5 * r.z;
+        if (sc + 1 < 3) { if (r.y > 0.0) { sc++; acc += 1.0; continue; } }
+        break;
+      }
+      acc += 10.0;
+      if (sc + 1 < 3) { if (r.w > 0.0) { sc++; continue; } else acc += 100.0; }
+      if (r.z > 0.7) break;
+      acc += 1000.0;
+    }"""
+LOOP_FLAG = """
+    for (int v = 0; v <= 3; ++v) {
+      bool skip = false;
+      count += 1;
+      if (!(r.x > 0.5)) {
+        acc += 0.5 * r.z;
+        if (sc + 1 < 3) { if (r.y > 0.0) { sc++; acc += 1.0; skip = true; } }
+        if (!skip) break;
+      }
+      if (!skip) {
+      acc += 10.0;
+      if (sc + 1 < 3) { if (r.w > 0.0) { sc++; skip = true; } else acc += 100.0; }
+      if (!skip) {
+      if (r.z > 0.7) break;
+      acc += 1000.0;
+      }}
+    }"""
+
+
+def loop_expected(r):
+    """the GLSL semantics of LOOP_CONTINUE, in float32"""
+    count = sc = 0
+    acc = np.float32(0.0)
+    for _ in range(4):
+        count += 1
+        if not r[0] > 0.5:
+            acc = np.float32(acc + np.float32(0.5) * r[2])
+            if sc + 1 < 3 and r[1] > 0.0:
+                sc += 1
+                acc = np.float32(acc + 1)
+                continue
+            break
+        acc = np.float32(acc + 10)
+        if sc + 1 < 3:
+            if r[3] > 0.0:
+                sc += 1
+                continue
+            acc = np.float32(acc + 100)
+        if r[2] > 0.7:
+            break
+        acc = np.float32(acc + 1000)
+    return count, sc, acc
+
+
+def check_loop_control(td):
+    """run LOOP_CONTINUE and its flag form LOOP_FLAG over 256 random inputs;
+    the flag form must give the GLSL result everywhere (it is what
+    without_continue produces).  Returns how many pixels the `continue` form
+    got wrong, for the record."""
+    W = H = 16
+    n = W * H
+    rs = np.random.RandomState(1)
+    x = np.zeros((n, 4), np.float32)
+    x[:, 0] = rs.uniform(0, 1, n)
+    x[:, 1] = np.where(rs.uniform(0, 1, n) > 0.2, 1.0, -1.0)
+    x[:, 2] = rs.uniform(0, 1, n)
+    x[:, 3] = np.where(rs.uniform(0, 1, n) > 0.5, 1.0, -1.0)
+    want = np.array([loop_expected(r) for r in x], np.float64)
+    wrong = {}
+    for tag, loop in (("continue", LOOP_CONTINUE), ("flag", LOOP_FLAG)):
+        body = ("  int count = 0; int sc = 0; float acc = 0.0;" + loop +
+                "\n  o0 = vec4(float(count), float(sc), acc, 0.0); o1 = r;")
+        got = run_tiny(td, "lc_" + tag, body, x, W, H)[0][:, :3].astype(np.float64)
+        wrong[tag] = int(np.sum(np.any(np.abs(got - want) > 1e-4 * np.maximum(1.0, np.abs(want)), axis=1)))
+    if wrong["flag"]:
+        raise RuntimeError(f"the flag form of the loop is mis-executed too: {wrong}")
+    return wrong["continue"]
+
+
+def run_tiny(td, tag, body, x, W, H, math=False):
+    """run a fragment program `body` (reads vec4 r = the RNG texel, writes
+    o0 and o1) over a W x H grid whose RNG state is x; returns (o0, o1)"""
+    n = W * H
+    vs = ("#version 300 es\nprecision highp float;\nin vec3 Position;\nin vec2 TexCoord;\nout vec2 vTexCoord;\n"
+          "void main(){gl_Position=vec4(Position,1.0);vTexCoord=TexCoord;}\n")
+    fs = ("#version 300 es\nprecision highp float;\nuniform sampler2D Radiance; uniform sampler2D RngData;\n"
+          "in vec2 vTexCoord;\nlayout(location=0) out vec4 o0; layout(location=1) out vec4 o1;\n"
+          "void main(){\n  vec4 r = texture(RngData, vTexCoord);\n" + body + "\n}\n")
+    if math:
+        fs = with_precise_math(fs)
+    f = {k: os.path.join(td, f"{tag}_{k}") for k in ("vs", "fs", "rad", "rng", "o0", "o1", "spec")}
+    open(f["vs"], "w").write(vs)
+    open(f["fs"], "w").write(fs)
+    np.tile(np.float32([0, 0, 0, 1]), (n, 1)).tofile(f["rad"])
+    np.asarray(x, np.float32).tofile(f["rng"])
+    open(f["spec"], "w").write(f"size {W} {H}\nframes 1\nvertex {f['vs']}\nfragment {f['fs']}\n"
+                               f"state {f['rad']} {f['rng']}\nout {f['o0']} {f['o1']}\n")
+    subprocess.run([HARNESS, GLES_DIR, f["spec"]], check=True, capture_output=True, timeout=120)
+    return (np.fromfile(f["o0"], np.float32).reshape(-1, 4), np.fromfile(f["o1"], np.float32).reshape(-1, 4))
+
+
+def check_min_max(td):
+    """min, max and clamp of the math block follow the GLSL definitions with a
+    NaN operand: min(3, NaN) = 3, max(1e-6, NaN) = 1e-6, min(NaN, 3) = NaN,
+    and agree with the builtins on ordinary values"""
+    x = np.float32([[-1, 4, 0, 0], [-2, 9, 0, 0], [4, -3, 0, 0], [0.25, 0.5, 0, 0]])
+    body = ("  float a = sqrt(r.x);\n"
+            "  o0 = vec4(min(3.0, a), max(1e-6, a), min(a, 3.0), clamp(a, 0.0, 1.0));\n"
+            "  vec3 v = min(vec3(3.0), vec3(a)); vec3 w = max(vec3(r.y), 0.5);\n"
+            "  o1 = vec4(v.x, w.x, clamp(r.y, -1.0, 1.0), float(min(int(r.y), 2)));")
+    o0, o1 = run_tiny(td, "minmax", body, x, 2, 2, math=True)
+    a = np.sqrt(np.where(x[:, 0] >= 0, x[:, 0], np.nan)).astype(np.float32)
+    nan = np.isnan(a)
+    ok = (np.array_equal(o0[nan, 0], np.full(nan.sum(), 3.0, np.float32)) and
+          np.allclose(o0[nan, 1], 1e-6) and np.all(np.isnan(o0[nan, 2])) and np.all(np.isnan(o0[nan, 3])) and
+          np.allclose(o0[~nan, 0], np.minimum(3.0, a[~nan])) and np.allclose(o0[~nan, 3], np.clip(a[~nan], 0, 1)) and
+          np.array_equal(o1[nan, 0], np.full(nan.sum(), 3.0, np.float32)) and
+          np.allclose(o1[:, 1], np.maximum(x[:, 1], 0.5)) and np.allclose(o1[:, 2], np.clip(x[:, 1], -1, 1)) and
+          np.allclose(o1[:, 3], np.minimum(np.trunc(x[:, 1]), 2)))
+    if not ok:
+        raise RuntimeError(f"min/max/clamp of the math block do not follow the GLSL definitions:\n{o0}\n{o1}")
 
 
 def uniform_lines(u):
@@ -243,6 +408,8 @@ def main():
     names = sys.argv[1:] or list(reference_cases.CASES)
     with tempfile.TemporaryDirectory() as td:
         print("precise_math ulp:", json.dumps({k: round(v, 2) for k, v in check_precise_math(td).items()}))
+        print("loop with continue: wrong results at", check_loop_control(td), "of 256 pixels; flag form exact")
+        check_min_max(td)
         for n in names:
             make(n, td)
 

@@ -52,6 +52,85 @@ float SDF_SURFACE(vec3 X) { return X.y + 1.0; }
     s.oracle_albedo_hook = 0
 
 
+def _atmosphere(s):
+    """sphere over plane inside a homogeneous scattering box the camera looks
+    into from outside (atmosphereMaxZ = 2.5 < camera z = 4), sun and sky,
+    anisotropic phase function, up to two real scatters along a path"""
+    r = s.renderer
+    r.sunPower = 0.5
+    r.sunAngularSize = 8.0
+    r.sunLatitude = 40.0
+    r.sunLongitude = 30.0
+    r.maxAtmosphereScatters = 3
+    v = s.materials.volume
+    v.atmosphereMinX, v.atmosphereMaxX = -3.0, 3.0
+    v.atmosphereMinY, v.atmosphereMaxY = -1.5, 3.0
+    v.atmosphereMinZ, v.atmosphereMaxZ = -3.0, 2.5
+    v.lof10_mfp = 0.3
+    v.scatteringColor = [0.5, 0.6, 0.7]
+    v.absorptionColor = [0.1, 0.1, 0.1]
+    v.anisotropy = 0.4
+
+
+def _fog_sphere_light(s):
+    """the faked fog (transmittance on light and shadow segments) and a
+    sphere light, no atmosphere"""
+    r = s.renderer
+    r.sphereLightPower = 1.5
+    r.sphereLightRadius = 0.3
+    r.sphereLightPosition = [1.5, 1.0, 1.0]
+    r.sphereLightColor = [1.0, 0.8, 0.6]
+    v = s.materials.volume
+    v.fogEnable = True
+    v.fogMFP = 0.5
+    v.fogMaxOpticalDepth = 2.0
+    v.fogTint = [0.4, 0.5, 0.6]
+    v.fogEmission = [0.8, 0.7, 0.6]
+
+
+def _atmosphere_fog_light(s):
+    """atmosphere, fog and sphere light together: the fog's in-scattered
+    emission, the sphere light sampled from the medium and hit through it"""
+    _fog_sphere_light(s)
+    v = s.materials.volume
+    v.atmosphereMinX, v.atmosphereMaxX = -10.0, 10.0
+    v.atmosphereMinY, v.atmosphereMaxY = -2.0, 10.0
+    v.atmosphereMinZ, v.atmosphereMaxZ = -10.0, 10.0
+    v.lof10_mfp = 0.6
+    v.scatteringColor = [0.4, 0.4, 0.4]
+    v.absorptionColor = [0.05, 0.1, 0.15]
+    v.anisotropy = -0.3
+    s.renderer.maxAtmosphereScatters = 2
+
+
+def _volume_emission(s):
+    """a volume emission field (VOLUME_EMISSION hook) in a scattering
+    atmosphere: the emission added at surface hits and the in-scattering
+    estimate's emission probe ray"""
+    _atmosphere(s)
+    s.glsl += scenes.VOLUME_EMISSION_GLSL
+    s.oracle_emission_hook = 1
+    v = s.materials.volume
+    v.emission = 0.4
+    v.emissionColor = [1.0, 0.5, 0.25]
+    s.renderer.maxAtmosphereScatters = 2
+
+
+def _subsurface(s):
+    """sub-surface random walk under the diffuse sphere and plane, sun on"""
+    su = s.materials.surface
+    su.subsurface = 0.7
+    su.subsurfaceMFP = 0.05
+    su.subsurfaceAlbedoRGB = [0.9, 0.6, 0.4]
+    su.subsurfaceAnisotropy = 0.3
+    su.subsurfaceDiffuseWeight = 0.5
+    su.specAlbedo = [0.2, 0.2, 0.2]
+    su.ior = 1.4
+    s.renderer.sunPower = 0.3
+    s.renderer.sunLatitude = 50.0
+    s.renderer.maxSSSSteps = 64
+
+
 CASES = {
     "sphere_plane": dict(builder="sphere_plane", kwargs=dict(width=64, height=64, spp=16), frames=16, seed=1234),
     "furnace_rgb": dict(builder="furnace", kwargs=dict(width=32, height=32, spp=8, dispersive=False), frames=8,
@@ -65,6 +144,16 @@ CASES = {
                             tweak=_tabulated_metal),
     "bk7_sphere": dict(builder="sphere_plane", kwargs=dict(width=48, height=40, spp=4), frames=4, seed=3,
                        tweak=_bk7_sphere),
+    "atmosphere": dict(builder="sphere_plane", kwargs=dict(width=40, height=32, spp=4), frames=4, seed=21,
+                       tweak=_atmosphere),
+    "fog_sphere_light": dict(builder="sphere_plane", kwargs=dict(width=40, height=32, spp=4), frames=4, seed=22,
+                             tweak=_fog_sphere_light),
+    "atmosphere_fog_light": dict(builder="sphere_plane", kwargs=dict(width=40, height=32, spp=4), frames=4,
+                                 seed=23, tweak=_atmosphere_fog_light),
+    "volume_emission": dict(builder="sphere_plane", kwargs=dict(width=40, height=32, spp=4), frames=4, seed=25,
+                            tweak=_volume_emission),
+    "subsurface": dict(builder="sphere_plane", kwargs=dict(width=40, height=32, spp=4), frames=4, seed=24,
+                       tweak=_subsurface),
 }
 
 

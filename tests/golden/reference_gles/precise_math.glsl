@@ -131,6 +131,31 @@ vec2 snl_atan(vec2 a, vec2 b) { return vec2(snl_atan(a.x, b.x), snl_atan(a.y, b.
 vec3 snl_atan(vec3 a, vec3 b) { return vec3(snl_atan(a.x, b.x), snl_atan(a.y, b.y), snl_atan(a.z, b.z)); }
 #undef SNL_VEC1
 
+// min, max and clamp as GLSL ES 3.00 defines them (section 8.3): min(x, y) is
+// "y if y < x, otherwise x", max(x, y) "y if x < y, otherwise x", clamp(x, a, b)
+// is min(max(x, a), b).  SwiftShader 4.1's builtins return the second operand
+// when either is NaN (min(3.0, NaN) = NaN), the definition returns x.
+#define SNL_MINMAX(T, B, S)                                                      \
+  T snl_min(T x, T y) { return mix(x, y, lessThan(y, x)); }                      \
+  T snl_max(T x, T y) { return mix(x, y, lessThan(x, y)); }                      \
+  T snl_min(T x, S y) { return snl_min(x, T(y)); }                               \
+  T snl_max(T x, S y) { return snl_max(x, T(y)); }                               \
+  T snl_clamp(T x, T a, T b) { return snl_min(snl_max(x, a), b); }               \
+  T snl_clamp(T x, S a, S b) { return snl_min(snl_max(x, T(a)), T(b)); }
+float snl_min(float x, float y) { return y < x ? y : x; }
+float snl_max(float x, float y) { return x < y ? y : x; }
+float snl_clamp(float x, float a, float b) { return snl_min(snl_max(x, a), b); }
+int snl_min(int x, int y) { return y < x ? y : x; }
+int snl_max(int x, int y) { return x < y ? y : x; }
+int snl_clamp(int x, int a, int b) { return snl_min(snl_max(x, a), b); }
+SNL_MINMAX(vec2, bvec2, float)
+SNL_MINMAX(vec3, bvec3, float)
+SNL_MINMAX(vec4, bvec4, float)
+#undef SNL_MINMAX
+
+#define min snl_min
+#define max snl_max
+#define clamp snl_clamp
 #define sin snl_sin
 #define cos snl_cos
 #define tan snl_tan

@@ -107,3 +107,32 @@ def test_batch_with_unit0_sampler_job(ctx):
     sb.close()
     assert np.array_equal(ra, run_alone(ctx, t, 5)[0])
     assert np.array_equal(rb, run_alone(ctx, other, 6)[0])
+
+
+@pytest.mark.gpu
+def test_batch_with_megakernel_jobs(ctx):
+    """sub-surface and atmosphere jobs run the megakernel inside a batch; each
+    equals the job rendered alone with the megakernel"""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import reference_cases
+    jobs = []
+    for name, side in (("subsurface", 40), ("atmosphere", 32)):
+        s = scenes.sphere_plane(side, side, 3)
+        reference_cases.CASES[name]["tweak"](s)
+        jobs.append(s)
+    jobs.append(scenes.batch_job("menger3", "METAL", 36, 3, 4, np.random.RandomState(3)))
+    sessions = [host.Session(s, s.width, s.height, ctx, seed=40 + j, flags=native.BUILD_PRECISE)
+                for j, s in enumerate(jobs)]
+    native.render_batch([(se.prog, se.state, s.spp) for se, s in zip(sessions, jobs)])
+    for j, (se, s) in enumerate(zip(sessions, jobs)):
+        rad, rng = se.radiance().copy(), se.state.read(1).copy()
+        se.close()
+        alone = host.Session(s, s.width, s.height, ctx, seed=40 + j, flags=native.BUILD_PRECISE)
+        if j < 2:
+            alone.prog.engine(native.ENGINE_MEGAKERNEL)
+        alone.render(s.spp)
+        assert np.array_equal(rad, alone.radiance()), s.name
+        assert np.array_equal(rng, alone.state.read(1)), s.name
+        alone.close()

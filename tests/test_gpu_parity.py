@@ -23,8 +23,8 @@ pytestmark = pytest.mark.gpu
 PRECISE = native.BUILD_PRECISE
 
 
-ENGINES = [native.ENGINE_WAVEFRONT, native.ENGINE_PERSISTENT]
-ENGINE_IDS = ["wavefront", "persistent"]
+ENGINES = [native.ENGINE_WAVEFRONT, native.ENGINE_PERSISTENT, native.ENGINE_MEGAKERNEL]
+ENGINE_IDS = ["wavefront", "persistent", "megakernel"]
 
 
 def session(ctx, scene, w, h, flags=PRECISE, engine=native.ENGINE_PERSISTENT):
@@ -227,3 +227,48 @@ def test_uniform_semantics(ctx):
     assert np.array_equal(sess.radiance(), ref.radiance())
     sess.close()
     ref.close()
+
+
+def _case_scene(name, W, H, spp):
+    """a reference fixture case (tests/golden/reference_cases.py) at another size"""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import reference_cases
+    c = reference_cases.CASES[name]
+    s = getattr(scenes, c["builder"])(W, H, spp)
+    c["tweak"](s)
+    return s
+
+
+@pytest.mark.parametrize("name", ["atmosphere", "atmosphere_fog_light"])
+def test_pathtrace_atmosphere(ctx, name):
+    """homogeneous atmosphere (in-scattering estimates, up to two real scatters,
+    fog, sphere light through the medium): megakernel against the oracle"""
+    s = _case_scene(name, 64, 48, 8)
+    g, r, close, rng_same, _ = compare_render(ctx, s, 64, 48, 8, engine=native.ENGINE_MEGAKERNEL)
+    assert close >= 0.97 and rng_same >= 0.97, (close, rng_same)
+    assert rrmse(unbiased(g), unbiased(r)) < 1e-2
+
+
+def test_atmosphere_ignores_engine_choice(ctx):
+    """atmosphere programs always take the megakernel: the engine knob does not
+    change a bit"""
+    s = _case_scene("atmosphere", 32, 24, 2)
+    out = []
+    for e in ENGINES:
+        sess = session(ctx, s, 32, 24, engine=e)
+        sess.render(2)
+        out.append((sess.radiance().copy(), sess.state.read(1).copy()))
+        sess.close()
+    for rad, rng in out[1:]:
+        assert np.array_equal(rad, out[0][0]) and np.array_equal(rng, out[0][1])
+
+
+@pytest.mark.parametrize("engine", [native.ENGINE_WAVEFRONT, native.ENGINE_MEGAKERNEL], ids=["wavefront", "megakernel"])
+def test_pathtrace_subsurface(ctx, engine):
+    """sub-surface random walk (randomwalk_SSS, SSS_exit_radiance) against the oracle"""
+    s = _case_scene("subsurface", 64, 48, 8)
+    g, r, close, rng_same, _ = compare_render(ctx, s, 64, 48, 8, engine=engine)
+    assert close >= 0.97 and rng_same >= 0.97, (close, rng_same)
+    assert rrmse(unbiased(g), unbiased(r)) < 1e-2

@@ -123,15 +123,20 @@ def test_oracle_first_hit_matches_reference(name):
 
 # ---------------------------------------------------------------- GPU: the HIP path
 
+ENGINE_OF = {"persistent": 1, "wavefront": 0, "megakernel": 2}  # native.ENGINE_*
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("engine", ["persistent", "wavefront"])
+@pytest.mark.parametrize("engine", list(ENGINE_OF))
 @pytest.mark.parametrize("name", CASES)
 def test_gpu_pathtrace_matches_reference(ctx, name, engine):
+    """atmosphere programs render with the megakernel whichever engine is
+    asked for; sub-surface programs with the wavefront unless the megakernel is"""
     from snelly_amd import host, native
     ref, _ = fixture(name)
     s, W, H, frames, seed = reference_cases.build(name)
     sess = host.Session(s, W, H, ctx, seed=seed, flags=native.BUILD_PRECISE)
-    sess.prog.engine(native.ENGINE_PERSISTENT if engine == "persistent" else native.ENGINE_WAVEFRONT)
+    sess.prog.engine(ENGINE_OF[engine])
     sess.render(frames)
     rad = sess.radiance().reshape(-1, 4)
     rng = sess.state.read(1).reshape(-1, 4)

@@ -165,6 +165,12 @@ def main():
     wavefront = {"path_samples_per_s": round(W * H * spp / (wf.ms_total * 1e-3), 1), "ms": round(wf.ms_total, 2),
                  "trace_ms": round(wf.ms_trace, 2),
                  "trace_tflops": round(wf.trace_steps * scene.flops_per_step / (wf.ms_trace * 1e-3) / 1e12, 3)}
+    # the megakernel (cameraPath per lane, rays traced in place), for comparison
+    sess.prog.engine(native.ENGINE_MEGAKERNEL)
+    sess.state.seed(7)
+    sess.render(spp)
+    mk = sess.prog.stats()
+    megakernel = {"path_samples_per_s": round(W * H * spp / (mk.ms_total * 1e-3), 1), "ms": round(mk.ms_total, 2)}
     # the persistent engine with an autotuned launch shape (untimed)
     best, _, _ = autotune.tune(sess, frames=8)
     for w in range(args.warmup):
@@ -225,7 +231,7 @@ def main():
         "config": {"workload": "menger4_metal_spectral", "width": W, "height": H, "spp": spp, "bounces": 8,
                    "mode": "plain", "engine": "persistent", "build": "fast", "launch_shape": best,
                    "path_samples_per_step": W * H * spp},
-        "engines": {"wavefront": wavefront},
+        "engines": {"wavefront": wavefront, "megakernel": megakernel},
         "roofline": roofline,
         "rrmse_per_step": None if rr is None else round(rr, 5),
         "spp_for_rrmse_1pct": None if rr is None else int(math.ceil(spp * (rr / 0.01) ** 2)),

@@ -18,6 +18,18 @@ SNL_HD float sample_jitter(float xi) {  // triangle filter (1928-1932)
   return xi < 0.5f ? sqrt(2.0f * xi) - 1.0f : 1.0f - sqrt(2.0f - 2.0f * xi);
 }
 
+// The pixel a queue item stands for.  Where the image divides into 8x8 tiles,
+// 64 consecutive items cover one tile, so a wave's lanes start neighbouring
+// rays; otherwise items are pixels in row order.  Chains of a multi-chain
+// state follow one another.
+__device__ inline int snl_item_pixel(const SnlKArgs& a, int item) {
+  if ((a.W & 7) || (a.H & 7)) return item;
+  int chain = item / a.imagePix, i = item - chain * a.imagePix;
+  int tile = i >> 6, j = i & 63, tw = a.W >> 3;
+  int tr = tile / tw, tc = tile - tr * tw;
+  return chain * a.imagePix + (tr * 8 + (j >> 3)) * a.W + tc * 8 + (j & 7);
+}
+
 __device__ inline vec4 ld4(const float* p, long long i) {
   const float4 v = reinterpret_cast<const float4*>(p)[i];
   return vec4(v.x, v.y, v.z, v.w);
@@ -403,7 +415,9 @@ __device__ inline void shade_hit(const SnlKArgs& a) {
 
         float skyPdf = 0.0f, sunPdf = 0.0f, sphPdf = 0.0f;
         if (!inDielectric) {
-          // directSurfaceLighting (1221-1295) with the shadow rays deferred
+          // directSurfaceLighting (1221-1295) with the shadow rays deferred; a
+          // light whose BSDF value is exactly zero adds exactly zero, its
+          // shadow ray is not traced
           int nmask = 0;
           vec3 wL, wW;
           if (P.skyPower > RADIANCE_EPSILON) {
@@ -416,7 +430,7 @@ __device__ inline void shade_hit(const SnlKArgs& a) {
               stRT(a, b + 4, i, fl); stRT(a, b + 7, i, Li);
               SNL_PF(b + 10, i) = max(PDF_EPSILON, skyPdf); SNL_PF(b + 11, i) = abs(dot(wW, basis.nW));
               SNL_PF(b + 12, i) = powerHeuristic(skyPdf, lp);
-              nmask |= 1;
+              if (anyNonzero(fl)) nmask |= 1;
             }
           }
           if (P.sunPower > RADIANCE_EPSILON) {
@@ -429,7 +443,7 @@ __device__ inline void shade_hit(const SnlKArgs& a) {
               stRT(a, b + 4, i, fl); stRT(a, b + 7, i, Li);
               SNL_PF(b + 10, i) = max(PDF_EPSILON, sunPdf); SNL_PF(b + 11, i) = abs(dot(wW, basis.nW));
               SNL_PF(b + 12, i) = powerHeuristic(sunPdf, lp);
-              nmask |= 2;
+              if (anyNonzero(fl)) nmask |= 2;
             }
           }
 #ifdef HAS_SPHERE_LIGHT
@@ -447,7 +461,7 @@ __device__ inline void shade_hit(const SnlKArgs& a) {
               stRT(a, b + 4, i, fl); stRT(a, b + 7, i, Li);
               SNL_PF(b + 10, i) = max(PDF_EPSILON, sphPdf); SNL_PF(b + 11, i) = abs(dot(wW, basis.nW));
               SNL_PF(b + 12, i) = powerHeuristic(sphPdf, lp);
-              nmask |= 4;
+              if (anyNonzero(fl)) nmask |= 4;
             }
           }
 #endif

@@ -7,8 +7,9 @@
 // random draws, which the queue-based engines do not stage.  It follows
 // cameraPath (pathtracer-fragment-shader.glsl:1625-1926) with
 // directSurfaceLighting (1221-1295), the atmosphere routines (1297-1457) and
-// the sub-surface walk (1468-1593).  A wave takes 64 pixels at a time from the
-// launch's queue; each lane runs all of its pixel's frames.
+// the sub-surface walk (1468-1593).  Each lane runs all frames of a pixel of
+// the launch's queue, one path vertex per loop iteration: a lane whose path
+// ends starts the next at once, and one whose pixel is done takes another.
 #pragma once
 
 namespace glsl {
@@ -27,7 +28,9 @@ struct Mega {
     return pt.transmittanceOverFreeSegment(pW, rayDir, segmentLength, rgb);
   }
 
-  // directSurfaceLighting (1221-1295)
+  // directSurfaceLighting (1221-1295).  A BSDF value of exactly zero makes a
+  // light's term exactly zero whatever its shadow ray finds, so that ray is
+  // not traced (the BSDF evaluation draws no random numbers).
   SNL_HD RT directSurfaceLighting(const vec3& pW, const Basis& basis, const vec3& winputW, int material, float wl,
                                   const vec3& rgb, vec4& rnd, float& skyPdf, float& sunPdf, float& sphPdf) {
     vec3 winputL = worldToLocal(winputW, basis);
@@ -36,26 +39,30 @@ struct Mega {
     if (P.skyPower > RADIANCE_EPSILON) {
       RT Li = pt.sampleSkyAtSurface(basis, rgb, rnd, woutputL, woutputW, skyPdf);
       if (averageComponent(Li) > RADIANCE_EPSILON) {
-        RT Tr = transmittanceOverSegment(pW, woutputW, P.maxLengthScale, rgb);
-        Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
-        if (averageComponent(Li) > RADIANCE_EPSILON) {
-          float bsdfPdf = max(PDF_EPSILON, pt.pdfBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true));
-          RT f = pt.evaluateBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true);
-          float misWeight = powerHeuristic(skyPdf, bsdfPdf);
-          Ldirect += f * Li / max(PDF_EPSILON, skyPdf) * abs(dot(woutputW, basis.nW)) * misWeight;
+        RT f = pt.evaluateBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true);
+        if (anyNonzero(f)) {
+          RT Tr = transmittanceOverSegment(pW, woutputW, P.maxLengthScale, rgb);
+          Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
+          if (averageComponent(Li) > RADIANCE_EPSILON) {
+            float bsdfPdf = max(PDF_EPSILON, pt.pdfBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true));
+            float misWeight = powerHeuristic(skyPdf, bsdfPdf);
+            Ldirect += f * Li / max(PDF_EPSILON, skyPdf) * abs(dot(woutputW, basis.nW)) * misWeight;
+          }
         }
       }
     }
     if (P.sunPower > RADIANCE_EPSILON) {
       RT Li = pt.sampleSunAtSurface(basis, rgb, rnd, woutputL, woutputW, sunPdf);
       if (averageComponent(Li) > RADIANCE_EPSILON) {
-        RT Tr = transmittanceOverSegment(pW, woutputW, P.maxLengthScale, rgb);
-        Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
-        if (averageComponent(Li) > RADIANCE_EPSILON) {
-          float bsdfPdf = max(PDF_EPSILON, pt.pdfBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true));
-          RT f = pt.evaluateBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true);
-          float misWeight = powerHeuristic(sunPdf, bsdfPdf);
-          Ldirect += f * Li / max(PDF_EPSILON, sunPdf) * abs(dot(woutputW, basis.nW)) * misWeight;
+        RT f = pt.evaluateBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true);
+        if (anyNonzero(f)) {
+          RT Tr = transmittanceOverSegment(pW, woutputW, P.maxLengthScale, rgb);
+          Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
+          if (averageComponent(Li) > RADIANCE_EPSILON) {
+            float bsdfPdf = max(PDF_EPSILON, pt.pdfBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true));
+            float misWeight = powerHeuristic(sunPdf, bsdfPdf);
+            Ldirect += f * Li / max(PDF_EPSILON, sunPdf) * abs(dot(woutputW, basis.nW)) * misWeight;
+          }
         }
       }
     }
@@ -67,13 +74,15 @@ struct Mega {
         float hitDist = max(0.0f, length(pHit - pW) - 3.0f * P.minLengthScale);
         woutputW = normalize(pHit - pW);
         woutputL = worldToLocal(woutputW, basis);
-        RT Tr = transmittanceOverSegment(pW, woutputW, hitDist, rgb);
-        Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
-        if (averageComponent(Li) > RADIANCE_EPSILON) {
-          float bsdfPdf = max(PDF_EPSILON, pt.pdfBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true));
-          RT f = pt.evaluateBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true);
-          float misWeight = powerHeuristic(sphPdf, bsdfPdf);
-          Ldirect += f * Li / max(PDF_EPSILON, sphPdf) * abs(dot(woutputW, basis.nW)) * misWeight;
+        RT f = pt.evaluateBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true);
+        if (anyNonzero(f)) {
+          RT Tr = transmittanceOverSegment(pW, woutputW, hitDist, rgb);
+          Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
+          if (averageComponent(Li) > RADIANCE_EPSILON) {
+            float bsdfPdf = max(PDF_EPSILON, pt.pdfBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true));
+            float misWeight = powerHeuristic(sphPdf, bsdfPdf);
+            Ldirect += f * Li / max(PDF_EPSILON, sphPdf) * abs(dot(woutputW, basis.nW)) * misWeight;
+          }
         }
       }
     }
@@ -261,200 +270,226 @@ struct Mega {
     return min(rt1(P.radianceClamp), Ldirect);
   }
 
-  // cameraPath (1625-1926)
-  SNL_HD RT cameraPath(const vec3& primaryStart, const vec3& primaryDir, float wl, const vec3& rgb, vec4& rnd,
-                       int maxBounces, long long& normals) {
-    RT L = rt1(0.0f);
-    float misWeightSky = 1.0f, misWeightSun = 1.0f, misWeightSph = 1.0f;
-    vec3 pW = primaryStart, rayDir = primaryDir;
-    bool inDielectric = false;
-    if constexpr (SC::kHasDielectric) inDielectric = pt.S.SDF_DIELECTRIC(primaryStart) < 0.0f;
-    RT throughput = rt1(1.0f);
-    int atmosphere_scatters = 0;
+  // one camera path (cameraPath, 1625-1926) as the state of a lane, so that a
+  // lane whose path ends starts its next one at once instead of waiting for
+  // the longest path of its wave
+  struct Path {
+    vec3 pW, rayDir;
+    RT throughput, L;
+    float misWeightSky, misWeightSun, misWeightSph;
+    int vertex, atmosphere_scatters;
+    bool inDielectric;
+  };
+  SNL_HD void startPath(Path& ps, const vec3& primaryStart, const vec3& primaryDir) {
+    ps.L = rt1(0.0f);
+    ps.misWeightSky = ps.misWeightSun = ps.misWeightSph = 1.0f;
+    ps.pW = primaryStart;
+    ps.rayDir = primaryDir;
+    ps.inDielectric = false;
+    if constexpr (SC::kHasDielectric) ps.inDielectric = pt.S.SDF_DIELECTRIC(primaryStart) < 0.0f;
+    ps.throughput = rt1(1.0f);
+    ps.atmosphere_scatters = 0;
+    ps.vertex = 0;
+  }
+  // the loop's increment and condition: whether the path goes on
+  SNL_HD static bool advance(Path& ps, int maxBounces) { return ++ps.vertex <= maxBounces; }
+
+  // the body of cameraPath's vertex loop for vertex ps.vertex: `break` ends the
+  // path (false), `continue` and the end of the body go to the next vertex
+  SNL_HD bool vertexStep(Path& ps, float wl, const vec3& rgb, vec4& rnd, int maxBounces, long long& normals) {
+    RT& L = ps.L;
+    RT& throughput = ps.throughput;
+    vec3& pW = ps.pW;
+    vec3& rayDir = ps.rayDir;
+    float& misWeightSky = ps.misWeightSky;
+    float& misWeightSun = ps.misWeightSun;
+    float& misWeightSph = ps.misWeightSph;
+    bool& inDielectric = ps.inDielectric;
+    int& atmosphere_scatters = ps.atmosphere_scatters;
+    const int vertex = ps.vertex;
     (void)misWeightSph;
     (void)atmosphere_scatters;
-    for (int vertex = 0; vertex <= maxBounces; ++vertex) {
-      if (maxComponent(throughput) < THROUGHPUT_EPSILON) break;
-      vec3 pW_next;
-      int hitMaterial;
-      bool hit = pt.traceRay(pW, rayDir, pW_next, hitMaterial, P.maxLengthScale);
-      float rayLength = P.maxLengthScale;
-      if (hit) {
-        rayLength = length(pW_next - pW);
-      } else {
-        float lightSegmentLength = P.maxLengthScale;
-        RT Tr = rt1(1.0f);
-        bool litSph = false;
-#ifdef HAS_SPHERE_LIGHT
-        {
-          vec3 pLightHit;
-          RT LiSph = (P.sphereLightPower < RADIANCE_EPSILON) ? rt1(0.0f) : pt.sphereLightRadiance(pW, rayDir, rgb, pLightHit);
-          if (averageComponent(LiSph) > RADIANCE_EPSILON) {
-            litSph = true;
-            lightSegmentLength = length(pLightHit - pW);
-            Tr = pt.transmittanceOverFreeSegment(pW, rayDir, lightSegmentLength, rgb);
-            float scatter_prob = pt.atmosphericScatteringProbability(Tr, rgb);
-            L += throughput * Tr * misWeightSph * LiSph / (1.0f - scatter_prob);
-          }
-        }
-#endif
-        if (!litSph) {
-          lightSegmentLength = P.maxLengthScale;
-          Tr = pt.transmittanceOverFreeSegment(pW, rayDir, lightSegmentLength, rgb);
-          float scatter_prob = pt.atmosphericScatteringProbability(Tr, rgb);
-          if (averageComponent(Tr) > RADIANCE_EPSILON) {
-            if (!(vertex == 0 && !P.envMapVisible) && misWeightSky > 0.0f)
-              L += throughput * Tr * misWeightSky * pt.environmentRadiance(rayDir, rgb) / (1.0f - scatter_prob);
-            if (!(vertex == 0 && !P.sunVisibleDirectly) && misWeightSun > 0.0f)
-              L += throughput * Tr * misWeightSun * pt.sunRadiance(rayDir, rgb) / (1.0f - scatter_prob);
-          }
-        }
-        if constexpr (SC::kHasAtmosphere) {
-          if (vertex < 2) L += throughput * atmosphericInscatteringRadiance(pW, rayDir, lightSegmentLength, rgb, rnd);
-          if (atmosphere_scatters + 1 < SNL_MAX_ATMOSPHERE_SCATTERS) {
-            float scatter_prob = pt.atmosphericScatteringProbability(Tr, rgb);
-            if (scatter_prob > snl_rand(rnd)) {
-              vec3 pW_scatter(0.0f), woutputW(0.0f);
-              throughput *= atmosphericScatterSample(pW, rayDir, lightSegmentLength, rgb, rnd, pW_scatter, woutputW) /
-                            max(DENOM_TOLERANCE, scatter_prob);
-              pW = pW_scatter;
-              rayDir = woutputW;
-              atmosphere_scatters++;
-              continue;
-            }
-          }
-        }
-        break;
-      }
-      if (hitMaterial < 0) break;  // a max-step "hit" has no material: the path ends
-
+    (void)vertex;
+    if (maxComponent(throughput) < THROUGHPUT_EPSILON) return false;
+    vec3 pW_next;
+    int hitMaterial;
+    bool hit = pt.traceRay(pW, rayDir, pW_next, hitMaterial, P.maxLengthScale);
+    float rayLength = P.maxLengthScale;
+    if (hit) {
+      rayLength = length(pW_next - pW);
+    } else {
+      float lightSegmentLength = P.maxLengthScale;
       RT Tr = rt1(1.0f);
-      bool hitSphereLight = false;
+      bool litSph = false;
 #ifdef HAS_SPHERE_LIGHT
       {
         vec3 pLightHit;
         RT LiSph = (P.sphereLightPower < RADIANCE_EPSILON) ? rt1(0.0f) : pt.sphereLightRadiance(pW, rayDir, rgb, pLightHit);
         if (averageComponent(LiSph) > RADIANCE_EPSILON) {
-          float lightSegmentLength = length(pLightHit - pW);
-          if (lightSegmentLength < rayLength) {
-            hitSphereLight = true;
-            Tr = pt.transmittanceOverFreeSegment(pW, rayDir, lightSegmentLength, rgb);
-            L += throughput * Tr * misWeightSph * LiSph;
-            if constexpr (SC::kHasAtmosphere) {
-              if (vertex < 2) L += throughput * atmosphericInscatteringRadiance(pW, rayDir, lightSegmentLength, rgb, rnd);
-            }
-          }
+          litSph = true;
+          lightSegmentLength = length(pLightHit - pW);
+          Tr = pt.transmittanceOverFreeSegment(pW, rayDir, lightSegmentLength, rgb);
+          float scatter_prob = pt.atmosphericScatteringProbability(Tr, rgb);
+          L += throughput * Tr * misWeightSph * LiSph / (1.0f - scatter_prob);
         }
       }
 #endif
-      if constexpr (SC::kHasDielectric) {
-        if (inDielectric) throughput *= exp(-rayLength * pt.DIELECTRIC_ABSORPTION_EVAL(pW, rgb));
+      if (!litSph) {
+        lightSegmentLength = P.maxLengthScale;
+        Tr = pt.transmittanceOverFreeSegment(pW, rayDir, lightSegmentLength, rgb);
+        float scatter_prob = pt.atmosphericScatteringProbability(Tr, rgb);
+        if (averageComponent(Tr) > RADIANCE_EPSILON) {
+          if (!(vertex == 0 && !P.envMapVisible) && misWeightSky > 0.0f)
+            L += throughput * Tr * misWeightSky * pt.environmentRadiance(rayDir, rgb) / (1.0f - scatter_prob);
+          if (!(vertex == 0 && !P.sunVisibleDirectly) && misWeightSun > 0.0f)
+            L += throughput * Tr * misWeightSun * pt.sunRadiance(rayDir, rgb) / (1.0f - scatter_prob);
+        }
       }
       if constexpr (SC::kHasAtmosphere) {
-        if (vertex < 2) L += throughput * atmosphericInscatteringRadiance(pW, rayDir, rayLength, rgb, rnd);
-        if (!hitSphereLight) Tr = pt.transmittanceOverFreeSegment(pW, rayDir, rayLength, rgb);
+        if (vertex < 2) L += throughput * atmosphericInscatteringRadiance(pW, rayDir, lightSegmentLength, rgb, rnd);
         if (atmosphere_scatters + 1 < SNL_MAX_ATMOSPHERE_SCATTERS) {
           float scatter_prob = pt.atmosphericScatteringProbability(Tr, rgb);
           if (scatter_prob > snl_rand(rnd)) {
             vec3 pW_scatter(0.0f), woutputW(0.0f);
-            throughput *= atmosphericScatterSample(pW, rayDir, rayLength, rgb, rnd, pW_scatter, woutputW) /
+            throughput *= atmosphericScatterSample(pW, rayDir, lightSegmentLength, rgb, rnd, pW_scatter, woutputW) /
                           max(DENOM_TOLERANCE, scatter_prob);
             pW = pW_scatter;
             rayDir = woutputW;
             atmosphere_scatters++;
-            continue;
-          } else {
-            throughput /= max(DENOM_TOLERANCE, 1.0f - scatter_prob);
+            return advance(ps, maxBounces);
           }
         }
-        if (hitSphereLight) break;
-        throughput *= pt.transmittanceOverFreeSegment(pW, rayDir, rayLength, rgb);
       }
-      (void)Tr;
-      (void)hitSphereLight;
+      return false;
+    }
+    if (hitMaterial < 0) return false;  // a max-step "hit" has no material: the path ends
 
-      // surface interaction
-      pW = pW_next;
-      vec3 nW = pt.normal(pW, hitMaterial, true);
-      ++normals;
-      vec3 ngW = nW;
-      Basis basis = makeBasis(nW);
-      if constexpr (SC::kHasNormalMap) {
-        nW = pt.perturbNormal(pW, basis, hitMaterial);
-        if ((dot(nW, rayDir) > 0.0f) && (hitMaterial != MAT_DIELE)) nW = 2.0f * ngW * dot(ngW, nW) - nW;
-        basis = makeBasis(nW);
+    RT Tr = rt1(1.0f);
+    bool hitSphereLight = false;
+#ifdef HAS_SPHERE_LIGHT
+    {
+      vec3 pLightHit;
+      RT LiSph = (P.sphereLightPower < RADIANCE_EPSILON) ? rt1(0.0f) : pt.sphereLightRadiance(pW, rayDir, rgb, pLightHit);
+      if (averageComponent(LiSph) > RADIANCE_EPSILON) {
+        float lightSegmentLength = length(pLightHit - pW);
+        if (lightSegmentLength < rayLength) {
+          hitSphereLight = true;
+          Tr = pt.transmittanceOverFreeSegment(pW, rayDir, lightSegmentLength, rgb);
+          L += throughput * Tr * misWeightSph * LiSph;
+          if constexpr (SC::kHasAtmosphere) {
+            if (vertex < 2) L += throughput * atmosphericInscatteringRadiance(pW, rayDir, lightSegmentLength, rgb, rnd);
+          }
+        }
       }
-      bool do_subsurface_walk = false;
-      float prob_sss = 0.0f;
-      RT subsurfaceAlbedo = rt1(0.0f);
+    }
+#endif
+    if constexpr (SC::kHasDielectric) {
+      if (inDielectric) throughput *= exp(-rayLength * pt.DIELECTRIC_ABSORPTION_EVAL(pW, rgb));
+    }
+    if constexpr (SC::kHasAtmosphere) {
+      if (vertex < 2) L += throughput * atmosphericInscatteringRadiance(pW, rayDir, rayLength, rgb, rnd);
+      if (!hitSphereLight) Tr = pt.transmittanceOverFreeSegment(pW, rayDir, rayLength, rgb);
+      if (atmosphere_scatters + 1 < SNL_MAX_ATMOSPHERE_SCATTERS) {
+        float scatter_prob = pt.atmosphericScatteringProbability(Tr, rgb);
+        if (scatter_prob > snl_rand(rnd)) {
+          vec3 pW_scatter(0.0f), woutputW(0.0f);
+          throughput *= atmosphericScatterSample(pW, rayDir, rayLength, rgb, rnd, pW_scatter, woutputW) /
+                        max(DENOM_TOLERANCE, scatter_prob);
+          pW = pW_scatter;
+          rayDir = woutputW;
+          atmosphere_scatters++;
+          return advance(ps, maxBounces);
+        } else {
+          throughput /= max(DENOM_TOLERANCE, 1.0f - scatter_prob);
+        }
+      }
+      if (hitSphereLight) return false;
+      throughput *= pt.transmittanceOverFreeSegment(pW, rayDir, rayLength, rgb);
+    }
+    (void)Tr;
+    (void)hitSphereLight;
+
+    // surface interaction
+    pW = pW_next;
+    vec3 nW = pt.normal(pW, hitMaterial, true);
+    ++normals;
+    vec3 ngW = nW;
+    Basis basis = makeBasis(nW);
+    if constexpr (SC::kHasNormalMap) {
+      nW = pt.perturbNormal(pW, basis, hitMaterial);
+      if ((dot(nW, rayDir) > 0.0f) && (hitMaterial != MAT_DIELE)) nW = 2.0f * ngW * dot(ngW, nW) - nW;
+      basis = makeBasis(nW);
+    }
+    bool do_subsurface_walk = false;
+    float prob_sss = 0.0f;
+    RT subsurfaceAlbedo = rt1(0.0f);
+    if constexpr (SC::kHasSurface) {
+      if (P.subsurfaceMFP > 0.0f) {
+        subsurfaceAlbedo = pt.SUBSURFACE_ALBEDO_EVAL(pW, basis.nW, rgb);
+        float diffuse_weight = (1.0f - P.subsurface) * averageComponent(pt.SURFACE_DIFFUSE_REFL_EVAL(pW, basis.nW, -rayDir, rgb));
+        float spec_weight = averageComponent(pt.SURFACE_SPEC_REFL_EVAL(pW, basis.nW, -rayDir, rgb));
+        float subsurface_weight = P.subsurface * averageComponent(subsurfaceAlbedo) * 3.0f;
+        float total_weight = diffuse_weight + spec_weight + subsurface_weight;
+        prob_sss = subsurface_weight / (total_weight + DENOM_TOLERANCE);
+        prob_sss = clamp(prob_sss, 0.0f, 1.0f - PDF_EPSILON);
+        do_subsurface_walk = (snl_rand(rnd) < prob_sss);
+      }
+    }
+    if (!do_subsurface_walk) {
+      vec3 winputW = -rayDir;
+      vec3 winputL = worldToLocal(winputW, basis);
+      vec3 woutputL(0.0f);  // WebGL zero-initialises these locals
+      float bsdfPdf = 0.0f;
+      RT f = pt.sampleBsdf(pW, basis, winputL, hitMaterial, wl, rgb, true, woutputL, bsdfPdf, rnd);
+      vec3 woutputW = localToWorld(woutputL, basis);
+      if constexpr (SC::kHasDielectric) {
+        if (hitMaterial == MAT_DIELE && dot(winputW, ngW) * dot(woutputW, ngW) < 0.0f) inDielectric = !inDielectric;
+      }
+      if constexpr (SC::kHasVolumeEmission) L += throughput * pt.VOLUME_EMISSION_EVAL(pW, rgb);
+      rayDir = woutputW;
+      pW += ngW * sign(dot(rayDir, ngW)) * 3.0f * P.minLengthScale;
+      float skyPdf = 0.0f, sunPdf = 0.0f, sphPdf = 0.0f;
+      if (!inDielectric)
+        L += throughput * directSurfaceLighting(pW, basis, winputW, hitMaterial, wl, rgb, rnd, skyPdf, sunPdf, sphPdf);
+      RT fOverPdf = min(rt1(P.radianceClamp), f / max(PDF_EPSILON, bsdfPdf));
+      RT surface_throughput = fOverPdf * abs(dot(woutputW, nW)) / max(PDF_EPSILON, 1.0f - prob_sss);
+      throughput *= surface_throughput;
+      misWeightSky = powerHeuristic(bsdfPdf, skyPdf);
+      misWeightSun = powerHeuristic(bsdfPdf, sunPdf);
+      misWeightSph = powerHeuristic(bsdfPdf, sphPdf);
+    } else {
       if constexpr (SC::kHasSurface) {
-        if (P.subsurfaceMFP > 0.0f) {
-          subsurfaceAlbedo = pt.SUBSURFACE_ALBEDO_EVAL(pW, basis.nW, rgb);
-          float diffuse_weight = (1.0f - P.subsurface) * averageComponent(pt.SURFACE_DIFFUSE_REFL_EVAL(pW, basis.nW, -rayDir, rgb));
-          float spec_weight = averageComponent(pt.SURFACE_SPEC_REFL_EVAL(pW, basis.nW, -rayDir, rgb));
-          float subsurface_weight = P.subsurface * averageComponent(subsurfaceAlbedo) * 3.0f;
-          float total_weight = diffuse_weight + spec_weight + subsurface_weight;
-          prob_sss = subsurface_weight / (total_weight + DENOM_TOLERANCE);
-          prob_sss = clamp(prob_sss, 0.0f, 1.0f - PDF_EPSILON);
-          do_subsurface_walk = (snl_rand(rnd) < prob_sss);
-        }
-      }
-      if (!do_subsurface_walk) {
-        vec3 winputW = -rayDir;
-        vec3 winputL = worldToLocal(winputW, basis);
-        vec3 woutputL(0.0f);  // WebGL zero-initialises these locals
-        float bsdfPdf = 0.0f;
-        RT f = pt.sampleBsdf(pW, basis, winputL, hitMaterial, wl, rgb, true, woutputL, bsdfPdf, rnd);
-        vec3 woutputW = localToWorld(woutputL, basis);
-        if constexpr (SC::kHasDielectric) {
-          if (hitMaterial == MAT_DIELE && dot(winputW, ngW) * dot(woutputW, ngW) < 0.0f) inDielectric = !inDielectric;
-        }
-        if constexpr (SC::kHasVolumeEmission) L += throughput * pt.VOLUME_EMISSION_EVAL(pW, rgb);
+        RT diffuseAlbedoEntry = pt.SURFACE_DIFFUSE_REFL_EVAL(pW, basis.nW, -rayDir, rgb);
+        (void)diffuseAlbedoEntry;  // weighs the entry lobe, which the walk leaves at 1 (1486-1489)
+        RT walk_throughput = rt1(1.0f);
+        vec3 pExit;
+        basis.nW = ngW;
+        if (!randomwalk_SSS(pW, basis, rnd, P.subsurfaceMFP, subsurfaceAlbedo, walk_throughput, pExit)) return false;
+        pW = pExit;
+        nW = pt.normal(pExit, MAT_SURFA, true);
+        ++normals;
+        Basis basis_exit = makeBasis(nW);
+        float bsdfPdf;
+        vec3 dirExitL = sampleHemisphereCosineWeighted(rnd, bsdfPdf);
+        vec3 woutputW = localToWorld(dirExitL, basis_exit);
         rayDir = woutputW;
-        pW += ngW * sign(dot(rayDir, ngW)) * 3.0f * P.minLengthScale;
+        throughput *= walk_throughput;
         float skyPdf = 0.0f, sunPdf = 0.0f, sphPdf = 0.0f;
+        RT diffuseAlbedoExit = pt.SURFACE_DIFFUSE_REFL_EVAL(pW, nW, -rayDir, rgb);
         if (!inDielectric)
-          L += throughput * directSurfaceLighting(pW, basis, winputW, hitMaterial, wl, rgb, rnd, skyPdf, sunPdf, sphPdf);
-        RT fOverPdf = min(rt1(P.radianceClamp), f / max(PDF_EPSILON, bsdfPdf));
-        RT surface_throughput = fOverPdf * abs(dot(woutputW, nW)) / max(PDF_EPSILON, 1.0f - prob_sss);
-        throughput *= surface_throughput;
+          L += throughput * SSS_exit_radiance(pExit, basis_exit, rgb, rnd, diffuseAlbedoExit, skyPdf, sunPdf, sphPdf);
         misWeightSky = powerHeuristic(bsdfPdf, skyPdf);
         misWeightSun = powerHeuristic(bsdfPdf, sunPdf);
         misWeightSph = powerHeuristic(bsdfPdf, sphPdf);
-      } else {
-        if constexpr (SC::kHasSurface) {
-          RT diffuseAlbedoEntry = pt.SURFACE_DIFFUSE_REFL_EVAL(pW, basis.nW, -rayDir, rgb);
-          (void)diffuseAlbedoEntry;  // weighs the entry lobe, which the walk leaves at 1 (1486-1489)
-          RT walk_throughput = rt1(1.0f);
-          vec3 pExit;
-          basis.nW = ngW;
-          if (!randomwalk_SSS(pW, basis, rnd, P.subsurfaceMFP, subsurfaceAlbedo, walk_throughput, pExit)) break;
-          pW = pExit;
-          nW = pt.normal(pExit, MAT_SURFA, true);
-          ++normals;
-          Basis basis_exit = makeBasis(nW);
-          float bsdfPdf;
-          vec3 dirExitL = sampleHemisphereCosineWeighted(rnd, bsdfPdf);
-          vec3 woutputW = localToWorld(dirExitL, basis_exit);
-          rayDir = woutputW;
-          throughput *= walk_throughput;
-          float skyPdf = 0.0f, sunPdf = 0.0f, sphPdf = 0.0f;
-          RT diffuseAlbedoExit = pt.SURFACE_DIFFUSE_REFL_EVAL(pW, nW, -rayDir, rgb);
-          if (!inDielectric)
-            L += throughput * SSS_exit_radiance(pExit, basis_exit, rgb, rnd, diffuseAlbedoExit, skyPdf, sunPdf, sphPdf);
-          misWeightSky = powerHeuristic(bsdfPdf, skyPdf);
-          misWeightSun = powerHeuristic(bsdfPdf, sunPdf);
-          misWeightSph = powerHeuristic(bsdfPdf, sphPdf);
-          RT f = mix(rt1(1.0f), diffuseAlbedoExit, P.subsurfaceDiffuseWeight) / float(M_PI);
-          RT fOverPdf = min(rt1(P.radianceClamp), f / max(PDF_EPSILON, bsdfPdf));
-          RT surface_exit_throughput = fOverPdf * abs(dot(woutputW, nW));
-          throughput *= surface_exit_throughput / max(PDF_EPSILON, prob_sss);
-          pW += nW * sign(dot(rayDir, nW)) * 3.0f * P.minLengthScale;
-        }
+        RT f = mix(rt1(1.0f), diffuseAlbedoExit, P.subsurfaceDiffuseWeight) / float(M_PI);
+        RT fOverPdf = min(rt1(P.radianceClamp), f / max(PDF_EPSILON, bsdfPdf));
+        RT surface_exit_throughput = fOverPdf * abs(dot(woutputW, nW));
+        throughput *= surface_exit_throughput / max(PDF_EPSILON, prob_sss);
+        pW += nW * sign(dot(rayDir, nW)) * 3.0f * P.minLengthScale;
       }
-      if (maxComponent(throughput) < THROUGHPUT_EPSILON) break;
     }
-    return L;
+    if (maxComponent(throughput) < THROUGHPUT_EPSILON) return false;
+    return advance(ps, maxBounces);
   }
 };
 
@@ -463,58 +498,116 @@ __device__ inline void mega_body(const SnlKArgs& a) {
   SNL_SCENE_SETUP
   Mega<SC> mg(pt, P);
   const int lane = snl_lane();
+  const unsigned long long below = (1ull << lane) - 1ull;
   const int nItems = a.itemEnd - a.itemBase;
+  int* fetch = &a.cnt[C_FETCH];
+  int rb = 0, re = 0;
+  bool exhausted = false;
+  // the lane's pixel: its frames run back to back, RNG and running mean in registers
+  bool havePix = false, frameOpen = false, inPath = false;
+  int pix = 0, frame = 0, sample = 0;
+  vec4 rnd(1.0f), accum(0.0f);
+  vec2 pixel(0.0f);
+  // the frame being rendered (pathtrace, 1934-1989)
+  float wl = 0.0f;
+  vec3 rgb(0.0f), xyz(0.0f);
+  RT Lframe = rt1(0.0f);
+  typename Mega<SC>::Path ps;
   long long normals = 0;
   while (true) {
-    int base = 0;
-    if (lane == 0) base = atomicAdd(&a.cnt[C_FETCH], 64);
-    base = __shfl(base, 0);
-    if (base >= nItems) break;
-    int item = base + lane;
-    if (item >= nItems) continue;
-    int pix = a.itemBase + item;
-    vec4 rnd = ld4(a.rngstate, pix);
-    vec4 accum = ld4(a.radiance, pix);
-    vec2 pixel(float(pix % a.W) + 0.5f, float((pix % a.imagePix) / a.W) + 0.5f);
-    for (int frame = 0; frame < a.frames; ++frame) {
+    // ---- lanes without a pixel take the next ones of the range
+    unsigned long long need = __ballot(!havePix);
+    if (need != 0ull && rb >= re && !exhausted) {
+      int base = 0;
+      if (lane == 0) base = atomicAdd(fetch, a.chunk);
+      base = __shfl(base, 0);
+      if (base >= nItems) exhausted = true;
+      else { rb = base; re = min(base + a.chunk, nItems); }
+    }
+    if (need != 0ull && rb < re) {
+      int avail = re - rb;
+      int rank = __popcll(need & below);
+      if (!havePix && rank < avail) {
+        pix = snl_item_pixel(a, a.itemBase + rb + rank);
+        havePix = true;
+        frame = 0;
+        sample = 0;
+        frameOpen = false;
+        rnd = ld4(a.rngstate, pix);
+        accum = ld4(a.radiance, pix);
+        pixel = vec2(float(pix % a.W) + 0.5f, float((pix % a.imagePix) / a.W) + 0.5f);
+      }
+      rb += min(avail, (int)__popcll(need));
+    }
+    if (__ballot(havePix) == 0ull) {
+      if (exhausted) break;
+      continue;
+    }
+    // ---- a lane whose path ended starts the next sample, opening frames and
+    // handing back its pixel after the last one
+    if (havePix && !inPath) {
+      while (!frameOpen) {
+        if (frame >= a.frames) {
+          st4(a.radiance, pix, accum);
+          st4(a.rngstate, pix, rnd);
+          havePix = false;
+          break;
+        }
 #ifdef INTERACTIVE_MODE
-      if (snl_rand(rnd) < P.skipProbability) continue;
+        // main (1991-2007): a skipped frame leaves the pixel as it is
+        if (snl_rand(rnd) < P.skipProbability) { ++frame; continue; }
 #endif
-      // pathtrace (1934-1989)
 #ifdef DISPERSION_ENABLED
-      float xi = 0.1f + 0.8f * (0.5f + floor(float(P.wavelengthSamples) * snl_rand(rnd))) / float(P.wavelengthSamples);
-      float w = texture(P.ICDF, vec2(xi, 0.5f)).r;
-      float wl = 390.0f + (750.0f - 390.0f) * w;
-      vec3 xyz = vec3(texture(P.WavelengthToXYZ, vec2(w, 0.5f)));
-      vec3 rgb = xyzToRgb(xyz);
+        float xi = 0.1f + 0.8f * (0.5f + floor(float(P.wavelengthSamples) * snl_rand(rnd))) / float(P.wavelengthSamples);
+        float w = texture(P.ICDF, vec2(xi, 0.5f)).r;
+        wl = 390.0f + (750.0f - 390.0f) * w;
+        xyz = vec3(texture(P.WavelengthToXYZ, vec2(w, 0.5f)));
+        rgb = xyzToRgb(xyz);
 #else
-      float wl = 390.0f + (750.0f - 390.0f) * 0.5f;
-      vec3 rgb(0.0f);
+        wl = 390.0f + (750.0f - 390.0f) * 0.5f;
+        rgb = vec3(0.0f);
 #endif
-      RT L = rt1(0.0f);
-      for (int n = 0; n < a.spf; ++n) {
+        Lframe = rt1(0.0f);
+        sample = 0;
+        frameOpen = true;
+      }
+      if (havePix) {
         float jx = 0.5f * P.filterRadius * sample_jitter(snl_rand(rnd));
         float jy = 0.5f * P.filterRadius * sample_jitter(snl_rand(rnd));
         vec2 pixelj = pixel + vec2(jx, jy);
         vec3 primaryStart, primaryDir;
         if constexpr (SC::kHasCustomCamera) S.CONSTRUCT_PRIMARY_RAY(pixelj, rnd, primaryStart, primaryDir);
         else pt.constructPrimaryRay(pixelj, rnd, primaryStart, primaryDir, true);
-        L += mg.cameraPath(primaryStart, primaryDir, wl, rgb, rnd, a.maxBounces, normals);
+        mg.startPath(ps, primaryStart, primaryDir);
+        inPath = true;
       }
-      L /= float(a.spf);
-#ifdef DISPERSION_ENABLED
-      vec3 colorXYZ = xyz * L;
-#else
-      vec3 colorXYZ = rgbToXyz(L);
-#endif
-      float oldN = accum.w, newN = oldN + 1.0f;
-      accum = vec4((oldN * vec3(accum) + colorXYZ) / newN, newN);
     }
-    st4(a.radiance, pix, accum);
-    st4(a.rngstate, pix, rnd);
+    // ---- one vertex of every running path
+    if (inPath) {
+      inPath = mg.vertexStep(ps, wl, rgb, rnd, a.maxBounces, normals);
+      if (!inPath) {
+        Lframe += ps.L;
+        if (++sample == a.spf) {
+          RT L = Lframe / float(a.spf);
+#ifdef DISPERSION_ENABLED
+          vec3 colorXYZ = xyz * L;
+#else
+          vec3 colorXYZ = rgbToXyz(L);
+#endif
+          float oldN = accum.w, newN = oldN + 1.0f;
+          accum = vec4((oldN * vec3(accum) + colorXYZ) / newN, newN);
+          frameOpen = false;
+          ++frame;
+        }
+      }
+    }
   }
-  unsigned long long ntot = normals;
-  for (int off = 32; off > 0; off >>= 1) ntot += __shfl_xor(ntot, off);
+  unsigned long long tot = pt.steps, ntot = normals;
+  for (int off = 32; off > 0; off >>= 1) {
+    tot += __shfl_xor(tot, off);
+    ntot += __shfl_xor(ntot, off);
+  }
+  if (lane == 0 && tot) atomicAdd(reinterpret_cast<unsigned long long*>(&a.cnt[C_TRACE_STEPS]), tot);
   if (lane == 0 && ntot) atomicAdd(reinterpret_cast<unsigned long long*>(&a.cnt[C_NORMALS]), ntot);
 }
 

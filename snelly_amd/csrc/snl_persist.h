@@ -96,7 +96,7 @@ __device__ inline void persist_body(const SnlKArgs& a) {
       int avail = re - rb;
       int rank = __popcll(idleMask & below);
       if (st == PS_IDLE && rank < avail) {
-        pix = a.itemBase + rb + rank;
+        pix = snl_item_pixel(a, a.itemBase + rb + rank);
         sample = 0;
         frame = 0;
         st = PS_CAMERA;
@@ -254,6 +254,8 @@ __device__ inline void persist_body(const SnlKArgs& a) {
           rayDir = woutputW;
           pW += ngW * sign(dot(rayDir, ngW)) * 3.0f * P.minLengthScale;
           float skyPdf = 0.0f, sunPdf = 0.0f, sphPdf = 0.0f;
+          // a light whose BSDF value is exactly zero adds exactly zero whatever
+          // its shadow ray finds: that ray is not traced
           neeMask = 0;
           if (!inDiele) {
             vec3 wL, wW;
@@ -264,7 +266,7 @@ __device__ inline void persist_body(const SnlKArgs& a) {
                 nf0 = pt.evaluateBsdf(pW, basis, winputL, wL, hitMaterial, wl, rgb, true);
                 nl0 = Li; nd0 = wW; nt0 = P.maxLengthScale;
                 nden0 = max(PDF_EPSILON, skyPdf); ncos0 = abs(dot(wW, basis.nW)); nmis0 = powerHeuristic(skyPdf, lp);
-                neeMask |= 1;
+                if (anyNonzero(nf0)) neeMask |= 1;
               }
             }
             if (P.sunPower > RADIANCE_EPSILON) {
@@ -274,7 +276,7 @@ __device__ inline void persist_body(const SnlKArgs& a) {
                 nf1 = pt.evaluateBsdf(pW, basis, winputL, wL, hitMaterial, wl, rgb, true);
                 nl1 = Li; nd1 = wW; nt1 = P.maxLengthScale;
                 nden1 = max(PDF_EPSILON, sunPdf); ncos1 = abs(dot(wW, basis.nW)); nmis1 = powerHeuristic(sunPdf, lp);
-                neeMask |= 2;
+                if (anyNonzero(nf1)) neeMask |= 2;
               }
             }
 #ifdef HAS_SPHERE_LIGHT
@@ -288,7 +290,7 @@ __device__ inline void persist_body(const SnlKArgs& a) {
                 nf2 = pt.evaluateBsdf(pW, basis, winputL, wL, hitMaterial, wl, rgb, true);
                 nl2 = Li; nd2 = wW; nt2 = max(0.0f, length(pHit - pW) - 3.0f * P.minLengthScale);
                 nden2 = max(PDF_EPSILON, sphPdf); ncos2 = abs(dot(wW, basis.nW)); nmis2 = powerHeuristic(sphPdf, lp);
-                neeMask |= 4;
+                if (anyNonzero(nf2)) neeMask |= 4;
               }
             }
 #endif

@@ -40,6 +40,9 @@ SNL_HD float maxComponent(const vec3& v) { return max(max(v.r, v.g), v.b); }
 SNL_HD float maxComponent(float f) { return f; }
 SNL_HD float averageComponent(const vec3& v) { return (v.r + v.g + v.b) / 3.0f; }
 SNL_HD float averageComponent(float f) { return f; }
+// whether a radiance value has a nonzero component (NaN counts as nonzero)
+SNL_HD bool anyNonzero(const vec3& v) { return v.r != 0.0f || v.g != 0.0f || v.b != 0.0f; }
+SNL_HD bool anyNonzero(float f) { return f != 0.0f; }
 
 struct Basis { vec3 nW, tW, bW; };
 
@@ -217,6 +220,8 @@ struct PT {
   SC& S;
   const SnlParams& P;
   SNL_HD PT(SC& s, const SnlParams& p) : S(s), P(p) {}
+  // SDF evaluations of traceRay, for the step statistics
+  unsigned long long steps = 0;
 
   // ---------------- SDF marching (traceRay, 881-920)
   // Unified single-step form used by the persistent tracer: evaluates the
@@ -248,12 +253,14 @@ struct PT {
     material = MAT_INVAL;
     float sdf;
     sdfStep(start, false, sdf);
+    ++steps;
     float InitialSign = sign(sdf);
     float t = InitialSign * sdf;
     if (abs(t) >= maxMarchDist) return false;
     for (int n = 0; n < SNL_MAX_MARCH_STEPS; n++) {
       vec3 pW = start + t * dir;
       int m = sdfStep(pW, true, sdf);
+      ++steps;
       if (m != MAT_INVAL) { material = m; hit = start + t * dir; return true; }
       t += InitialSign * sdf;
       if (abs(t) >= maxMarchDist) return false;

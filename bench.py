@@ -171,7 +171,8 @@ def main():
     sess.render(spp)
     mk = sess.prog.stats()
     megakernel = {"path_samples_per_s": round(W * H * spp / (mk.ms_total * 1e-3), 1), "ms": round(mk.ms_total, 2)}
-    # the persistent engine with an autotuned launch shape (untimed)
+    # the pixel-queue engine (persistent state machine or megakernel) and its
+    # launch shape, autotuned untimed
     best, _, _ = autotune.tune(sess, frames=8)
     for w in range(args.warmup):
         sess.state.seed(1000 + w)
@@ -203,7 +204,7 @@ def main():
     samples = W * H * spp * args.steps
     value = samples / elapsed
 
-    # snl_persist is the whole render; its launches on the tile streams overlap,
+    # the queue kernel is the whole render; its launches on the tile streams overlap,
     # so the rate is total algorithmic FLOPs over the GPU time of the renders.
     # Counted: every SDF march step (SDF + 8 for the step itself) and every
     # central-difference normal (6 SDF evaluations); shading is not counted.
@@ -214,7 +215,7 @@ def main():
                 "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                 "traffic": round(PERSIST_TRAFFIC_B_PER_PX * pixels_per_launch),
                 "traffic_algorithmic": 64 * pixels_per_launch,
-                "kernel": "snl_persist", "launches": launches, "gpu_ms": round(trace_ms, 2),
+                "kernel": autotune.KERNEL_NAMES[best["engine"]], "launches": launches, "gpu_ms": round(trace_ms, 2),
                 "flops_per_launch": flops / launches, "flops_per_step": scene.flops_per_step,
                 "steps_per_launch": trace_steps / launches, "normal_evals_per_launch": normal_evals / launches,
                 "sdf_flops": scene.sdf_flops,
@@ -229,7 +230,8 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (analytic SDF scene, PCG-seeded per-pixel RNG)",
         "config": {"workload": "menger4_metal_spectral", "width": W, "height": H, "spp": spp, "bounces": 8,
-                   "mode": "plain", "engine": "persistent", "build": "fast", "launch_shape": best,
+                   "mode": "plain", "engine": autotune.ENGINE_NAMES[best["engine"]], "build": "fast",
+                   "launch_shape": {k: v for k, v in best.items() if k != "engine"},
                    "path_samples_per_step": W * H * spp},
         "engines": {"wavefront": wavefront, "megakernel": megakernel},
         "roofline": roofline,

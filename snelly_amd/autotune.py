@@ -1,6 +1,8 @@
-"""Autotuner for the persistent engine's launch shape.
+"""Autotuner for the engine and launch shape of pixel-queue renders.
 
-Knobs: pixel tiles pipelined on separate streams, resident blocks per launch
+Knobs: the engine (the persistent state machine or the megakernel, which
+take pixels from the same queues), pixel tiles pipelined on separate streams,
+resident blocks per launch
 (waves per CU = 4 x blocks / CUs), the shading threshold (lanes of a wave that
 must wait before the wave runs the shading code), the number of pixels a
 wave takes from its tile's queue per atomic and the march steps a lane takes
@@ -26,16 +28,17 @@ def time_render(sess, frames):
 def tune(sess, frames=4, cu=256, verbose=False):
     prog = sess.prog
     space = {
+        "engine": [native.ENGINE_PERSISTENT, native.ENGINE_MEGAKERNEL],
         "tiles": [1, 2, 4],
         "blocks": [cu * 2, cu * 3, cu * 4, cu * 6, cu * 8],
         "thresh": [8, 16, 24, 32, 48],
         "chunk": [32, 64, 128, 256],
         "burst": [1, 2, 4, 8, 16],
     }
-    best = {"tiles": 1, "blocks": cu * 6, "thresh": 32, "chunk": 32, "burst": 4}
+    best = {"engine": native.ENGINE_PERSISTENT, "tiles": 1, "blocks": cu * 6, "thresh": 32, "chunk": 32, "burst": 4}
 
     def run(cfg):
-        prog.engine(native.ENGINE_PERSISTENT, cfg["tiles"], cfg["blocks"], cfg["thresh"], cfg["chunk"])
+        prog.engine(cfg["engine"], cfg["tiles"], cfg["blocks"], cfg["thresh"], cfg["chunk"])
         prog.march_burst(cfg["burst"])
         return time_render(sess, frames)
 
@@ -54,6 +57,11 @@ def tune(sess, frames=4, cu=256, verbose=False):
                     print("autotune", cfg, "%.2f ms" % (t * 1e3), flush=True)
                 if t < best_t:
                     best, best_t = cfg, t
-    prog.engine(native.ENGINE_PERSISTENT, best["tiles"], best["blocks"], best["thresh"], best["chunk"])
+    prog.engine(best["engine"], best["tiles"], best["blocks"], best["thresh"], best["chunk"])
     prog.march_burst(best["burst"])
     return best, best_t, trials
+
+
+ENGINE_NAMES = {native.ENGINE_WAVEFRONT: "wavefront", native.ENGINE_PERSISTENT: "persistent",
+                native.ENGINE_MEGAKERNEL: "megakernel"}
+KERNEL_NAMES = {native.ENGINE_PERSISTENT: "snl_persist", native.ENGINE_MEGAKERNEL: "snl_mega"}

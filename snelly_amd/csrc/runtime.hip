@@ -850,8 +850,15 @@ static int renderPersistent(snl_prog p, snl_state st, int frames, bool mega) {
     HIPCHK(ctx, hipStreamWaitEvent(ts, ctx->joinEvent, 0));
     SnlKArgs a = base;
     a.cnt = st->tileCnt + k * C_COUNT;
-    a.itemBase = (int)(st->npix * k / K);
-    a.itemEnd = (int)(st->npix * (k + 1) / K);
+    // tile k: the 64-pixel stripes s with s % K == k (the last one may be short)
+    long long stripes = (st->npix + 63) / 64, last = stripes - 1;
+    long long mine = stripes / K + (k < stripes % K ? 1 : 0);
+    long long items = mine * 64;
+    if (mine > 0 && last % K == k) items -= stripes * 64 - st->npix;
+    a.itemBase = 0;
+    a.itemEnd = (int)items;
+    a.tileIndex = k;
+    a.tileCount = K;
     void* args[] = {&a};
     if (unit0) {
       // one launch per frame, each after a snapshot of the radiance
@@ -1056,7 +1063,7 @@ int snl_set_engine(snl_prog p, int engine, int tiles, int persist_blocks, int sh
     return fail(p->ctx, SNL_E_ARG, "bad engine");
   p->engine = engine;
   if (tiles > 0) p->tiles = std::min(tiles, SNL_MAX_TILES);
-  if (persist_blocks > 0) p->persistBlocks = persist_blocks;
+  if (persist_blocks > 0) (engine == SNL_ENGINE_MEGAKERNEL ? p->megaBlocks : p->persistBlocks) = persist_blocks;
   if (shade_thresh > 0) p->shadeThresh = std::min(shade_thresh, 64);
   if (chunk > 0) p->persistChunk = chunk;
   return SNL_OK;

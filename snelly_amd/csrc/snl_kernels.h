@@ -18,6 +18,14 @@ SNL_HD float sample_jitter(float xi) {  // triangle filter (1928-1932)
   return xi < 0.5f ? sqrt(2.0f * xi) - 1.0f : 1.0f - sqrt(2.0f - 2.0f * xi);
 }
 
+// The queue item of entry j of a launch's range: with K > 1 tiles launched
+// side by side, tile k takes the 64-item stripes k, k + K, k + 2K, ..., so the
+// tiles hold alike mixes of cheap and expensive pixels.
+__device__ inline int snl_tile_item(const SnlKArgs& a, int j) {
+  if (a.tileCount <= 1) return a.itemBase + j;
+  return ((j >> 6) * a.tileCount + a.tileIndex) * 64 + (j & 63);
+}
+
 // The pixel a queue item stands for.  Where the image divides into 8x8 tiles,
 // 64 consecutive items cover one tile, so a wave's lanes start neighbouring
 // rays; otherwise items are pixels in row order.  Chains of a multi-chain

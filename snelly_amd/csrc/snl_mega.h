@@ -528,7 +528,7 @@ __device__ inline void mega_body(const SnlKArgs& a) {
       int avail = re - rb;
       int rank = __popcll(need & below);
       if (!havePix && rank < avail) {
-        pix = snl_item_pixel(a, a.itemBase + rb + rank);
+        pix = snl_item_pixel(a, snl_tile_item(a, rb + rank));
         havePix = true;
         frame = 0;
         sample = 0;

@@ -96,7 +96,7 @@ __device__ inline void persist_body(const SnlKArgs& a) {
       int avail = re - rb;
       int rank = __popcll(idleMask & below);
       if (st == PS_IDLE && rank < avail) {
-        pix = snl_item_pixel(a, a.itemBase + rb + rank);
+        pix = snl_item_pixel(a, snl_tile_item(a, rb + rank));
         sample = 0;
         frame = 0;
         st = PS_CAMERA;

@@ -47,7 +47,8 @@ struct SnlKArgs {
   int W, H, npix;
   int vertex, sample, spf;
   int chunk;          // trace refill chunk per wave
-  int itemBase, itemEnd;  // persistent kernel: pixel range of this tile
+  int itemBase, itemEnd;  // persistent kernel: queue range of this tile
+  int tileIndex, tileCount;  // persistent kernel: tile k of K takes every K-th 64-item stripe
   int shadeThresh;        // persistent kernel: lanes waiting before a wave shades
   int frames;             // persistent kernel: frames each pixel runs before the lane moves on
   int maxBounces;         // __MAX_BOUNCES__: the last vertex index a path may reach

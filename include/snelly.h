@@ -37,6 +37,13 @@ extern "C" {
 /* program build flags */
 #define SNL_BUILD_PRECISE 1  /* no FMA contraction (-ffp-contract=off) */
 #define SNL_BUILD_FAST 2     /* approximate (1-ulp) float divide and sqrt outside the RNG */
+/* Renderer.renderMode (js/renderer.js:334-352): the program renders with the
+ * ambient-occlusion integrator (shaders/ao-fragment-shader.glsl) or the
+ * simplified path tracer (shaders/simplepathtracer-fragment-shader.glsl)
+ * instead of the path tracer; both are RGB integrators, DISPERSION_ENABLED
+ * does not apply.  At most one of the two. */
+#define SNL_BUILD_AO 4
+#define SNL_BUILD_PTSIMPLE 8
 
 typedef struct snl_ctx_s* snl_ctx;
 typedef struct snl_prog_s* snl_prog;

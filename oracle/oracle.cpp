@@ -10,6 +10,9 @@
 //     atmosphere 1299-1457, randomwalk_SSS 1468-1593,
 //     constructPrimaryRay 1602-1622, cameraPath 1625-1926, pathtrace 1934-1989)
 //   shaders/normals-fragment-shader.glsl     (traceDistance 70-113, main 293-345)
+//   shaders/ao-fragment-shader.glsl          (Transmittance 154-159, main 395-488)
+//   shaders/simplepathtracer-fragment-shader.glsl (traceRay 508-534, fog 570-604,
+//     directSurfaceLighting 719-770, SSS 783-882, cameraPath 913-1083, main 1091-1129)
 //   shaders/pick-fragment-shader.glsl        (main 133-152)
 //   shaders/tonemapper-fragment-shader.glsl  (main 36-97)
 // Scenes are hand-written C++ SDFs (no GLSL translation), so the oracle is
@@ -106,6 +109,7 @@ inline V3 A3(const float* a) { return v3(a[0], a[1], a[2]); }
 struct Config {
   int dispersive = 0, interactive = 0, maxBounces = 3, maxMarch = 256, spf = 1, sphereLight = 0;
   int atmosphere = 0, fog = 0, maxAtmScatters = 1;  // HAS_ATMOSPHERE, HAS_FOG, __MAX_ATMOSPHERE_SCATTERS__
+  int mode = 0;  // Renderer.renderMode: 0 'pt', 1 'ao', 2 'ptsimple' 
 } C;
 
 // ---------------------------------------------------------------- scene (hand-written SDFs)
@@ -1252,6 +1256,307 @@ void pathtracePixel(int ix, int iy, float* rad, float* rnd) {
   rad[0] = newL.x; rad[1] = newL.y; rad[2] = newL.z; rad[3] = newN;
 }
 
+// traceDistance (normals-fragment-shader.glsl:70-113, ao-fragment-shader.glsl:100-144):
+// t starts at 0 and the step precedes the evaluation
+bool traceDistance(V3 s, V3 d, float maxDist, V3& hit, int& material) {
+  float sdf = 1.0e20f;
+  for (int m : {MAT_SURFA, MAT_METAL, MAT_DIELE}) if (hasMat(m)) sdf = fmin_(sdf, std::fabs(evalSdf(sdfs[m], s)));
+  float InitialSign = fsign(sdf), t = 0.0f;
+  int iters = 0;
+  material = MAT_INVAL;
+  for (int k = 0; k < C.maxMarch; k++) {
+    t += InitialSign * sdf;
+    if (t >= maxDist) break;
+    V3 pW = s + t * d;
+    sdf = 1.0e20f;
+    bool h = false;
+    for (int m : {MAT_SURFA, MAT_METAL, MAT_DIELE}) {
+      if (!hasMat(m)) continue;
+      float v = std::fabs(evalSdf(sdfs[m], pW));
+      if (v < P.minLengthScale) { material = m; h = true; break; }
+      sdf = fmin_(sdf, v);
+    }
+    if (h) break;
+    iters++;
+  }
+  hit = s + t * d;
+  return !(t >= maxDist) && !(P.maxStepsIsMiss != 0.0f && iters >= C.maxMarch);
+}
+
+inline void accumulate(float* rad, V3 c) {
+  float oldN = rad[3], newN = oldN + 1.0f;
+  V3 newL = (oldN * v3(rad[0], rad[1], rad[2]) + c) / newN;
+  rad[0] = newL.x; rad[1] = newL.y; rad[2] = newL.z; rad[3] = newN;
+}
+
+// ambient occlusion main (ao-fragment-shader.glsl:395-488)
+void aoPixel(int ix, int iy, float* rad, float* rnd) {
+  const float RAD_EPS = 1.0e-5f;  // this program's RADIANCE_EPSILON (62)
+  if (C.interactive && randf(rnd) < P.skipProbability) return;
+  Tracer<V3> T;
+  T.wl = 570.0f;
+  T.rgb = v3(0.0f);
+  V3 RGB = v3(0.0f);
+  for (int n = 0; n < C.spf; ++n) {
+    float r1 = randf(rnd);
+    float r2 = randf(rnd);
+    float px = ix + 0.5f + (-0.5f + r1), py = iy + 0.5f + (-0.5f + r2);
+    V3 s, d;
+    constructPrimaryRay(px, py, rnd, s, d, false);
+    V3 pW;
+    int m;
+    if (traceDistance(s, d, P.maxLengthScale, pW, m)) {
+      if (m == MAT_INVAL) continue;  // max-step "hit": no normal defined; the sample adds nothing
+      V3 nW = T.normal(pW, m, false);
+      Basis b = makeBasis(nW, false);
+      V3 f = surfaceDiffuseHook(A3(P.surfaceDiffuseAlbedoRGB), pW) / PI_F;
+      V3 Ld = v3(0.0f);
+      V3 dPw = 3.0f * P.minLengthScale * nW;
+      V3 h = v3(0.0f);
+      int hm;
+      if (P.skyPower > RAD_EPS) {
+        float skyPdf = 0.0f;
+        V3 wL = sampleHemisphereCosineWeighted(rnd, skyPdf);
+        V3 wW = localToWorld(wL, b);
+        V3 Li = T.environmentRadiance(wW);
+        Li = Li * (traceDistance(pW + dPw, wW, P.maxLengthScale, h, hm) ? 0.0f : 1.0f);
+        Ld = Ld + f * Li / fmax_(PDF_EPSILON, skyPdf) * std::fabs(dot(wW, nW));
+      }
+      if (P.sunPower > RAD_EPS) {
+        float theta_max = P.sunAngularSize * PI_F / 180.0f;
+        float theta = theta_max * std::sqrt(randf(rnd));
+        float ct = std::cos(theta), st = std::sqrt(fmax_(0.0f, 1.0f - ct * ct));
+        float phi = 2.0f * PI_F * randf(rnd);
+        float sunPdf = 1.0f / (2.0f * PI_F * (1.0f - std::cos(theta_max)));
+        V3 wW = localToWorld(v3(st * std::cos(phi), st * std::sin(phi), ct), makeBasis(A3(P.sunDir), false));
+        V3 wL = worldToLocal(wW, b);
+        V3 Li = wL.z < 0.0f ? v3(0.0f) : T.sunRadiance(wW);
+        if (avgC(Li) > RAD_EPS) {
+          Li = Li * (traceDistance(pW + dPw, wW, P.maxLengthScale, h, hm) ? 0.0f : 1.0f);
+          Ld = Ld + f * Li / fmax_(PDF_EPSILON, sunPdf) * std::fabs(dot(wW, nW));
+        }
+      }
+      RGB = RGB + Ld;
+    } else if (P.envMapVisible != 0.0f) {
+      RGB = RGB + T.environmentRadiance(d);
+    }
+  }
+  accumulate(rad, rgbToXyz(RGB / float(C.spf)));
+}
+
+// the simplified path tracer (simplepathtracer-fragment-shader.glsl): RGB,
+// the Surface material only, fog only, no sphere light, no shadowStrength
+struct Simple : Tracer<V3> {
+  bool traceSurface(V3 start, V3 dir, V3& hit, int& material, float maxMarchDist) const {  // 508-534
+    material = MAT_INVAL;
+    if (!hasMat(MAT_SURFA)) return false;
+    float sdf = fmin_(1.0e20f, std::fabs(evalSdf(sdfs[MAT_SURFA], start)));
+    float InitialSign = fsign(sdf);
+    float t = InitialSign * sdf;
+    if (std::fabs(t) >= maxMarchDist) return false;
+    for (int n = 0; n < C.maxMarch; n++) {
+      V3 pW = start + t * dir;
+      float v = std::fabs(evalSdf(sdfs[MAT_SURFA], pW));
+      if (v < P.minLengthScale) { material = MAT_SURFA; hit = start + t * dir; return true; }
+      sdf = fmin_(1.0e20f, v);
+      t += InitialSign * sdf;
+      if (std::fabs(t) >= maxMarchDist) return false;
+    }
+    return P.maxStepsIsMiss == 0.0f;
+  }
+  V3 trFree(float len) const {  // 570-582
+    V3 Tr = v3(1.0f);
+    if (C.fog && P.fogEnable != 0.0f) {
+      V3 fogAbsorption = v3(1.0f) - A3(P.fogTint);
+      Tr = Tr * vexp(-(P.fogMaxOpticalDepth * (v3(1.0f) - vexp(-(len * fogAbsorption / P.fogMFP)))));
+    }
+    return Tr;
+  }
+  V3 trSeg(V3 pW, V3 dir, float len) const {  // 584-597
+    V3 h = v3(0.0f);
+    int m;
+    if (traceSurface(pW, dir, h, m, len)) return v3(0.0f);
+    return C.fog ? trFree(len) : v3(1.0f);
+  }
+  V3 inscatter(V3 Tr) const { return P.fogEnable == 0.0f ? v3(0.0f) : (v3(1.0f) - Tr) * A3(P.fogEmission); }
+  V3 direct(V3 pW, const Basis& b, V3 winW, float* rnd, float& skyPdf, float& sunPdf) const {  // 719-770
+    V3 wi = worldToLocal(winW, b);
+    V3 Ld = v3(0.0f);
+    if (P.skyPower > RADIANCE_EPSILON) {
+      V3 wL = sampleHemisphereCosineWeighted(rnd, skyPdf);
+      V3 wW = localToWorld(wL, b);
+      V3 Li = environmentRadiance(wW);
+      if (avgC(Li) > RADIANCE_EPSILON) {
+        Li = Li * trSeg(pW, wW, P.maxLengthScale);
+        if (avgC(Li) > RADIANCE_EPSILON) {
+          float bp = fmax_(PDF_EPSILON, pdfBsdf(pW, b, wi, wL, MAT_SURFA));
+          V3 f = evaluateBsdf(pW, b, wi, wL, MAT_SURFA);
+          Ld = Ld + f * Li / fmax_(PDF_EPSILON, skyPdf) * std::fabs(dot(wW, b.nW)) * powerHeuristic(skyPdf, bp);
+        }
+      }
+    }
+    if (P.sunPower > RADIANCE_EPSILON) {
+      V3 wW = sampleSunDir(rnd, sunPdf);
+      V3 wL = worldToLocal(wW, b);
+      V3 Li = wL.z < 0.0f ? v3(0.0f) : sunRadiance(wW);
+      if (avgC(Li) > RADIANCE_EPSILON) {
+        Li = Li * trSeg(pW, wW, P.maxLengthScale);
+        if (avgC(Li) > RADIANCE_EPSILON) {
+          float bp = fmax_(PDF_EPSILON, pdfBsdf(pW, b, wi, wL, MAT_SURFA));
+          V3 f = evaluateBsdf(pW, b, wi, wL, MAT_SURFA);
+          Ld = Ld + f * Li / fmax_(PDF_EPSILON, sunPdf) * std::fabs(dot(wW, b.nW)) * powerHeuristic(sunPdf, bp);
+        }
+      }
+    }
+    return vmin(v3(P.radianceClamp), Ld);
+  }
+  bool walk(V3 pW, const Basis& b, float* rnd, V3 albedoSSS, V3& walkT, V3& pExit) const {  // 783-834
+    V3 pWalk = pW - 3.0f * P.minLengthScale * b.nW;
+    float pdfDir;
+    V3 dW = -localToWorld(sampleHemisphereCosineWeighted(rnd, pdfDir), b);
+    walkT = v3(1.0f);
+    for (int n = 0; n < 256; ++n) {
+      if ((float)n > P.maxSSSSteps) break;
+      float step = -std::log(randf(rnd)) * P.subsurfaceMFP * P.lengthScale;
+      V3 h = v3(0.0f);
+      int m;
+      if (traceSurface(pWalk, dW, h, m, step)) { pExit = h; return true; }
+      if (n > 4) {
+        float cont = fclamp(10.0f * maxC(walkT), 0.0f, 1.0f);
+        if (randf(rnd) < 1.0f - cont) break;
+        walkT = walkT / cont;
+      }
+      dW = samplePhaseFunction(dW, P.subsurfaceAnisotropy, rnd);
+      pWalk = pWalk + step * dW;
+      walkT = walkT * albedoSSS;
+    }
+    return false;
+  }
+  V3 exitRad(V3 pW, const Basis& b, float* rnd, V3 albedoExit, float& skyPdf, float& sunPdf) const {  // 837-882
+    V3 Ld = v3(0.0f);
+    V3 f = mixR(v3(1.0f), albedoExit, P.subsurfaceDiffuseWeight) / PI_F;
+    if (P.skyPower > RADIANCE_EPSILON) {
+      V3 wL = sampleHemisphereCosineWeighted(rnd, skyPdf);
+      V3 wW = localToWorld(wL, b);
+      V3 Li = environmentRadiance(wW) * trSeg(pW, wW, P.maxLengthScale);
+      if (avgC(Li) > RADIANCE_EPSILON)
+        Ld = Ld + f * Li / fmax_(PDF_EPSILON, skyPdf) * std::fabs(dot(wW, b.nW)) *
+                      powerHeuristic(skyPdf, pdfHemisphereCosineWeighted(wL));
+    }
+    if (P.sunPower > RADIANCE_EPSILON) {
+      V3 wW = sampleSunDir(rnd, sunPdf);
+      V3 wL = worldToLocal(wW, b);
+      V3 Li = (wL.z < 0.0f ? v3(0.0f) : sunRadiance(wW)) * trSeg(pW, wW, P.maxLengthScale);
+      if (avgC(Li) > RADIANCE_EPSILON)
+        Ld = Ld + f * Li / fmax_(PDF_EPSILON, sunPdf) * std::fabs(dot(wW, b.nW)) *
+                      powerHeuristic(sunPdf, pdfHemisphereCosineWeighted(wL));
+    }
+    return vmin(v3(P.radianceClamp), Ld);
+  }
+  V3 path(V3 start, V3 dir, float* rnd) const {  // cameraPath 913-1083
+    V3 L = v3(0.0f);
+    float misSky = 1.0f, misSun = 1.0f;
+    V3 pW = start, rayDir = dir;
+    V3 throughput = v3(1.0f);
+    for (int vertex = 0; vertex <= C.maxBounces; ++vertex) {
+      if (maxC(throughput) < THROUGHPUT_EPSILON) break;
+      V3 pNext;
+      int hm;
+      bool hit = traceSurface(pW, rayDir, pNext, hm, P.maxLengthScale);
+      float rayLength = P.maxLengthScale;
+      V3 Tr;
+      if (hit) {
+        if (hm == MAT_INVAL) break;  // max-step "hit" without a material: ends the path
+        rayLength = length(pNext - pW);
+        Tr = trFree(rayLength);
+      } else {
+        Tr = trFree(P.maxLengthScale);
+        if (!C.fog || avgC(Tr) > RADIANCE_EPSILON) {
+          if (!(vertex == 0 && P.envMapVisible == 0.0f) && misSky > 0.0f)
+            L = L + throughput * Tr * misSky * environmentRadiance(rayDir);
+          if (!(vertex == 0 && P.sunVisibleDirectly == 0.0f) && misSun > 0.0f)
+            L = L + throughput * Tr * misSun * sunRadiance(rayDir);
+        }
+        if (C.fog && vertex < 2) L = L + throughput * inscatter(Tr);
+        break;
+      }
+      if (C.fog) {
+        if (vertex < 2) L = L + throughput * inscatter(Tr);
+        throughput = throughput * Tr;
+      }
+      pW = pNext;
+      V3 nW = normal(pW, MAT_SURFA);
+      V3 ngW = nW;
+      Basis b = makeBasis(nW);
+      bool sss = false;
+      float probSSS = 0.0f;
+      V3 albedoSSS = v3(0.0f);
+      if (hasMat(MAT_SURFA) && P.subsurfaceMFP > 0.0f) {
+        albedoSSS = A3(P.subsurfaceAlbedoRGB);
+        float diffW = (1.0f - P.subsurface) * avgC(diffuseRefl(pW));
+        float specW = avgC(A3(P.surfaceSpecAlbedoRGB));
+        float sssW = P.subsurface * avgC(albedoSSS) * 3.0f;
+        probSSS = fclamp(sssW / (diffW + specW + sssW + DENOM_TOLERANCE), 0.0f, 1.0f - PDF_EPSILON);
+        sss = randf(rnd) < probSSS;
+      }
+      if (!sss) {
+        V3 winW = -rayDir;
+        V3 wi = worldToLocal(winW, b);
+        V3 wo = v3(0.0f);
+        float bsdfPdf = 0.0f;
+        V3 f = sampleBsdf(pW, b, wi, MAT_SURFA, wo, bsdfPdf, rnd);
+        V3 woW = localToWorld(wo, b);
+        rayDir = woW;
+        pW = pW + ngW * fsign(dot(rayDir, ngW)) * 3.0f * P.minLengthScale;
+        float skyPdf = 0.0f, sunPdf = 0.0f;
+        L = L + throughput * direct(pW, b, winW, rnd, skyPdf, sunPdf);
+        V3 fOverPdf = vmin(v3(P.radianceClamp), f / fmax_(PDF_EPSILON, bsdfPdf));
+        throughput = throughput * (fOverPdf * std::fabs(dot(woW, nW)) / fmax_(PDF_EPSILON, 1.0f - probSSS));
+        misSky = powerHeuristic(bsdfPdf, skyPdf);
+        misSun = powerHeuristic(bsdfPdf, sunPdf);
+      } else {
+        V3 walkT = v3(1.0f), pExit = v3(0.0f);
+        b.nW = ngW;
+        if (!walk(pW, b, rnd, albedoSSS, walkT, pExit)) break;
+        pW = pExit;
+        nW = normal(pExit, MAT_SURFA);
+        Basis be = makeBasis(nW);
+        float bsdfPdf;
+        V3 woW = localToWorld(sampleHemisphereCosineWeighted(rnd, bsdfPdf), be);
+        rayDir = woW;
+        throughput = throughput * walkT;
+        float skyPdf = 0.0f, sunPdf = 0.0f;
+        V3 albedoExit = diffuseRefl(pW);
+        L = L + throughput * exitRad(pExit, be, rnd, albedoExit, skyPdf, sunPdf);
+        misSky = powerHeuristic(bsdfPdf, skyPdf);
+        misSun = powerHeuristic(bsdfPdf, sunPdf);
+        V3 f = mixR(v3(1.0f), albedoExit, P.subsurfaceDiffuseWeight) / PI_F;
+        V3 fOverPdf = vmin(v3(P.radianceClamp), f / fmax_(PDF_EPSILON, bsdfPdf));
+        throughput = throughput * (fOverPdf * std::fabs(dot(woW, nW)) / fmax_(PDF_EPSILON, probSSS));
+        pW = pW + nW * fsign(dot(rayDir, nW)) * 3.0f * P.minLengthScale;
+      }
+    }
+    return L;
+  }
+};
+
+// pathtrace + main of the simplified program (1091-1129): no interactive skip
+void simplePixel(int ix, int iy, float* rad, float* rnd) {
+  Simple T;
+  T.wl = 390.0f + (750.0f - 390.0f) * 0.5f;
+  T.rgb = v3(0.0f);
+  V3 L = v3(0.0f);
+  float px = ix + 0.5f, py = iy + 0.5f;
+  for (int n = 0; n < C.spf; ++n) {
+    float jx = 0.5f * P.filterRadius * sample_jitter(randf(rnd));
+    float jy = 0.5f * P.filterRadius * sample_jitter(randf(rnd));
+    V3 s, d;
+    constructPrimaryRay(px + jx, py + jy, rnd, s, d, true);
+    L = L + T.path(s, d, rnd);
+  }
+  accumulate(rad, rgbToXyz(L / float(C.spf)));
+}
+
 // normals program main (normals-fragment-shader.glsl:293-345)
 void normalsPixel(int ix, int iy, float* rad, float* rnd) {
   Tracer<V3> T;
@@ -1262,28 +1567,9 @@ void normalsPixel(int ix, int iy, float* rad, float* rnd) {
     float px = ix + 0.5f + (-0.5f + r1), py = iy + 0.5f + (-0.5f + r2);
     V3 s, d;
     constructPrimaryRay(px, py, rnd, s, d, false);
-    // traceDistance (70-113)
-    float sdf = 1.0e20f;
-    for (int m : {MAT_SURFA, MAT_METAL, MAT_DIELE}) if (hasMat(m)) sdf = fmin_(sdf, std::fabs(evalSdf(sdfs[m], s)));
-    float InitialSign = fsign(sdf), t = 0.0f;
-    int iters = 0, material = MAT_INVAL;
-    for (int k = 0; k < C.maxMarch; k++) {
-      t += InitialSign * sdf;
-      if (t >= P.maxLengthScale) break;
-      V3 pW = s + t * d;
-      sdf = 1.0e20f;
-      bool h = false;
-      for (int m : {MAT_SURFA, MAT_METAL, MAT_DIELE}) {
-        if (!hasMat(m)) continue;
-        float v = std::fabs(evalSdf(sdfs[m], pW));
-        if (v < P.minLengthScale) { material = m; h = true; break; }
-        sdf = fmin_(sdf, v);
-      }
-      if (h) break;
-      iters++;
-    }
-    V3 hit = s + t * d;
-    bool isHit = !(t >= P.maxLengthScale) && !(P.maxStepsIsMiss != 0.0f && iters >= C.maxMarch);
+    V3 hit;
+    int material;
+    bool isHit = traceDistance(s, d, P.maxLengthScale, hit, material);
     if (isHit) {
       V3 nW = material == MAT_INVAL ? v3(0.0f) : T.normal(hit, material, false);
       colorXYZ = colorXYZ + rgbToXyz(0.5f * (nW + v3(1.0f)));
@@ -1295,6 +1581,14 @@ void normalsPixel(int ix, int iy, float* rad, float* rnd) {
   float oldN = rad[3], newN = oldN + 1.0f;
   V3 newL = (oldN * v3(rad[0], rad[1], rad[2]) + colorXYZ) / newN;
   rad[0] = newL.x; rad[1] = newL.y; rad[2] = newL.z; rad[3] = newN;
+}
+
+// one frame of a pixel with the program of Renderer.renderMode
+void renderPixel(int ix, int iy, float* rad, float* rnd) {
+  if (C.mode == 1) aoPixel(ix, iy, rad, rnd);
+  else if (C.mode == 2) simplePixel(ix, iy, rad, rnd);
+  else if (C.dispersive) pathtracePixel<float>(ix, iy, rad, rnd);
+  else pathtracePixel<V3>(ix, iy, rad, rnd);
 }
 
 uint32_t pcg_next(uint64_t& state, uint64_t inc) {
@@ -1341,6 +1635,8 @@ void oracle_sdf(int material, int kind, float a, float b, float c, float d) {
 }
 void oracle_albedo_hook(int kind) { albedoHook = kind; }
 void oracle_emission_hook(int kind) { emissionHook = kind; }
+// Renderer.renderMode: 0 'pt', 1 'ao', 2 'ptsimple'
+void oracle_mode(int mode) { C.mode = mode; }
 // HAS_ATMOSPHERE, HAS_FOG and __MAX_ATMOSPHERE_SCATTERS__
 void oracle_volume(int atmosphere, int fog, int maxAtmScatters) {
   C.atmosphere = atmosphere; C.fog = fog; C.maxAtmScatters = maxAtmScatters;
@@ -1425,8 +1721,7 @@ int oracle_render(float* radiance, float* rng, int W, int H, int frames, int str
 #pragma omp parallel for schedule(dynamic, 64)
 #endif
       for (long long i = offset; i < npix; i += stride) {
-        if (C.dispersive) pathtracePixel<float>((int)(i % W), (int)(i / W), radiance + 4 * i, rng + 4 * i);
-        else pathtracePixel<V3>((int)(i % W), (int)(i / W), radiance + 4 * i, rng + 4 * i);
+        renderPixel((int)(i % W), (int)(i / W), radiance + 4 * i, rng + 4 * i);
       }
     }
     return 0;
@@ -1436,10 +1731,7 @@ int oracle_render(float* radiance, float* rng, int W, int H, int frames, int str
 #endif
   for (long long i = offset; i < npix; i += stride) {
     int ix = (int)(i % W), iy = (int)(i / W);
-    for (int f = 0; f < frames; ++f) {
-      if (C.dispersive) pathtracePixel<float>(ix, iy, radiance + 4 * i, rng + 4 * i);
-      else pathtracePixel<V3>(ix, iy, radiance + 4 * i, rng + 4 * i);
-    }
+    for (int f = 0; f < frames; ++f) renderPixel(ix, iy, radiance + 4 * i, rng + 4 * i);
   }
   return 0;
 }

@@ -34,6 +34,7 @@ def lib():
         L.oracle_albedo_hook.argtypes = [ctypes.c_int]
         L.oracle_volume.argtypes = [ctypes.c_int] * 3
         L.oracle_emission_hook.argtypes = [ctypes.c_int]
+        L.oracle_mode.argtypes = [ctypes.c_int]
         L.oracle_dielectric.argtypes = [ctypes.c_int, fp, ctypes.c_int]
         L.oracle_metal.argtypes = [ctypes.c_int, fp, fp]
         L.oracle_luts.argtypes = [fp, fp]
@@ -95,6 +96,7 @@ class Oracle:
             L.oracle_sdf(m, kind, a, b, c, d)
         L.oracle_albedo_hook(scene.oracle_albedo_hook)
         L.oracle_emission_hook(getattr(scene, "oracle_emission_hook", 0))
+        L.oracle_mode({"ao": 1, "ptsimple": 2}.get(r.renderMode, 0))
         diel = mats.dielectric
         model = DIELECTRIC_MODELS[type(diel).__name__]
         coeffs = f3([diel.iorVal, diel.abbe] if model == 0 else diel.coeffs)

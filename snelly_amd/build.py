@@ -16,7 +16,7 @@ LIB = os.path.join(HERE, "libsnelly_hip.so")
 
 # headers that hipRTC programs #include, in dependency order
 EMBEDDED = ["glsl.h", "glsl_swz2.h", "glsl_swz3.h", "glsl_swz4.h", "snl_params.h", "snl_prelude.h",
-            "snl_wavefront.h", "snl_pt.h", "snl_kernels.h", "snl_persist.h", "snl_mega.h"]
+            "snl_wavefront.h", "snl_pt.h", "snl_kernels.h", "snl_persist.h", "snl_mega.h", "snl_modes.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # packed FP32 instructions are disabled for every gfx950 code object we build

@@ -76,11 +76,12 @@ struct snl_prog_s {
   snl_ctx ctx;
   hipModule_t mod = nullptr;
   hipFunction_t k_reset, k_rotate, k_camera, k_trace, k_bin, k_miss, k_surf, k_metal, k_diele, k_accum, k_normals,
-      k_pick, k_tonemap, k_persist, k_persist_reset, k_mega;
+      k_pick, k_tonemap, k_persist, k_persist_reset, k_mega, k_mode = nullptr;
+  unsigned mode = 0;  // SNL_BUILD_AO or SNL_BUILD_PTSIMPLE, else 0 (the path tracer)
   int engine = SNL_ENGINE_PERSISTENT;
   int tiles = 2, persistBlocks = 0, shadeThresh = 32, persistChunk = 64, marchBurst = 4;
   bool hasSurface = false, hasMetal = false, hasDiele = false, hasAtmosphere = false;
-  int megaBlocks = 0;
+  int megaBlocks = 0, modeBlocks = 0;
   int maxBounces = 3, maxMarch = 256, spf = 1;
   SnlParams hostP;
   std::vector<char> hostU;
@@ -171,6 +172,19 @@ struct Compiled {
 // Assemble the hipRTC source, compile (or fetch from the on-disk cache).
 int compileProgram(snl_ctx ctx, const char* shader, const char* ior, const char* defines, int maxMarch,
                    int maxBounces, int maxAtm, int spf, unsigned flags, Compiled& out, char* log, int loglen) {
+  const unsigned mode = flags & (SNL_BUILD_AO | SNL_BUILD_PTSIMPLE);
+  if (mode == (SNL_BUILD_AO | SNL_BUILD_PTSIMPLE))
+    return fail(ctx, SNL_E_ARG, "SNL_BUILD_AO and SNL_BUILD_PTSIMPLE exclude each other");
+  // the AO and simplified programs are RGB only: DISPERSION_ENABLED is dropped
+  std::string defs = defines ? defines : "";
+  if (mode) {
+    const std::string d = "#define DISPERSION_ENABLED";
+    for (size_t at = defs.find(d); at != std::string::npos; at = defs.find(d, at)) {
+      size_t e = at + d.size();
+      if (e < defs.size() && (isalnum((unsigned char)defs[e]) || defs[e] == '_')) { at = e; continue; }
+      defs.erase(at, d.size());
+    }
+  }
   std::string glsl = std::string(shader ? shader : "") + "\n" + (ior ? ior : "") + "\n";
   // Renderer.compileShaders substitutes these tokens throughout the assembled
   // source, scene code included (js/renderer.js:288-294)
@@ -183,16 +197,17 @@ int compileProgram(snl_ctx ctx, const char* shader, const char* ior, const char*
     for (size_t at = glsl.find(tk.first); at != std::string::npos; at = glsl.find(tk.first, at + v.size()))
       glsl.replace(at, strlen(tk.first), v);
   }
-  out.defines = snl::parseDefines(defines ? defines : "");
+  out.defines = snl::parseDefines(defs);
   if (!snl::translateScene(glsl, builtinNames(), out.defines, out.tr)) {
     if (log && loglen > 0) snprintf(log, loglen, "%s", out.tr.error.c_str());
     return fail(ctx, SNL_E_COMPILE, "scene translation failed: %s", out.tr.error.c_str());
   }
   std::ostringstream src;
   src << "#include \"snl_prelude.h\"\n#include \"snl_wavefront.h\"\n";
-  src << (defines ? defines : "") << "\n";
+  src << defs << "\n";
   src << "namespace glsl {\n" << out.tr.code << "\n}  // namespace glsl\n";
-  src << "#line 1 \"snl_pt.h\"\n#include \"snl_pt.h\"\n#include \"snl_kernels.h\"\n#include \"snl_persist.h\"\n#include \"snl_mega.h\"\n";
+  src << "#line 1 \"snl_pt.h\"\n#include \"snl_pt.h\"\n#include \"snl_kernels.h\"\n#include \"snl_persist.h\"\n#include \"snl_mega.h\"\n"
+         "#include \"snl_modes.h\"\n";
   std::string source = src.str();
   if (const char* dump = getenv("SNL_DUMP_SOURCE")) {  // for offline ISA inspection
     std::ofstream f(dump);
@@ -204,6 +219,8 @@ int compileProgram(snl_ctx ctx, const char* shader, const char* ior, const char*
                                    "-DSNL_MAX_MARCH_STEPS=" + std::to_string(maxMarch),
                                    "-DSNL_MAX_ATMOSPHERE_SCATTERS=" + std::to_string(maxAtm),
                                    "-DSNL_MAX_SAMPLES_PER_FRAME=" + std::to_string(spf)};
+  if (mode == SNL_BUILD_AO) opts.push_back("-DSNL_MODE_AO");
+  if (mode == SNL_BUILD_PTSIMPLE) opts.push_back("-DSNL_MODE_PTSIMPLE");
   if (flags & SNL_BUILD_PRECISE) opts.push_back("-ffp-contract=off");
   if ((flags & SNL_BUILD_FAST) && !(flags & SNL_BUILD_PRECISE))
     opts.push_back("-fno-hip-fp32-correctly-rounded-divide-sqrt");
@@ -399,6 +416,8 @@ int snl_program_build(snl_ctx ctx, const char* shader, const char* ior, const ch
       {&p->k_tonemap, "snl_tonemap"}, {&p->k_persist, "snl_persist"}, {&p->k_persist_reset, "snl_persist_reset"},
       {&p->k_mega, "snl_mega"}};
   for (auto& f : fns) HIPCHK(ctx, hipModuleGetFunction(f.f, p->mod, f.n));
+  p->mode = flags & (SNL_BUILD_AO | SNL_BUILD_PTSIMPLE);
+  if (p->mode) HIPCHK(ctx, hipModuleGetFunction(&p->k_mode, p->mod, "snl_mode_kernel"));
   p->hasSurface = c.defines.count("HAS_SURFACE") > 0;
   p->hasMetal = c.defines.count("HAS_METAL") > 0;
   p->hasDiele = c.defines.count("HAS_DIELECTRIC") > 0;
@@ -437,6 +456,11 @@ int snl_program_build(snl_ctx ctx, const char* shader, const char* ior, const ch
   occ = 0;
   if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, p->k_mega, 256, 0) != hipSuccess || occ < 1) occ = 2;
   p->megaBlocks = ctx->numCU * occ;
+  if (p->mode) {
+    occ = 0;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, p->k_mode, 256, 0) != hipSuccess || occ < 1) occ = 2;
+    p->modeBlocks = ctx->numCU * occ;
+  }
   HIPCHK(ctx, hipEventCreate(&p->ev0));
   HIPCHK(ctx, hipEventCreate(&p->ev1));
   HIPCHK(ctx, hipEventCreate(&p->et0));
@@ -818,8 +842,9 @@ static int renderWavefront(snl_prog p, snl_state st, int frames) {
 
 // Persistent engine: per frame and pixel tile one launch of snl_persist; the
 // tiles run on their own streams so one tile's tail overlaps another's work.
-// The megakernel (snl_mega) launches the same way.
-static int renderPersistent(snl_prog p, snl_state st, int frames, bool mega) {
+// The megakernel (snl_mega) and the AO / simplified integrators
+// (snl_mode_kernel) launch the same way.
+static int renderPersistent(snl_prog p, snl_state st, int frames, hipFunction_t kern, int blocksAll) {
   snl_ctx ctx = p->ctx;
   bool unit0 = false;
   int rc = prepareUnit0(p, st, unit0);
@@ -843,8 +868,7 @@ static int renderPersistent(snl_prog p, snl_state st, int frames, bool mega) {
   }
   HIPCHK(ctx, hipEventRecord(p->ev0, ctx->stream));
   HIPCHK(ctx, hipEventRecord(ctx->joinEvent, ctx->stream));
-  int blocks = std::max(1, (mega ? p->megaBlocks : p->persistBlocks) / K);
-  hipFunction_t kern = mega ? p->k_mega : p->k_persist;
+  int blocks = std::max(1, blocksAll / K);
   for (int k = 0; k < K; ++k) {
     hipStream_t ts = ctx->tileStream[k];
     HIPCHK(ctx, hipStreamWaitEvent(ts, ctx->joinEvent, 0));
@@ -932,9 +956,16 @@ unsigned long long snl_state_bytes(int width, int height, int chains) {
 }
 
 // batched jobs run the persistent kernel, or the megakernel where that one
-// lacks a feature (the sub-surface walk, the atmosphere) or it is asked for
-static bool useMega(snl_prog p) {
-  return p->hasAtmosphere || p->engine == SNL_ENGINE_MEGAKERNEL || p->hostP.subsurfaceMFP > 0.0f;
+// lacks a feature (the sub-surface walk, the atmosphere) or it is asked for;
+// AO and simplified programs their own kernel
+static hipFunction_t queueKernel(snl_prog p, int& blocks) {
+  if (p->mode) { blocks = p->modeBlocks; return p->k_mode; }
+  if (p->hasAtmosphere || p->engine == SNL_ENGINE_MEGAKERNEL || p->hostP.subsurfaceMFP > 0.0f) {
+    blocks = p->megaBlocks;
+    return p->k_mega;
+  }
+  blocks = p->persistBlocks;
+  return p->k_persist;
 }
 
 int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, const int* frames,
@@ -1007,10 +1038,10 @@ int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, 
     a.itemEnd = (int)st->npix;
     // one lane per pixel up to the program's full persistent grid; the
     // megakernel takes sub-surface and atmosphere programs
-    bool mega = useMega(p);
-    hipFunction_t kern = mega ? p->k_mega : p->k_persist;
+    int resident = 0;
+    hipFunction_t kern = queueKernel(p, resident);
     long long want = (st->npix + 255) / 256;
-    int blocks = (int)std::max(1LL, std::min<long long>(mega ? p->megaBlocks : p->persistBlocks, want));
+    int blocks = (int)std::max(1LL, std::min<long long>(resident, want));
     void* args[] = {&a};
     int launches = unit0[j] ? frames[j] : (frames[j] > 0 ? 1 : 0);
     a.frames = unit0[j] ? 1 : frames[j];
@@ -1051,9 +1082,11 @@ int snl_render(snl_prog p, snl_state st, int frames) {
   // only the megakernel renders the atmosphere; the persistent kernel does not
   // run the sub-surface walk, such programs use the wavefront unless the
   // megakernel is asked for
-  if (p->hasAtmosphere || p->engine == SNL_ENGINE_MEGAKERNEL) return renderPersistent(p, st, frames, true);
+  if (p->mode) return renderPersistent(p, st, frames, p->k_mode, p->modeBlocks);
+  if (p->hasAtmosphere || p->engine == SNL_ENGINE_MEGAKERNEL) return renderPersistent(p, st, frames, p->k_mega, p->megaBlocks);
   bool needWavefront = p->hostP.subsurfaceMFP > 0.0f;
-  if (p->engine == SNL_ENGINE_PERSISTENT && !needWavefront) return renderPersistent(p, st, frames, false);
+  if (p->engine == SNL_ENGINE_PERSISTENT && !needWavefront)
+    return renderPersistent(p, st, frames, p->k_persist, p->persistBlocks);
   return renderWavefront(p, st, frames);
 }
 

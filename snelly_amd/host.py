@@ -589,6 +589,13 @@ INT_UNIFORMS = {"maxSSSSteps", "maxStepsIsMiss", "wavelengthSamples", "envMapVis
                 "haveEnvMap", "fogEnable"}
 
 
+def mode_flags(renderer):
+    """build flags of Renderer.renderMode (js/renderer.js:334-352): 'ao' and
+    'ptsimple' select their integrators; 'pt' and 'normals' use the path
+    tracer's program ('normals' renders through snl_render_normals)"""
+    return {"ao": native.BUILD_AO, "ptsimple": native.BUILD_PTSIMPLE}.get(renderer.renderMode, 0)
+
+
 class Session:
     """One scene on one GPU: program + accumulation state + LUT textures."""
 
@@ -600,7 +607,7 @@ class Session:
         self.shader, self.ior, self.defines = assemble(scene.glsl, r, mats)
         self.prog = native.Program(self.ctx, self.shader, self.ior, self.defines, int(round(r.maxMarchSteps)),
                                    int(round(r.maxBounces)), int(round(r.maxAtmosphereScatters)),
-                                   int(round(r.maxSamplesPerFrame)), flags)
+                                   int(round(r.maxSamplesPerFrame)), flags | mode_flags(r))
         self.state = native.State(self.ctx, width, height, seed, chains)
         self.textures = {}
         self.textures["WavelengthToXYZ"] = native.Texture(self.ctx, xyz_lut(), 1024, 1, 1, 4, True, True)
@@ -622,7 +629,10 @@ class Session:
         return u
 
     def render(self, frames):
-        self.prog.render(self.state, frames)
+        if self.scene.renderer.renderMode == "normals":
+            self.prog.render_normals(self.state, frames)
+        else:
+            self.prog.render(self.state, frames)
 
     def radiance(self):
         return self.state.read(0)

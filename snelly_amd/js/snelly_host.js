@@ -622,10 +622,13 @@ class Renderer {
     const ior = mats.getLoadedDielectric().ior() + '\n' + mats.getLoadedMetal().ior();
     return { shader, ior, defines };
   }
+  // renderMode 'ao' / 'ptsimple' select those integrators (js/renderer.js:334-352)
+  modeFlags() { return { ao: 4, ptsimple: 8 }[this.renderMode] || 0; }
   compileShaders(compileOnly = false) {
     const src = this.assembleShaders();
     const args = [src.shader, src.ior, src.defines, Math.round(this.maxMarchSteps), Math.round(this.maxBounces),
-                  Math.round(this.maxAtmosphereScatters), Math.round(this.maxSamplesPerFrame), this.buildFlags];
+                  Math.round(this.maxAtmosphereScatters), Math.round(this.maxSamplesPerFrame),
+                  this.buildFlags | this.modeFlags()];
     if (compileOnly) { addon.programBuild(null, ...args); return; }
     if (this.program) addon.programDestroy(this.program);
     this.program = addon.programBuild(this.snelly.ctx, ...args);

@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(HERE, "libsnelly_hip.so")
 T_FLOAT, T_VEC2, T_VEC3, T_VEC4, T_INT, T_MAT2, T_MAT3, T_MAT4, T_SAMPLER2D, T_SAMPLER3D = range(1, 11)
 BUILD_PRECISE = 1
 BUILD_FAST = 2
+BUILD_AO = 4
+BUILD_PTSIMPLE = 8
 
 # every symbol include/snelly.h declares
 EXPORTS = [

@@ -13,7 +13,7 @@ def program_key(scene, flags=0):
     r = scene.renderer
     shader, ior, defines = host.assemble(scene.glsl, r, scene.materials)
     return (shader, ior, defines, int(round(r.maxMarchSteps)), int(round(r.maxBounces)),
-            int(round(r.maxAtmosphereScatters)), int(round(r.maxSamplesPerFrame)), int(flags))
+            int(round(r.maxAtmosphereScatters)), int(round(r.maxSamplesPerFrame)), int(flags) | host.mode_flags(r))
 
 
 def _compile(key):

@@ -390,12 +390,14 @@ def run_reference(scene, W, H, frames, seed, td, name="case", program="pathtrace
 
 def make(name, td):
     scene, W, H, frames, seed = reference_cases.build(name)
-    out_rad, out_rng, gl, secs = run_reference(scene, W, H, frames, seed, td, name)
+    # Renderer.compileShaders' program per renderMode (js/renderer.js:334-352)
+    program = {"ao": "ao", "ptsimple": "simplepathtracer"}.get(scene.renderer.renderMode, "pathtracer")
+    out_rad, out_rng, gl, secs = run_reference(scene, W, H, frames, seed, td, name, program)
     # first-hit passes: the pick program per pixel, one frame of the normals program
     pick, _, _, _ = run_reference(scene, W, H, 1, seed, td, name, "pick")
     nrm_rad, nrm_rng, _, _ = run_reference(scene, W, H, 1, seed, td, name, "normals")
     meta = dict(case=name, width=W, height=H, frames=frames, seed=seed, gl=gl, seconds=round(secs, 2),
-                shaders=["shaders/pathtracer-fragment-shader.glsl", "shaders/pick-fragment-shader.glsl",
+                shaders=[f"shaders/{program}-fragment-shader.glsl", "shaders/pick-fragment-shader.glsl",
                          "shaders/normals-fragment-shader.glsl"])
     np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), radiance=out_rad, rng=out_rng,
                         pick=pick[:, :2].copy(), normals=nrm_rad, normals_rng=nrm_rng,

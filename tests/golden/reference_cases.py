@@ -131,6 +131,42 @@ def _subsurface(s):
     s.renderer.maxSSSSteps = 64
 
 
+def _ao(s):
+    """renderMode 'ao' (shaders/ao-fragment-shader.glsl): sky and sun occlusion,
+    the interactive skip"""
+    _sun_and_specular(s)
+    s.renderer.renderMode = "ao"
+    s.renderer.interactive = True
+    s.renderer.skipProbability = 0.3
+
+
+def _ao_menger(s):
+    """'ao' over a metal Menger sponge: every material is shaded with the
+    Surface diffuse colour"""
+    s.renderer.renderMode = "ao"
+    s.renderer.maxSamplesPerFrame = 2
+    s.materials.surface.diffuseAlbedo = [0.9, 0.6, 0.3]
+
+
+def _ptsimple(s):
+    """renderMode 'ptsimple' (shaders/simplepathtracer-fragment-shader.glsl):
+    RGB, sun and sky, specular lobe, fog"""
+    _sun_and_specular(s)
+    s.renderer.renderMode = "ptsimple"
+    v = s.materials.volume
+    v.fogEnable = True
+    v.fogMFP = 0.6
+    v.fogMaxOpticalDepth = 1.5
+    v.fogTint = [0.4, 0.5, 0.6]
+    v.fogEmission = [0.8, 0.7, 0.6]
+
+
+def _ptsimple_subsurface(s):
+    """'ptsimple' with the sub-surface walk"""
+    _subsurface(s)
+    s.renderer.renderMode = "ptsimple"
+
+
 CASES = {
     "sphere_plane": dict(builder="sphere_plane", kwargs=dict(width=64, height=64, spp=16), frames=16, seed=1234),
     "furnace_rgb": dict(builder="furnace", kwargs=dict(width=32, height=32, spp=8, dispersive=False), frames=8,
@@ -152,6 +188,13 @@ CASES = {
                                  seed=23, tweak=_atmosphere_fog_light),
     "volume_emission": dict(builder="sphere_plane", kwargs=dict(width=40, height=32, spp=4), frames=4, seed=25,
                             tweak=_volume_emission),
+    "ao": dict(builder="sphere_plane", kwargs=dict(width=48, height=40, spp=8), frames=8, seed=31, tweak=_ao),
+    "ao_menger": dict(builder="menger_metal", kwargs=dict(width=40, height=40, spp=4), frames=4, seed=32,
+                      tweak=_ao_menger),
+    "ptsimple": dict(builder="sphere_plane", kwargs=dict(width=40, height=32, spp=4), frames=4, seed=33,
+                     tweak=_ptsimple),
+    "ptsimple_subsurface": dict(builder="sphere_plane", kwargs=dict(width=40, height=32, spp=4), frames=4, seed=34,
+                                tweak=_ptsimple_subsurface),
     "subsurface": dict(builder="sphere_plane", kwargs=dict(width=40, height=32, spp=4), frames=4, seed=24,
                        tweak=_subsurface),
 }

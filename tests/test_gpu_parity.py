@@ -272,3 +272,18 @@ def test_pathtrace_subsurface(ctx, engine):
     g, r, close, rng_same, _ = compare_render(ctx, s, 64, 48, 8, engine=engine)
     assert close >= 0.97 and rng_same >= 0.97, (close, rng_same)
     assert rrmse(unbiased(g), unbiased(r)) < 1e-2
+
+
+@pytest.mark.parametrize("name", ["ao", "ao_menger", "ptsimple", "ptsimple_subsurface"])
+def test_render_modes(ctx, name):
+    """renderMode 'ao' and 'ptsimple' (their own integrators) against the oracle"""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import reference_cases
+    c = reference_cases.CASES[name]
+    s = getattr(scenes, c["builder"])(64, 48, 8)
+    c["tweak"](s)
+    g, r, close, rng_same, _ = compare_render(ctx, s, 64, 48, 8)
+    assert close >= 0.97 and rng_same >= 0.97, (close, rng_same)
+    assert rrmse(unbiased(g), unbiased(r)) < 1e-2

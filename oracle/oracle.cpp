@@ -300,6 +300,24 @@ float K_METAL(float wl) {
 }
 
 std::vector<float> lutXYZ(4096, 0.0f), lutICDF(4096, 0.0f);
+// the environment map: linear RGB texels, row 0 first, sampled with GL LINEAR
+// filtering and CLAMP_TO_EDGE (the texture GLU.loadImageAndCreateTextureInfo makes)
+std::vector<float> envTex;
+int envW = 0, envH = 0;
+V3 envTexel(int i, int j) {
+  i = i < 0 ? 0 : (i >= envW ? envW - 1 : i);
+  j = j < 0 ? 0 : (j >= envH ? envH - 1 : j);
+  const float* t = &envTex[3 * ((size_t)j * envW + i)];
+  return v3(t[0], t[1], t[2]);
+}
+V3 mixV(V3 a, V3 b, float t) { return a * (1.0f - t) + b * t; }
+V3 sampleEnv(float u, float v) {
+  float fu = u * float(envW) - 0.5f, fv = v * float(envH) - 0.5f;
+  float iu = std::floor(fu), iv = std::floor(fv);
+  float au = fu - iu, av = fv - iv;
+  int i0 = (int)iu, j0 = (int)iv;
+  return mixV(mixV(envTexel(i0, j0), envTexel(i0 + 1, j0), au), mixV(envTexel(i0, j0 + 1), envTexel(i0 + 1, j0 + 1), au), av);
+}
 V3 lookupXYZ(float w) {
   float f = w * 1024.0f - 0.5f, fl = std::floor(f), a = f - fl;
   int i0 = (int)fl, i1 = i0 + 1;
@@ -699,16 +717,26 @@ struct Tracer {
   }
 
   // lights (1051-1212)
-  R environmentRadiance(V3 dirW) const {
+  R environmentRadiance(V3 dirW) const {  // environmentRadianceRGB (1051-1070)
+    float rot_phi = PI_F * P.envMapPhiRotation / 180.0f;
     float rot_theta = PI_F * P.envMapThetaRotation / 180.0f;
     V3 pole = v3(std::sin(rot_theta), std::cos(rot_theta), 0.0f);
+    V3 RGB = v3(1.0f);
+    if (P.haveEnvMap != 0.0f) {
+      Basis sb = makeBasis(pole);
+      V3 dirL = worldToLocal(dirW, sb);
+      float phi = std::atan2(dirL.y, dirL.x) + PI_F + rot_phi;
+      phi -= 2.0f * PI_F * std::floor(phi / (2.0f * PI_F));
+      float theta = std::acos(dirL.z);
+      RGB = sampleEnv(phi / (2.0f * PI_F), theta / PI_F);
+    }
     float t = dot(dirW, pole);
     float tt = P.envMapTransitionAngle / 180.0f;
     float e0 = -tt, e1 = tt;
     float s = fclamp((t - e0) / (e1 - e0), 0.0f, 1.0f);
     s = s * s * (3.0f - 2.0f * s);
     V3 tint = A3(P.skyTintDown) * (1.0f - s) + A3(P.skyTintUp) * s;
-    V3 RGB = v3(1.0f) * (P.skyPower * tint);
+    RGB = RGB * (P.skyPower * tint);
     return A(RGB);
   }
   V3 sampleSunDir(float* rnd, float& pdfDir) const {
@@ -1574,7 +1602,15 @@ void normalsPixel(int ix, int iy, float* rad, float* rnd) {
       V3 nW = material == MAT_INVAL ? v3(0.0f) : T.normal(hit, material, false);
       colorXYZ = colorXYZ + rgbToXyz(0.5f * (nW + v3(1.0f)));
     } else if (P.envMapVisible != 0.0f) {
-      colorXYZ = colorXYZ + P.skyPower * v3(1.0f);
+      // environmentRadianceXYZ (normals-fragment-shader.glsl:234-253)
+      if (P.haveEnvMap != 0.0f) {
+        float phi = std::atan2(d.x, d.z) + PI_F + PI_F * P.envMapRotation / 180.0f;
+        phi -= 2.0f * PI_F * std::floor(phi / (2.0f * PI_F));
+        float theta = std::acos(d.y);
+        colorXYZ = colorXYZ + rgbToXyz(sampleEnv(phi / (2.0f * PI_F), theta / PI_F) * P.skyPower);
+      } else {
+        colorXYZ = colorXYZ + P.skyPower * v3(1.0f);
+      }
     }
   }
   colorXYZ = colorXYZ / float(C.spf);
@@ -1609,6 +1645,8 @@ void oracle_reset() {
   for (auto& s : sdfs) s = SdfSpec();
   albedoHook = 0;
   emissionHook = 0;
+  envW = envH = 0;
+  envTex.clear();
   dieleModel = 1;
   dieleC.clear();
   metalModel = 0;
@@ -1637,6 +1675,12 @@ void oracle_albedo_hook(int kind) { albedoHook = kind; }
 void oracle_emission_hook(int kind) { emissionHook = kind; }
 // Renderer.renderMode: 0 'pt', 1 'ao', 2 'ptsimple'
 void oracle_mode(int mode) { C.mode = mode; }
+// the environment map: w x h linear RGB texels, row 0 first (w = 0: none)
+void oracle_envmap(int w, int h, const float* rgb) {
+  envW = w;
+  envH = h;
+  envTex.assign(rgb, rgb + (size_t)3 * w * h);
+}
 // HAS_ATMOSPHERE, HAS_FOG and __MAX_ATMOSPHERE_SCATTERS__
 void oracle_volume(int atmosphere, int fog, int maxAtmScatters) {
   C.atmosphere = atmosphere; C.fog = fog; C.maxAtmScatters = maxAtmScatters;

@@ -35,6 +35,7 @@ def lib():
         L.oracle_volume.argtypes = [ctypes.c_int] * 3
         L.oracle_emission_hook.argtypes = [ctypes.c_int]
         L.oracle_mode.argtypes = [ctypes.c_int]
+        L.oracle_envmap.argtypes = [ctypes.c_int, ctypes.c_int, fp]
         L.oracle_dielectric.argtypes = [ctypes.c_int, fp, ctypes.c_int]
         L.oracle_metal.argtypes = [ctypes.c_int, fp, fp]
         L.oracle_luts.argtypes = [fp, fp]
@@ -83,9 +84,14 @@ class Oracle:
         r, mats = scene.renderer, scene.materials
         scene.camera.aspect = width / height
         u = host.frame_uniforms(r, scene.camera, scene.scales, mats, width, height)
+        env = getattr(scene, "env_map", None)
+        u["haveEnvMap"] = int(env is not None)
         for k, v in u.items():
             arr = np.atleast_1d(np.asarray(v, np.float32)).ravel()
             L.oracle_set(k.encode(), fp(arr), arr.size)
+        if env is not None:
+            self.env = f3(host.srgb_to_linear(np.asarray(env[2], np.uint8)).reshape(-1))
+            L.oracle_envmap(env[0], env[1], fp(self.env))
         sphere_light = 1 if r.sphereLightPower > -7.0 else 0
         L.oracle_config(int(r.dispersive), int(r.interactive), int(round(r.maxBounces)),
                         int(round(r.maxMarchSteps)), int(round(r.maxSamplesPerFrame)), sphere_light)

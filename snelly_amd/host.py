@@ -12,6 +12,8 @@ The JS host (snelly_amd/js) does the same for scene scripts.
 import json
 import math
 import os
+import struct
+import zlib
 
 import numpy as np
 
@@ -62,6 +64,64 @@ def inverse_cdf(spectrum, lmin=390.0, lmax=750.0, n=1024):
         lo, hi = float(cdf[idx - 1]), float(cdf[idx])
         icdf[k] = np.float32((idx - 1.0) / n + (1.0 / n) * (xi - lo) / (hi - lo))
     return icdf
+
+
+# ---------------------------------------------------------------- environment map
+def srgb_to_linear(rgb8):
+    """the SRGB8 texture decode of GLU.loadImageAndCreateTextureInfo
+    (js/gl.js:532-559): the sRGB transfer function of each 8-bit channel,
+    rounded to float32"""
+    c = np.asarray(rgb8, np.float64) / 255.0
+    lin = np.where(c <= 0.04045, c / 12.92, ((c + 0.055) / 1.055) ** 2.4)
+    return lin.astype(np.float32)
+
+
+def _paeth(a, b, c):
+    p = a + b - c
+    pa, pb, pc = abs(p - a), abs(p - b), abs(p - c)
+    return a if (pa <= pb and pa <= pc) else (b if pb <= pc else c)
+
+
+def load_png(path):
+    """decode an 8-bit RGB or RGBA, non-interlaced PNG (the env-map images
+    scenes name; the browser decodes them in the reference) into
+    (width, height, uint8 RGB array of shape (h, w, 3)), top row first"""
+    with open(path, "rb") as f:
+        data = f.read()
+    if data[:8] != b"\x89PNG\r\n\x1a\n":
+        raise ValueError(f"{path}: not a PNG file")
+    pos, idat, hdr = 8, b"", None
+    while pos < len(data):
+        n, kind = struct.unpack(">I4s", data[pos:pos + 8])
+        chunk = data[pos + 8:pos + 8 + n]
+        if kind == b"IHDR":
+            hdr = struct.unpack(">IIBBBBB", chunk)
+        elif kind == b"IDAT":
+            idat += chunk
+        elif kind == b"IEND":
+            break
+        pos += 12 + n
+    w, h, depth, ctype, _, _, interlace = hdr
+    if depth != 8 or ctype not in (2, 6) or interlace:
+        raise ValueError(f"{path}: only 8-bit RGB/RGBA non-interlaced PNGs are read")
+    bpp = 3 if ctype == 2 else 4
+    raw = zlib.decompress(idat)
+    stride = w * bpp
+    out = np.zeros((h, stride), np.uint8)
+    prev = np.zeros(stride, np.int32)
+    for y in range(h):
+        ft = raw[y * (stride + 1)]
+        line = np.frombuffer(raw, np.uint8, stride, y * (stride + 1) + 1).astype(np.int32)
+        cur = np.zeros(stride, np.int32)
+        for x in range(stride):
+            a = cur[x - bpp] if x >= bpp else 0
+            b = prev[x]
+            c = prev[x - bpp] if x >= bpp else 0
+            pred = (0, a, b, (a + b) // 2, _paeth(a, b, c))[ft]
+            cur[x] = (line[x] + pred) & 255
+        out[y] = cur
+        prev = cur
+    return w, h, out.reshape(h, w, bpp)[:, :, :3].copy()
 
 
 def xyz_lut():
@@ -615,12 +675,20 @@ class Session:
         self.textures["ICDF"] = native.Texture(self.ctx, icdf, 4096, 1, 1, 1, True, True)
         for name, (data, w, h) in mats.metal.textures().items():
             self.textures[name] = native.Texture(self.ctx, data, w, h, 1, 1, True, True)
+        # the scene's environment map (Scene.envMap(), js/renderer.js:143-159):
+        # (width, height, uint8 RGB rows, top row first as decoded); texture row
+        # 0 is the image's first row, as WebGL uploads it without a flip
+        if getattr(scene, "env_map", None) is not None:
+            ew, eh, rgb8 = scene.env_map
+            lin = srgb_to_linear(np.asarray(rgb8, np.uint8).reshape(eh, ew, 3))
+            self.textures["envMap"] = native.Texture(self.ctx, lin, ew, eh, 1, 3, True, True)
         self.sync()
 
     def sync(self):
         sc = self.scene
         sc.camera.aspect = self.width / self.height
         u = frame_uniforms(sc.renderer, sc.camera, sc.scales, sc.materials, self.width, self.height)
+        u["haveEnvMap"] = int("envMap" in self.textures)
         u.update(sc.user_uniforms)
         for k, v in u.items():
             self.prog.set(k, v, native.T_INT if k in INT_UNIFORMS else None)

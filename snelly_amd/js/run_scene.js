@@ -41,7 +41,7 @@ function main() {
   const { Scene } = host.loadScene(o.scene);
   const t0 = Date.now();
   const opts = { width: o.width, height: o.height, seed: o.seed, buildFlags: o.flags,
-                 compileOnly: o.compileOnly };
+                 compileOnly: o.compileOnly, baseDir: require('path').dirname(o.scene) };
   if (o.engine !== null) opts.engine = o.engine;
   const snelly = new host.Snelly(new Scene(), opts);
   const tCompile = Date.now() - t0;

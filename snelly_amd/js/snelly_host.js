@@ -107,6 +107,59 @@ class Shader {
   bind() {}
 }
 
+// ------------------------------------------------------------------ env-map images
+// The reference's GLU.loadImageAndCreateTextureInfo (js/gl.js:532-559) lets the
+// browser decode the image and uploads it as SRGB8.  Here a local 8-bit RGB or
+// RGBA PNG is decoded (zlib from Node) and its sRGB transfer function applied.
+function loadPng(file) {
+  const zlib = require('zlib');
+  const buf = fs.readFileSync(file);
+  const sig = [0x89, 0x50, 0x4e, 0x47, 0x0d, 0x0a, 0x1a, 0x0a];
+  if (!sig.every((b, i) => buf[i] === b)) throw new Error(file + ': not a PNG file');
+  let pos = 8, w = 0, h = 0, depth = 0, ctype = 0, interlace = 0;
+  const idat = [];
+  while (pos < buf.length) {
+    const n = buf.readUInt32BE(pos), kind = buf.toString('latin1', pos + 4, pos + 8);
+    const chunk = buf.subarray(pos + 8, pos + 8 + n);
+    if (kind === 'IHDR') { w = chunk.readUInt32BE(0); h = chunk.readUInt32BE(4); depth = chunk[8]; ctype = chunk[9]; interlace = chunk[12]; }
+    else if (kind === 'IDAT') idat.push(chunk);
+    else if (kind === 'IEND') break;
+    pos += 12 + n;
+  }
+  if (depth !== 8 || (ctype !== 2 && ctype !== 6) || interlace) throw new Error(file + ': only 8-bit RGB/RGBA non-interlaced PNGs are read');
+  const bpp = ctype === 2 ? 3 : 4, stride = w * bpp;
+  const raw = zlib.inflateSync(Buffer.concat(idat));
+  const rgb = new Uint8Array(w * h * 3);
+  let prev = new Uint8Array(stride);
+  for (let y = 0; y < h; ++y) {
+    const ft = raw[y * (stride + 1)], line = raw.subarray(y * (stride + 1) + 1, (y + 1) * (stride + 1));
+    const cur = new Uint8Array(stride);
+    for (let x = 0; x < stride; ++x) {
+      const a = x >= bpp ? cur[x - bpp] : 0, b = prev[x], c = x >= bpp ? prev[x - bpp] : 0;
+      let pred = 0;
+      if (ft === 1) pred = a;
+      else if (ft === 2) pred = b;
+      else if (ft === 3) pred = (a + b) >> 1;
+      else if (ft === 4) {
+        const p = a + b - c, pa = Math.abs(p - a), pb = Math.abs(p - b), pc = Math.abs(p - c);
+        pred = (pa <= pb && pa <= pc) ? a : (pb <= pc ? b : c);
+      }
+      cur[x] = (line[x] + pred) & 255;
+    }
+    for (let x = 0; x < w; ++x) for (let k = 0; k < 3; ++k) rgb[3 * (y * w + x) + k] = cur[x * bpp + k];
+    prev = cur;
+  }
+  return { width: w, height: h, rgb };
+}
+function srgbToLinear(bytes) {
+  const out = new Float32Array(bytes.length);
+  for (let i = 0; i < bytes.length; ++i) {
+    const c = bytes[i] / 255.0;
+    out[i] = c <= 0.04045 ? c / 12.92 : Math.pow((c + 0.055) / 1.055, 2.4);
+  }
+  return out;
+}
+
 class Texture {
   constructor(ctx, width, height, channels, isFloat, isLinear, isClamped, texels, depth = 1) {
     this.width = width; this.height = height; this.channels = channels;
@@ -573,6 +626,24 @@ class Renderer {
     this.numSamples = 0; this.spp = 0; this.numFramesSinceReset = 0; this.skipProbability = 0.0;
     this.frametime_measure_ms = 0.0;
     this.program = null; this.state = null; this.buildFlags = 0; this.engine = 1;
+    this.envMap = null;
+  }
+  // the scene's env map (js/renderer.js:140-159): a local PNG file, resolved
+  // against the scene file's folder; remote URLs are not fetched (the map then
+  // stays unloaded, haveEnvMap 0, as before a browser's load completes)
+  loadEnvMap(baseDir) {
+    const sceneObj = this.snelly.getScene();
+    if (typeof sceneObj.envMap === 'undefined') return;
+    const url = sceneObj.envMap();
+    if (typeof url !== 'string' || url === '' || /^[a-z]+:\/\//i.test(url)) return;
+    const file = path.isAbsolute(url) ? url : path.join(baseDir || '.', url);
+    if (!fs.existsSync(file)) return;
+    try {
+      const img = loadPng(file);
+      this.envMap = new Texture(this.snelly.ctx, img.width, img.height, 3, false, true, true, srgbToLinear(img.rgb));
+    } catch (e) {
+      console.warn('[snelly] env map not loaded: ' + e.message);
+    }
   }
   // Renderer.compileShaders (js/renderer.js:217-316)
   assembleShaders() {
@@ -697,7 +768,8 @@ class Renderer {
     P.uniformF('sphereLightRadius', this.sphereLightRadius * sn.lengthScale);
     P.uniformF('sphereLightPower', Math.pow(10.0, this.sphereLightPower) / (4.0 * Math.PI * Math.pow(this.sphereLightRadius, 2.0)));
     P.uniform3Fv('sphereLightColor', this.sphereLightColor);
-    P.uniformI('haveEnvMap', 0);
+    P.uniformI('haveEnvMap', this.envMap ? 1 : 0);
+    if (this.envMap) P.uniformTexture('envMap', this.envMap);
     P.uniformF('exposure', this.exposure);
     P.uniformF('invGamma', 1.0 / this.gamma);
     P.uniformF('contrast', this.contrast);
@@ -784,6 +856,7 @@ class Snelly {
       this.emissionIcdf = new Texture(this.ctx, 4 * this.SPECTRUM_SAMPLES, 1, 1, true, true, true, null);
     }
     this.spectra = {};
+    if (this.ctx) this.pathtracer.loadEnvMap(opts.baseDir);
     this.initScene(opts);
     this.initialized = true;
   }
@@ -908,6 +981,7 @@ function loadScene(file) {
 }
 
 module.exports = {
+  loadPng, srgbToLinear,
   addon, T, BUILD, THREE, GLU, Shader, Texture, Spectrum, BlackbodySpectrum, FlatSpectrum, MonochromaticSpectrum,
   Material, Surface, Volume, Metal, TabulatedMetal, Dielectric, AbbeDielectric, SellmeierDielectric,
   Sellmeier2Dielectric, PolyanskiyDielectric, CauchyDielectric, Materials, Renderer, Snelly, GUI,

@@ -121,6 +121,8 @@ class Scene:
         self.camera = host.Camera()
         self.scales = host.Scales()
         self.user_uniforms = {}
+        # Scene.envMap(): (width, height, uint8 RGB image rows, top row first) or None
+        self.env_map = None
         # the same geometry written by hand for the CPU oracle:
         # (material id 0 dielectric / 1 metal / 2 surface, kind, a, b, c, d)
         self.oracle_sdfs = []

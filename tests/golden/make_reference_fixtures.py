@@ -348,6 +348,8 @@ def run_reference(scene, W, H, frames, seed, td, name="case", program="pathtrace
     (radiance, rng, gl_info, seconds)"""
     scene.camera.aspect = W / H
     u = host.frame_uniforms(scene.renderer, scene.camera, scene.scales, scene.materials, W, H)
+    env = getattr(scene, "env_map", None)
+    u["haveEnvMap"] = int(env is not None)
     u.update(scene.user_uniforms)
     if program == "pick":  # Renderer.pick sets the length scales unclamped (js/renderer.js:436-438)
         u.update(lengthScale=scene.scales.lengthScale, minLengthScale=scene.scales.minLengthScale,
@@ -368,6 +370,10 @@ def run_reference(scene, W, H, frames, seed, td, name="case", program="pathtrace
     # textures at the units Renderer.render and the materials bind them to
     tex = [(2, "WavelengthToXYZ", host.xyz_lut(), 1024, 4),
            (3, "ICDF", host.inverse_cdf(lambda wl: host.blackbody(wl, scene.renderer.skyTemperature)), 4096, 1)]
+    if env is not None:  # the env map at unit 6 (js/renderer.js:595-602), uploaded as SRGB8 bytes
+        p = os.path.join(td, f"{name}_envmap")
+        np.asarray(env[2], np.uint8).tofile(p)
+        spec.append(f"texture_srgb8 6 envMap {env[0]} {env[1]} {p}")
     mt = scene.materials.metal.textures()
     if "iorTex" in mt:
         tex.append((4, "iorTex", mt["iorTex"][0], 64, 1))

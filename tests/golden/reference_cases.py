@@ -131,6 +131,31 @@ def _subsurface(s):
     s.renderer.maxSSSSteps = 64
 
 
+def env_map_image(w=32, h=16, seed=5):
+    """a synthetic 8-bit RGB lat-long environment image: colour gradients with
+    noise (top row first, as an image decodes)"""
+    import numpy as np
+    rs = np.random.RandomState(seed)
+    y, x = np.mgrid[0:h, 0:w]
+    img = np.stack([40 + 200 * x / (w - 1), 30 + 210 * (1 - y / (h - 1)), 128 + 100 * np.sin(x / 3.0)], -1)
+    img = img + rs.uniform(-25, 25, img.shape)
+    return np.clip(np.round(img), 0, 255).astype(np.uint8)
+
+
+def _env_map(s):
+    """lat-long environment-map lighting with rotated sky pole and azimuth
+    (environmentRadianceRGB, pathtracer-fragment-shader.glsl:1051-1070)"""
+    s.env_map = (32, 16, env_map_image())
+    r = s.renderer
+    r.envMapPhiRotation = 40.0
+    r.envMapThetaRotation = 20.0
+    r.envMapTransitionAngle = 60.0
+    r.skyTintUp = [1.0, 0.95, 0.9]
+    r.skyTintDown = [0.6, 0.7, 0.8]
+    s.materials.surface.specAlbedo = [0.3, 0.3, 0.3]
+    s.materials.surface.roughness = 0.3
+
+
 def _ao(s):
     """renderMode 'ao' (shaders/ao-fragment-shader.glsl): sky and sun occlusion,
     the interactive skip"""
@@ -188,6 +213,8 @@ CASES = {
                                  seed=23, tweak=_atmosphere_fog_light),
     "volume_emission": dict(builder="sphere_plane", kwargs=dict(width=40, height=32, spp=4), frames=4, seed=25,
                             tweak=_volume_emission),
+    "env_map": dict(builder="sphere_plane", kwargs=dict(width=40, height=32, spp=4), frames=4, seed=35,
+                    tweak=_env_map),
     "ao": dict(builder="sphere_plane", kwargs=dict(width=48, height=40, spp=8), frames=8, seed=31, tweak=_ao),
     "ao_menger": dict(builder="menger_metal", kwargs=dict(width=40, height=40, spp=4), frames=4, seed=32,
                       tweak=_ao_menger),

@@ -14,6 +14,9 @@
 //   state RAD_IN RNG_IN                    initial state (W*H*4 float32 each)
 //   texture UNIT NAME W H CH LINEAR CLAMP PATH   bind a float texture at UNIT and
 //                                          point sampler NAME at it ('-': no sampler)
+//   texture_srgb8 UNIT NAME W H PATH       bind W*H RGB bytes as an SRGB8 texture (LINEAR,
+//                                          CLAMP_TO_EDGE), as GLU.loadImageAndCreateTextureInfo
+//                                          uploads an image (js/gl.js:532-559)
 //   u1i NAME v | u1f NAME v | u2f NAME a b | u3f NAME a b c | u4f NAME a b c d
 //   u1fv NAME n v... | u3fv NAME n v...
 //   outputs 1|2                            render targets (2: radiance + rng)
@@ -60,7 +63,7 @@ enum : EGLint {
 enum : GLenum {
   GL_FRAGMENT_SHADER_ = 0x8B30, GL_VERTEX_SHADER_ = 0x8B31, GL_COMPILE_STATUS_ = 0x8B81, GL_LINK_STATUS_ = 0x8B82,
   GL_TEXTURE_2D_ = 0x0DE1, GL_TEXTURE0_ = 0x84C0, GL_RGBA32F_ = 0x8814, GL_R32F_ = 0x822E, GL_RG32F_ = 0x8230,
-  GL_RGB32F_ = 0x8815, GL_RGBA_ = 0x1908, GL_RGB_ = 0x1907, GL_RG_ = 0x8227, GL_RED_ = 0x1903, GL_FLOAT_ = 0x1406,
+  GL_RGB32F_ = 0x8815, GL_SRGB8_ = 0x8C41, GL_UNSIGNED_BYTE_ = 0x1401, GL_RGBA_ = 0x1908, GL_RGB_ = 0x1907, GL_RG_ = 0x8227, GL_RED_ = 0x1903, GL_FLOAT_ = 0x1406,
   GL_FRAMEBUFFER_ = 0x8D40, GL_COLOR_ATTACHMENT0_ = 0x8CE0, GL_FRAMEBUFFER_COMPLETE_ = 0x8CD5,
   GL_TEXTURE_MIN_FILTER_ = 0x2801, GL_TEXTURE_MAG_FILTER_ = 0x2800, GL_LINEAR_ = 0x2601, GL_NEAREST_ = 0x2600,
   GL_TEXTURE_WRAP_S_ = 0x2802, GL_TEXTURE_WRAP_T_ = 0x2803, GL_CLAMP_TO_EDGE_ = 0x812F, GL_REPEAT_ = 0x2901,
@@ -147,6 +150,17 @@ static GLuint makeTexture(int w, int h, int ch, bool linear, bool clamp, const f
   return t;
 }
 
+static GLuint makeSrgb8Texture(int w, int h, const unsigned char* rgb) {
+  GLuint t = 0;
+  gl.glGenTextures(1, &t);
+  gl.glBindTexture(GL_TEXTURE_2D_, t);
+  gl.glTexImage2D(GL_TEXTURE_2D_, 0, (GLint)GL_SRGB8_, w, h, 0, GL_RGB_, GL_UNSIGNED_BYTE_, rgb);
+  gl.glTexParameteri(GL_TEXTURE_2D_, GL_TEXTURE_WRAP_S_, GL_CLAMP_TO_EDGE_);
+  gl.glTexParameteri(GL_TEXTURE_2D_, GL_TEXTURE_WRAP_T_, GL_CLAMP_TO_EDGE_);
+  gl.glTexParameteri(GL_TEXTURE_2D_, GL_TEXTURE_MIN_FILTER_, GL_LINEAR_);
+  return t;
+}
+
 struct Tex { int unit; std::string name; GLuint id; };
 struct Uni { std::string kind, name; std::vector<double> v; };
 
@@ -174,6 +188,11 @@ int main(int argc, char** argv) {
       else if (d == "fragment") w >> fsPath;
       else if (d == "state") w >> radIn >> rngIn;
       else if (d == "out") { w >> radOut; w >> rngOut; }
+      else if (d == "texture_srgb8") {
+        std::vector<std::string> f(5);
+        for (auto& x : f) w >> x;
+        texSpecs.push_back({-1 - atoi(f[0].c_str()), f});
+      }
       else if (d == "texture") {
         std::vector<std::string> f(8);
         for (auto& x : f) w >> x;
@@ -244,6 +263,15 @@ int main(int argc, char** argv) {
   std::vector<Tex> texs;
   for (auto& ts : texSpecs) {
     const auto& f = ts.second;
+    if (ts.first < 0) {  // texture_srgb8: unit is -1 - UNIT
+      int w = atoi(f[2].c_str()), h = atoi(f[3].c_str());
+      std::string bytes = slurp(f[4]);
+      if ((3 * w) % 4 != 0) die("texture_srgb8: rows must be whole 4-byte words");
+      if (bytes.size() != (size_t)w * h * 3) die("texture_srgb8: size mismatch");
+      GLuint id = makeSrgb8Texture(w, h, (const unsigned char*)bytes.data());
+      texs.push_back({-1 - ts.first, f[1], id});
+      continue;
+    }
     int w = atoi(f[2].c_str()), h = atoi(f[3].c_str()), ch = atoi(f[4].c_str());
     std::vector<float> data = readFloats(f[7], (size_t)w * h * ch);
     GLuint id = makeTexture(w, h, ch, f[5] == "1", f[6] == "1", data.data());

@@ -22,6 +22,11 @@ The Mandelbox dielectric (16 bounces through a fractal) is chaotic: paths
 separate after a few internal bounces, so its radiance is compared as
 statistics (mean per channel within 4 standard errors) and its first hits per
 pixel.
+The environment-map case holds every bar but the 1e-4 one: an SRGB8 texture's
+decode and filter precision are the implementation's (the GLES implementation
+that wrote the fixture decodes texel centres to within 1.4e-3 relative and
+filters to within a median 2e-4 of exact float bilinear filtering of the
+exactly decoded texels, which is what the HIP path and the oracle compute).
 """
 import json
 import math
@@ -38,6 +43,7 @@ import reference_cases  # noqa: E402
 
 CASES = list(reference_cases.CASES)
 CHAOTIC = {"mandelbox_dielectric"}
+TEXTURE_PRECISION = {"env_map"}
 ORACLE_CASES = [c for c in CASES if reference_cases.build(c)[0].oracle_sdfs]
 
 
@@ -62,7 +68,9 @@ def check_pathtrace(name, rad, rng, ref):
     same_rng = np.all(rng == ref["rng"], axis=1).mean()
     c2, c3, c4 = (rel_close(rad, ref["radiance"], t).mean() for t in (1e-2, 1e-3, 1e-4))
     assert same_rng >= 0.99, same_rng
-    assert c2 >= 0.99 and c3 >= 0.97 and c4 >= 0.85, (c2, c3, c4)
+    assert c2 >= 0.99 and c3 >= 0.97, (c2, c3, c4)
+    if name not in TEXTURE_PRECISION:
+        assert c4 >= 0.85, (c2, c3, c4)
 
 
 def check_pick(pk, ref):

@@ -124,6 +124,23 @@ def load_png(path):
     return w, h, out.reshape(h, w, bpp)[:, :, :3].copy()
 
 
+def save_png(path, rgba8):
+    """write an (h, w, 4) uint8 image whose row 0 is the bottom row (as the
+    tonemapper and gl.readPixels return it) as an RGBA PNG, top row first --
+    the canvas.toBlob('image/png') of the 'P' key (js/snelly.js:791-806)"""
+    img = np.asarray(rgba8, np.uint8)
+    h, w = img.shape[:2]
+    rows = img[::-1].reshape(h, w * 4)
+    raw = b"".join(b"\x00" + rows[y].tobytes() for y in range(h))
+
+    def chunk(kind, data):
+        return struct.pack(">I", len(data)) + kind + data + struct.pack(">I", zlib.crc32(kind + data) & 0xffffffff)
+    ihdr = struct.pack(">IIBBBBB", w, h, 8, 6, 0, 0, 0)
+    with open(path, "wb") as f:
+        f.write(b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", ihdr) + chunk(b"IDAT", zlib.compress(raw)) +
+                chunk(b"IEND", b""))
+
+
 def xyz_lut():
     """1024 x 4 RGBA32F wavelength -> XYZ table (js/color.js:24-283)."""
     t = np.asarray(tables()["xyz_lut"], np.float32)
@@ -704,6 +721,11 @@ class Session:
 
     def radiance(self):
         return self.state.read(0)
+
+    def save_png(self, path):
+        """the tonemapped image as a PNG file"""
+        _, rgba8 = self.prog.tonemap(self.state)
+        save_png(path, rgba8.reshape(self.height, self.width, 4))
 
     def close(self):
         for t in self.textures.values():

@@ -151,6 +151,43 @@ function loadPng(file) {
   }
   return { width: w, height: h, rgb };
 }
+// an RGBA8 image (rows bottom first, as gl.readPixels returns them) as a PNG
+// file, top row first -- the canvas.toBlob('image/png') of the 'P' key
+// (js/snelly.js:791-806)
+const CRC_TABLE = (() => {
+  const t = new Uint32Array(256);
+  for (let n = 0; n < 256; ++n) {
+    let c = n;
+    for (let k = 0; k < 8; ++k) c = (c & 1) ? (0xedb88320 ^ (c >>> 1)) : (c >>> 1);
+    t[n] = c >>> 0;
+  }
+  return t;
+})();
+function crc32(buf) {
+  let c = 0xffffffff;
+  for (let i = 0; i < buf.length; ++i) c = CRC_TABLE[(c ^ buf[i]) & 255] ^ (c >>> 8);
+  return (c ^ 0xffffffff) >>> 0;
+}
+function savePng(file, width, height, rgba) {
+  const zlib = require('zlib');
+  const raw = Buffer.alloc((width * 4 + 1) * height);
+  for (let y = 0; y < height; ++y) {
+    const src = (height - 1 - y) * width * 4;
+    raw[y * (width * 4 + 1)] = 0;
+    for (let i = 0; i < width * 4; ++i) raw[y * (width * 4 + 1) + 1 + i] = rgba[src + i];
+  }
+  const chunk = (kind, data) => {
+    const len = Buffer.alloc(4); len.writeUInt32BE(data.length);
+    const body = Buffer.concat([Buffer.from(kind, 'latin1'), data]);
+    const crc = Buffer.alloc(4); crc.writeUInt32BE(crc32(body));
+    return Buffer.concat([len, body, crc]);
+  };
+  const ihdr = Buffer.alloc(13);
+  ihdr.writeUInt32BE(width, 0); ihdr.writeUInt32BE(height, 4);
+  ihdr[8] = 8; ihdr[9] = 6; ihdr[10] = 0; ihdr[11] = 0; ihdr[12] = 0;
+  fs.writeFileSync(file, Buffer.concat([Buffer.from([0x89, 0x50, 0x4e, 0x47, 0x0d, 0x0a, 0x1a, 0x0a]),
+    chunk('IHDR', ihdr), chunk('IDAT', zlib.deflateSync(raw)), chunk('IEND', Buffer.alloc(0))]));
+}
 function srgbToLinear(bytes) {
   const out = new Float32Array(bytes.length);
   for (let i = 0; i < bytes.length; ++i) {
@@ -354,8 +391,25 @@ function wavelengthToXYZTable() {
 
 // ------------------------------------------------------------------ materials (js/materials.js)
 class Material { constructor(name) { this._name = name; } getName() { return this._name; } }
+// The repr() of the materials and Snelly.dumpScene print settings as scene
+// code (js/materials.js:56-70, 222-243, 488-495, 740-749; js/snelly.js:296-406):
+// the user pastes it into a scene, so the lines keep the reference's form.
+const v3s = (v) => `[${v[0]}, ${v[1]}, ${v[2]}]`;
 
 class Surface extends Material {
+  repr() {
+    return `
+    surface.roughness = ${this.roughness};
+    surface.ior = ${this.ior};
+    surface.diffuseAlbedo = ${v3s(this.diffuseAlbedo)};
+    surface.specAlbedo = ${v3s(this.specAlbedo)};
+    surface.subsurface = ${this.subsurface};
+    surface.subsurfaceAlbedoRGB = ${v3s(this.subsurfaceAlbedoRGB)};
+    surface.subsurfaceMFP = ${this.subsurfaceMFP};
+    surface.subsurfaceAnisotropy = ${this.subsurfaceAnisotropy};
+    surface.subsurfaceDiffuseWeight = ${this.subsurfaceDiffuseWeight};
+    `;
+  }
   constructor(name) {
     super(name);
     this.diffuseAlbedo = [1, 1, 1]; this.specAlbedo = [1, 1, 1]; this.roughness = 0.1; this.ior = 1.5;
@@ -377,6 +431,26 @@ class Surface extends Material {
 }
 
 class Volume extends Material {
+  repr() {
+    return `
+    volume.atmosphereMinX = ${this.atmosphereMinX};
+    volume.atmosphereMinY = ${this.atmosphereMinY};
+    volume.atmosphereMinZ = ${this.atmosphereMinZ};
+    volume.atmosphereMaxX = ${this.atmosphereMaxX};
+    volume.atmosphereMaxY = ${this.atmosphereMaxY};
+    volume.atmosphereMaxZ = ${this.atmosphereMaxZ};
+    volume.lof10_mfp = ${this.lof10_mfp};
+    volume.scatteringColor = ${v3s(this.scatteringColor)};
+    volume.absorptionColor = ${v3s(this.absorptionColor)};
+    volume.emission = ${this.emission};
+    volume.emissionColor = ${v3s(this.emissionColor)};
+    volume.fogEmission = ${v3s(this.fogEmission)};
+    volume.fogTint = ${v3s(this.fogTint)};
+    volume.fogMFP = ${this.fogMFP};
+    volume.fogMaxOpticalDepth = ${this.fogMaxOpticalDepth};
+    volume.fogEnable = ${this.fogEnable};
+    `;
+  }
   constructor(name) {
     super(name);
     this.atmosphereMinX = -10; this.atmosphereMaxX = 10; this.atmosphereMinY = 0; this.atmosphereMaxY = 10;
@@ -405,6 +479,12 @@ class Volume extends Material {
 }
 
 class Metal extends Material {
+  repr() {
+    return `
+    metal.roughness = ${this.roughness};
+    metal.specAlbedo = ${v3s(this.specAlbedo)};
+    `;
+  }
   constructor(name) { super(name); this.roughness = 0.02; this.specAlbedo = [1, 1, 1]; }
   syncShader(s) {
     s.uniformF('metalRoughness', this.roughness);
@@ -445,6 +525,15 @@ float K_METAL(float wavelength_nm)
 }
 
 class Dielectric extends Material {
+  repr() {
+    return `
+    dielectric.absorptionColor = ${v3s(this.absorptionColor)};
+    dielectric.absorptionScale = ${this.absorptionScale}; // mfp in multiples of scene scale
+    dielectric.roughness = ${this.roughness};
+    dielectric.iorVal = ${this.iorVal};
+    dielectric.abbe = ${this.abbe};
+    `;
+  }
   constructor(name) {
     super(name);
     this.roughness = 0.005; this.absorptionScale = -1.0; this.absorptionColor = [1, 1, 1];
@@ -859,6 +948,7 @@ class Snelly {
     if (this.ctx) this.pathtracer.loadEnvMap(opts.baseDir);
     this.initScene(opts);
     this.initialized = true;
+    if (this.camera.auto_focus && !opts.compileOnly) this.auto_focus();
   }
   getVersion() { return [1, 19, 0]; }
   getRenderer() { return this.pathtracer; }
@@ -918,7 +1008,122 @@ class Snelly {
   }
   reset(no_recompile = false) {
     if (!this.initialized || this.terminated) return;
+    if (this.camera.auto_focus) this.auto_focus();
     this.pathtracer.reset(no_recompile);
+  }
+  // Snelly.auto_focus (js/snelly.js:707-722): focus on the first hit under the
+  // centre of the image
+  auto_focus() {
+    const r = this.pathtracer;
+    const pickData = r.pick(0.5 * r._width, 0.5 * r._height);
+    if (pickData == null) return;
+    const fd = Math.max(1.0e-6, Math.abs(pickData.distance));
+    const newFocalDistance = Math.log10(fd / this.lengthScale);
+    if (newFocalDistance === this.camera.focalDistance) return;
+    this.camera.focalDistance = newFocalDistance;
+    if (this.gui) this.gui.sync();
+  }
+  // the tonemapped image as a PNG file (the 'P' key, js/snelly.js:791-806)
+  saveImage(file) {
+    const r = this.pathtracer;
+    r.syncUniforms(r.shaderObj);
+    savePng(file, r._width, r._height, addon.tonemapRGBA8(r.program, r.state, r._width * r._height));
+  }
+  // Snelly.dumpScene (js/snelly.js:296-406): the current settings as scene code
+  dumpScene() {
+    const camera = this.camera, controls = this.camControls, renderer = this.pathtracer, materials = this.materials;
+    let code = `
+{
+    /******* copy-pasted console output on 'O', begin *******/\n`;
+    code += `
+    let renderer  = snelly.getRenderer();
+    let camera    = snelly.getCamera();
+    let controls  = snelly.getControls();
+    let materials = snelly.getMaterials();
+    `;
+    if (typeof this.sceneObj.initGenerator !== 'undefined') code += this.sceneObj.initGenerator();
+    code += this.guiVisible ? `\nsnelly.showGUI(true);\n` : `\nsnelly.showGUI(false);\n`;
+    const r = renderer;
+    code += `
+    /** Camera settings **/
+    camera.fov = ${camera.fov};
+    camera.aperture = ${camera.aperture};
+    camera.focalDistance = ${camera.focalDistance};
+    camera.auto_focus = ${camera.auto_focus};
+    camera.up.set(${camera.up.x}, ${camera.up.y}, ${camera.up.z});
+    camera.position.set(${camera.position.x}, ${camera.position.y}, ${camera.position.z});
+    controls.target.set(${controls.target.x}, ${controls.target.y}, ${controls.target.z});
+
+    /** Renderer settings **/
+    // General rendering settings
+    renderer.renderMode = '${r.renderMode}';
+    renderer.dispersive = ${r.dispersive};
+    renderer.maxSamplesPerFrame = ${r.maxSamplesPerFrame};
+    renderer.maxSpp = ${r.maxSpp};
+    renderer.maxBounces = ${r.maxBounces};
+    renderer.maxAtmosphereScatters = ${r.maxAtmosphereScatters};
+    renderer.maxMarchSteps = ${r.maxMarchSteps};
+    renderer.maxStepsIsMiss = ${r.maxStepsIsMiss};
+    renderer.interactive = ${r.interactive};
+    renderer.goalFPS = ${r.goalFPS};
+    renderer.minsSPPToRedraw = ${r.minsSPPToRedraw};
+    renderer.filterRadius = ${r.filterRadius};
+    renderer.radianceClamp = ${r.radianceClamp};
+    renderer.wavelengthSamples = ${r.wavelengthSamples};
+    renderer.shadowStrength = ${r.shadowStrength};
+    // Tone-mapping
+    renderer.exposure = ${r.exposure};
+    renderer.gamma = ${r.gamma};
+    renderer.contrast = ${r.contrast};
+    renderer.saturation = ${r.saturation};
+    renderer.hueShift = ${r.hueShift};
+    // Lights
+        // sky light
+        renderer.skyPower = ${r.skyPower};
+        renderer.skyTintUp = ${v3s(r.skyTintUp)};
+        renderer.skyTintDown = ${v3s(r.skyTintDown)};
+        renderer.envMapVisible = ${r.envMapVisible};
+        renderer.envMapPhiRotation = ${r.envMapPhiRotation};
+        renderer.envMapThetaRotation = ${r.envMapThetaRotation};
+        renderer.envMapTransitionAngle = ${r.envMapTransitionAngle};
+        // sun light
+        renderer.sunPower = ${r.sunPower};
+        renderer.sunColor = ${v3s(r.sunColor)};
+        renderer.sunAngularSize = ${r.sunAngularSize};
+        renderer.sunLatitude = ${r.sunLatitude};
+        renderer.sunLongitude = ${r.sunLongitude};
+        renderer.sunVisibleDirectly = ${r.sunVisibleDirectly};
+        // sphere light
+        renderer.sphereLightPosition = ${v3s(r.sphereLightPosition)};
+        renderer.sphereLightRadius = ${r.sphereLightRadius};
+        renderer.sphereLightPower = ${r.sphereLightPower};
+        renderer.sphereLightColor = ${v3s(r.sphereLightColor)};
+`;
+    code += `
+    /** Material settings **/`;
+    const shader = this.sceneObj.shader();
+    if (shader.indexOf('SDF_SURFACE(') !== -1) {
+      code += `
+    let surface = materials.loadSurface();`;
+      code += materials.loadSurface().repr();
+    }
+    if (shader.indexOf('SDF_METAL(') !== -1) {
+      code += `
+    let metal = materials.loadMetal('${materials.getLoadedMetal().getName()}');`;
+      code += materials.loadMetal(materials.getLoadedMetal().getName()).repr();
+    }
+    if (shader.indexOf('SDF_DIELECTRIC(') !== -1) {
+      code += `
+    let dielectric = materials.loadDielectric('${materials.getLoadedDielectric().getName()}');`;
+      code += materials.loadDielectric(materials.getLoadedDielectric().getName()).repr();
+    }
+    code += `
+    let volume = materials.loadVolume();`;
+    code += materials.loadVolume().repr();
+    code += `
+    /******* copy-pasted console output on 'O', end *******/
+}`;
+    return code;
   }
   render(frames = 1) {
     if (!this.initialized || this.terminated) return;
@@ -981,7 +1186,7 @@ function loadScene(file) {
 }
 
 module.exports = {
-  loadPng, srgbToLinear,
+  loadPng, srgbToLinear, savePng,
   addon, T, BUILD, THREE, GLU, Shader, Texture, Spectrum, BlackbodySpectrum, FlatSpectrum, MonochromaticSpectrum,
   Material, Surface, Volume, Metal, TabulatedMetal, Dielectric, AbbeDielectric, SellmeierDielectric,
   Sellmeier2Dielectric, PolyanskiyDielectric, CauchyDielectric, Materials, Renderer, Snelly, GUI,

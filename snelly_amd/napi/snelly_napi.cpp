@@ -287,6 +287,20 @@ napi_value Tonemap(napi_env env, napi_callback_info info) {
   if (rc) return throwSnl(env, rc, nullptr);
   return arr;
 }
+// tonemapRGBA8(prog, state, npix) -> Uint8Array rgba (4 per pixel, GL UNORM8)
+napi_value TonemapRGBA8(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!a.ok(env, info)) return nullptr;
+  size_t n = (size_t)getNum(env, a.argv[2]) * 4;
+  napi_value ab, arr;
+  void* raw = nullptr;
+  napi_create_arraybuffer(env, n, &raw, &ab);
+  napi_create_typedarray(env, napi_uint8_array, n, ab, 0, &arr);
+  int rc = snl_tonemap(static_cast<snl_prog>(getPtr(env, a.argv[0])), static_cast<snl_state>(getPtr(env, a.argv[1])),
+                       nullptr, static_cast<unsigned char*>(raw));
+  if (rc) return throwSnl(env, rc, nullptr);
+  return arr;
+}
 napi_value Stats(napi_env env, napi_callback_info info) {
   Args a;
   if (!a.ok(env, info)) return nullptr;
@@ -322,7 +336,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"uniformTexture", UniformTexture}, {"textureCreate", TextureCreate}, {"textureUpload", TextureUpload},
       {"textureDestroy", TextureDestroy}, {"stateCreate", StateCreate}, {"stateDestroy", StateDestroy},
       {"stateSeed", StateSeed}, {"stateClear", StateClear}, {"stateRead", StateRead}, {"stateWrite", StateWrite},
-      {"render", Render}, {"renderNormals", RenderNormals}, {"pick", Pick}, {"tonemap", Tonemap},
+      {"render", Render}, {"renderNormals", RenderNormals}, {"pick", Pick}, {"tonemap", Tonemap}, {"tonemapRGBA8", TonemapRGBA8},
       {"stats", Stats}, {"setEngine", SetEngine}, {"version", Version}};
   for (auto& f : fns) {
     napi_value fn;

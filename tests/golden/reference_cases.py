@@ -156,6 +156,37 @@ def _env_map(s):
     s.materials.surface.roughness = 0.3
 
 
+def _normal_maps(s):
+    """SURFACE_NORMAL_MAP and METAL_NORMAL_MAP hooks (perturbNormal,
+    pathtracer-fragment-shader.glsl:942-968, and the flipping of 1790-1793):
+    a bumpy plane under a bumpy tabulated-aluminium sphere.  No oracle scene:
+    held against the reference only."""
+    s.glsl = """
+float SDF_SURFACE(vec3 X) { return X.y + 1.0; }
+float SDF_METAL(vec3 X) { return length(X) - 1.0; }
+vec3 SURFACE_NORMAL_MAP(vec3 X, vec3 N) { return vec3(0.3*sin(6.0*X.x), 0.3*cos(4.0*X.z), 1.0); }
+vec3 METAL_NORMAL_MAP(vec3 X, vec3 N) { return vec3(0.25*sin(9.0*X.y), 0.25*sin(7.0*X.x), 1.0); }
+"""
+    s.materials.metal.roughness = 0.15
+    s.renderer.sunPower = 0.3
+    s.oracle_sdfs = []
+
+
+def _custom_camera(s):
+    """a scene-defined CONSTRUCT_PRIMARY_RAY (an orthographic camera built from
+    the camera uniforms, pathtracer-fragment-shader.glsl:1963-1967).  No
+    oracle scene: held against the reference only."""
+    s.glsl += """
+void CONSTRUCT_PRIMARY_RAY(in vec2 pixel, inout vec4 rnd, inout vec3 primaryStart, inout vec3 primaryDir)
+{
+    vec2 ndc = -1.0 + 2.0*(pixel/resolution.xy);
+    primaryStart = camPos + 1.8*(ndc.x*camAspect*camX + ndc.y*camY);
+    primaryDir = camDir;
+}
+"""
+    s.oracle_sdfs = []
+
+
 def _ao(s):
     """renderMode 'ao' (shaders/ao-fragment-shader.glsl): sky and sun occlusion,
     the interactive skip"""
@@ -215,6 +246,10 @@ CASES = {
                             tweak=_volume_emission),
     "env_map": dict(builder="sphere_plane", kwargs=dict(width=40, height=32, spp=4), frames=4, seed=35,
                     tweak=_env_map),
+    "normal_maps": dict(builder="sphere_plane", kwargs=dict(width=40, height=32, spp=4), frames=4, seed=36,
+                        tweak=_normal_maps),
+    "custom_camera": dict(builder="sphere_plane", kwargs=dict(width=40, height=32, spp=4), frames=4, seed=37,
+                          tweak=_custom_camera),
     "ao": dict(builder="sphere_plane", kwargs=dict(width=48, height=40, spp=8), frames=8, seed=31, tweak=_ao),
     "ao_menger": dict(builder="menger_metal", kwargs=dict(width=40, height=40, spp=4), frames=4, seed=32,
                       tweak=_ao_menger),

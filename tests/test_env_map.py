@@ -99,3 +99,14 @@ def test_js_and_python_decoders_agree(tmp_path):
     assert (j["w"], j["h"]) == (w, h)
     assert np.array_equal(np.array(j["rgb"], np.uint8), rgb.reshape(-1))
     assert np.array_equal(np.array(j["lin"], np.float32), host.srgb_to_linear(rgb).reshape(-1))
+
+
+def test_save_png_round_trip(tmp_path):
+    """save_png writes row 0 (the bottom row) last; load_png reads it back"""
+    rs = np.random.RandomState(4)
+    img = rs.randint(0, 256, (6, 5, 4)).astype(np.uint8)
+    p = tmp_path / "o.png"
+    host.save_png(str(p), img)
+    w, h, rgb = host.load_png(str(p))
+    assert (w, h) == (5, 6)
+    assert np.array_equal(rgb, img[::-1, :, :3])

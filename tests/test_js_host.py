@@ -49,7 +49,7 @@ def test_addon_exports():
     names = ["ctxCreate", "ctxDestroy", "sync", "programBuild", "programDestroy", "uniformIndex", "uniformSet",
              "uniformTexture", "textureCreate", "textureUpload", "textureDestroy", "stateCreate", "stateDestroy",
              "stateSeed", "stateClear", "stateRead", "stateWrite", "render", "renderNormals", "pick", "tonemap",
-             "stats", "setEngine", "version"]
+             "tonemapRGBA8", "stats", "setEngine", "version"]
     code = f"const a = require({json.dumps(ADDON)}); console.log(JSON.stringify([Object.keys(a), a.version()]));"
     out = subprocess.run([NODE, "-e", code], capture_output=True, text=True, check=True).stdout
     keys, version = json.loads(out.strip().splitlines()[-1])
@@ -157,3 +157,67 @@ def test_js_render_matches_oracle():
     close = np.all(np.abs(g[:, :3] - rad[:, :3]) <= 1e-4 * np.maximum(np.abs(rad[:, :3]), 1e-3), axis=1)
     assert close.mean() >= 0.99
     assert np.all(g_rng == rng, axis=1).mean() >= 0.99
+
+
+HOST_JS = os.path.join(ROOT, "snelly_amd", "js", "snelly_host.js")
+
+
+@needs_node
+def test_dump_scene_round_trip():
+    """Snelly.dumpScene (the 'O' key) prints the settings as scene code; run
+    against a fresh Snelly it restores them"""
+    code = f"""
+const h = require({json.dumps(HOST_JS)});
+const {{ Scene }} = h.loadScene({json.dumps(os.path.join(SCENES, 'sphere_plane.html'))});
+const a = new h.Snelly(new Scene(), {{ compileOnly: true, width: 64, height: 48 }});
+const r = a.getRenderer(), c = a.getCamera(), m = a.getMaterials();
+r.skyPower = 0.7; r.renderMode = 'ao'; r.sunLatitude = 33.5; r.skyTintUp = [0.1, 0.2, 0.3]; r.maxBounces = 7;
+c.fov = 33; c.aperture = -2.5; c.position.set(1.5, 2.5, 3.5); a.getControls().target.set(0.1, 0.2, 0.3);
+m.loadSurface().roughness = 0.27; m.loadSurface().diffuseAlbedo = [0.4, 0.5, 0.6];
+m.loadVolume().fogEnable = true; m.loadVolume().fogTint = [0.3, 0.2, 0.1];
+const code = a.dumpScene();
+const b = new h.Snelly(new Scene(), {{ compileOnly: true, width: 64, height: 48 }});
+new Function('snelly', code)(b);
+const pick = (s) => {{ const r = s.getRenderer(), c = s.getCamera(), m = s.getMaterials();
+  return [r.skyPower, r.renderMode, r.sunLatitude, r.skyTintUp, r.maxBounces, c.fov, c.aperture,
+          c.position.toArray(), s.getControls().target.toArray(), m.loadSurface().roughness,
+          m.loadSurface().diffuseAlbedo, m.loadVolume().fogEnable, m.loadVolume().fogTint]; }};
+console.log(JSON.stringify([pick(a), pick(b), code.includes('copy-pasted console output')]));
+"""
+    r = subprocess.run([NODE, "-e", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    a, b, marker = json.loads(r.stdout.strip().splitlines()[-1])
+    assert a == b and marker
+    assert a[1] == "ao" and a[3] == [0.1, 0.2, 0.3]
+
+
+@needs_node
+@pytest.mark.gpu
+def test_js_autofocus_and_png():
+    """camera.auto_focus focuses on the first hit under the image centre
+    (Snelly.auto_focus); saveImage writes the tonemapped frame as a PNG that
+    decodes to the tonemapper's UNORM8 output, top row first"""
+    from snelly_amd import host
+    with tempfile.TemporaryDirectory() as td:
+        png = os.path.join(td, "frame.png")
+        code = f"""
+const h = require({json.dumps(HOST_JS)});
+const {{ Scene }} = h.loadScene({json.dumps(os.path.join(SCENES, 'sphere_plane.html'))});
+const sn = new h.Snelly(new Scene(), {{ width: 40, height: 30, override: (s) => {{ s.getCamera().auto_focus = true; }} }});
+const r = sn.getRenderer();
+const p = r.pick(20, 15);
+sn.render(3);
+sn.saveImage({json.dumps(png)});
+r.syncUniforms(r.shaderObj);
+const rgba = h.addon.tonemapRGBA8(r.program, r.state, 40 * 30);
+console.log(JSON.stringify([sn.getCamera().focalDistance, p.distance, sn.lengthScale, Array.from(rgba)]));
+"""
+        out = subprocess.run([NODE, "-e", code], capture_output=True, text=True, timeout=300)
+        assert out.returncode == 0, out.stderr
+        fd, dist, ls, rgba = json.loads(out.stdout.strip().splitlines()[-1])
+        assert 0.0 < dist < 100.0
+        assert abs(fd - np.log10(max(1e-6, abs(dist)) / ls)) < 1e-12
+        w, h, rgb = host.load_png(png)
+        assert (w, h) == (40, 30)
+        want = np.array(rgba, np.uint8).reshape(30, 40, 4)[::-1, :, :3]
+        assert np.array_equal(rgb, want)

@@ -32,9 +32,10 @@ os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("SNL_BENCH_HW_QUEUES", "16")
 METRIC = "path samples/s (plain vs batched/served) at RMSE≤1%; p50/p99 job latency"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-# HBM bytes per pixel per snl_persist launch, from the FETCH_SIZE / WRITE_SIZE
-# counter runs scaled by the copy yardstick (profiles/snl_persist_menger4_1024_traffic_valu.md)
-PERSIST_TRAFFIC_B_PER_PX = 67.1
+# HBM bytes per pixel per launch of the queue kernels, from the FETCH_SIZE /
+# WRITE_SIZE counter runs scaled by the copy yardstick
+# (profiles/<kernel>_menger4_1024_traffic_valu.md)
+TRAFFIC_B_PER_PX = {"snl_persist": 67.1, "snl_mega": 67.0}
 
 
 def unbiased(rad):
@@ -142,6 +143,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--served-jobs", type=int, default=200)
+    ap.add_argument("--engine", choices=["auto", "persistent", "megakernel"], default="auto",
+                    help="pixel-queue engine of the timed renders (auto: the faster one, tuned)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the path is per-image data parallel "
@@ -173,7 +176,9 @@ def main():
     megakernel = {"path_samples_per_s": round(W * H * spp / (mk.ms_total * 1e-3), 1), "ms": round(mk.ms_total, 2)}
     # the pixel-queue engine (persistent state machine or megakernel) and its
     # launch shape, autotuned untimed
-    best, _, _ = autotune.tune(sess, frames=8)
+    engines = {"auto": (native.ENGINE_PERSISTENT, native.ENGINE_MEGAKERNEL),
+               "persistent": (native.ENGINE_PERSISTENT,), "megakernel": (native.ENGINE_MEGAKERNEL,)}[args.engine]
+    best, _, _ = autotune.tune(sess, frames=8, engines=engines)
     for w in range(args.warmup):
         sess.state.seed(1000 + w)
         sess.render(spp)
@@ -213,7 +218,8 @@ def main():
     pixels_per_launch = W * H * args.steps / launches
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                "traffic": round(PERSIST_TRAFFIC_B_PER_PX * pixels_per_launch),
+                "traffic": (None if TRAFFIC_B_PER_PX[autotune.KERNEL_NAMES[best["engine"]]] is None else
+                            round(TRAFFIC_B_PER_PX[autotune.KERNEL_NAMES[best["engine"]]] * pixels_per_launch)),
                 "traffic_algorithmic": 64 * pixels_per_launch,
                 "kernel": autotune.KERNEL_NAMES[best["engine"]], "launches": launches, "gpu_ms": round(trace_ms, 2),
                 "flops_per_launch": flops / launches, "flops_per_step": scene.flops_per_step,

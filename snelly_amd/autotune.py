@@ -25,17 +25,17 @@ def time_render(sess, frames):
     return time.perf_counter() - t
 
 
-def tune(sess, frames=4, cu=256, verbose=False):
+def tune(sess, frames=4, cu=256, verbose=False, engines=(native.ENGINE_PERSISTENT, native.ENGINE_MEGAKERNEL)):
     prog = sess.prog
     space = {
-        "engine": [native.ENGINE_PERSISTENT, native.ENGINE_MEGAKERNEL],
+        "engine": list(engines),
         "tiles": [1, 2, 4],
         "blocks": [cu * 2, cu * 3, cu * 4, cu * 6, cu * 8],
         "thresh": [8, 16, 24, 32, 48],
         "chunk": [32, 64, 128, 256],
         "burst": [1, 2, 4, 8, 16],
     }
-    best = {"engine": native.ENGINE_PERSISTENT, "tiles": 1, "blocks": cu * 6, "thresh": 32, "chunk": 32, "burst": 4}
+    best = {"engine": engines[0], "tiles": 1, "blocks": cu * 6, "thresh": 32, "chunk": 32, "burst": 4}
 
     def run(cfg):
         prog.engine(cfg["engine"], cfg["tiles"], cfg["blocks"], cfg["thresh"], cfg["chunk"])

@@ -202,10 +202,17 @@ int compileProgram(snl_ctx ctx, const char* shader, const char* ior, const char*
     if (log && loglen > 0) snprintf(log, loglen, "%s", out.tr.error.c_str());
     return fail(ctx, SNL_E_COMPILE, "scene translation failed: %s", out.tr.error.c_str());
   }
+  // the fast build lets the scene's divisions become products with a 1-ulp
+  // reciprocal (within GLSL ES 3.00's 2.5-ulp bound for x / y), so a divisor that
+  // is constant along a march is inverted once; the RNG in the prelude stays
+  // correctly rounded
+  const bool fastDiv = (flags & SNL_BUILD_FAST) && !(flags & SNL_BUILD_PRECISE);
   std::ostringstream src;
   src << "#include \"snl_prelude.h\"\n#include \"snl_wavefront.h\"\n";
   src << defs << "\n";
+  if (fastDiv) src << "#pragma clang fp reciprocal(on)\n";
   src << "namespace glsl {\n" << out.tr.code << "\n}  // namespace glsl\n";
+  if (fastDiv) src << "#pragma clang fp reciprocal(off)\n";
   src << "#line 1 \"snl_pt.h\"\n#include \"snl_pt.h\"\n#include \"snl_kernels.h\"\n#include \"snl_persist.h\"\n#include \"snl_mega.h\"\n"
          "#include \"snl_modes.h\"\n";
   std::string source = src.str();
@@ -222,8 +229,7 @@ int compileProgram(snl_ctx ctx, const char* shader, const char* ior, const char*
   if (mode == SNL_BUILD_AO) opts.push_back("-DSNL_MODE_AO");
   if (mode == SNL_BUILD_PTSIMPLE) opts.push_back("-DSNL_MODE_PTSIMPLE");
   if (flags & SNL_BUILD_PRECISE) opts.push_back("-ffp-contract=off");
-  if ((flags & SNL_BUILD_FAST) && !(flags & SNL_BUILD_PRECISE))
-    opts.push_back("-fno-hip-fp32-correctly-rounded-divide-sqrt");
+  if (fastDiv) opts.push_back("-fno-hip-fp32-correctly-rounded-divide-sqrt");
   std::string key = source;
   for (auto& o : opts) key += "\n" + o;
   for (int i = 0; i < kNumEmbedded; ++i) key += kEmbeddedSources[i];

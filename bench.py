@@ -35,7 +35,7 @@ FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per pixel per launch of the queue kernels, from the FETCH_SIZE /
 # WRITE_SIZE counter runs scaled by the copy yardstick
 # (profiles/<kernel>_menger4_1024_traffic_valu.md)
-TRAFFIC_B_PER_PX = {"snl_persist": 67.1, "snl_mega": 67.0}
+TRAFFIC_B_PER_PX = {"snl_persist": 67.1, "snl_mega": 67.1}
 
 
 def unbiased(rad):

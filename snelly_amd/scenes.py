@@ -288,6 +288,7 @@ def batch_job(family, material, side, spp, bounces, rs):
         glsl, extent, flops = MANDELBOX_SDF_GLSL, None, None
     glsl = glsl.replace("@MAT@", material)
     s = Scene(f"{family}_{material.lower()}", glsl, side, side, spp)
+    mid = {"DIELECTRIC": 0, "METAL": 1, "SURFACE": 2}[material]
     r = s.renderer
     r.dispersive = True
     r.maxBounces = bounces
@@ -296,14 +297,21 @@ def batch_job(family, material, side, spp, bounces, rs):
     r.sunPower = -7.0
     s.camera.fov = 45.0
     s.camera.aperture = -math.inf
+    # the same SDFs restated in the oracle (material id, kind, parameters)
+    if family == "sphere":
+        s.oracle_sdfs = [(mid, 1, 1.0, 0, 0, 0)]
+    elif family == "torus":
+        s.oracle_sdfs = [(mid, 5, 1.0, 0.3, 0, 0)]
     if family.startswith("menger"):
         s.user_uniforms = {"size": 1.0, "scale": 1.0}
+        s.oracle_sdfs = [(mid, 3, 1.0, 1.0, float(family[-1]), 0)]
     if family == "mandelbox":
         # |scale| < 1.5 gives degenerate boxes; draw again
         sc = 0.0
         while abs(sc) < 1.5:
             sc = float(rs.uniform(-2.0, 2.5))
         s.user_uniforms = {"Scale": sc, "MinRad2": 0.25}
+        s.oracle_sdfs = [(mid, 4, sc, 12.0, 10.0, 0.25)]
         extent = 2.0 * (abs(sc) + 1.0) / (abs(sc) - 1.0) if sc > 0 else 2.0
         r.maxMarchSteps = 512
     d = 2.6 * extent

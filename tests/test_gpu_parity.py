@@ -142,6 +142,29 @@ def test_pathtrace_mandelbox_dielectric(ctx, engine):
     assert rrmse(unbiased(g), unbiased(r)) < 5e-2
 
 
+# the scene families of the batched and served workloads (scenes.batch_job), each
+# with the oracle's restatement of its SDF; the last Mandelbox has the negative
+# scale of served job 52 (scale -1.897, rough dielectric with Beer absorption)
+BATCH_FAMILY_CASES = [("sphere", "DIELECTRIC", None), ("torus", "METAL", None), ("torus", "SURFACE", None),
+                      ("menger3", "DIELECTRIC", None), ("menger5", "SURFACE", None), ("mandelbox", "METAL", None),
+                      ("mandelbox", "DIELECTRIC", -1.8974167769271963)]
+
+
+@pytest.mark.parametrize("engine", [native.ENGINE_PERSISTENT, native.ENGINE_MEGAKERNEL], ids=["persistent", "megakernel"])
+@pytest.mark.parametrize("family,material,scale", BATCH_FAMILY_CASES,
+                         ids=[f"{f}_{m.lower()}" for f, m, _ in BATCH_FAMILY_CASES])
+def test_batch_families_match_oracle(ctx, family, material, scale, engine):
+    """the batch and served families against the oracle, 40x40 at 4 spp,
+    6 bounces: per-pixel agreement where the paths stay the same"""
+    s = scenes.batch_job(family, material, 40, 4, 6, np.random.RandomState(3))
+    if scale is not None:
+        s.user_uniforms["Scale"] = scale
+        s.oracle_sdfs = [(0, 4, scale, 12.0, 10.0, 0.25)]
+    g, r, close, rng_same, _ = compare_render(ctx, s, 40, 40, 4, engine=engine)
+    assert close >= (0.8 if material == "DIELECTRIC" else 0.95), close
+    assert rng_same >= (0.8 if material == "DIELECTRIC" else 0.95), rng_same
+
+
 @pytest.mark.parametrize("engine", ENGINES, ids=ENGINE_IDS)
 def test_pathtrace_fast_build_statistics(ctx, engine):
     """the default FMA build against the oracle: contraction changes roundings,

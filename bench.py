@@ -36,6 +36,9 @@ FP32_PEAK_TFLOPS = 157.3
 # WRITE_SIZE counter runs scaled by the copy yardstick
 # (profiles/<kernel>_menger4_1024_traffic_valu.md)
 TRAFFIC_B_PER_PX = {"snl_persist": 67.1, "snl_mega": 67.1}
+# served: path samples per tick beyond each job's smallest quantum, given to the
+# jobs with the least remaining work first
+TICK_SAMPLES = 1 << 23
 
 
 def unbiased(rad):
@@ -119,12 +122,14 @@ def served(ctx, n_jobs):
     reaches its target (10-25%), by the continuous-batching server."""
     from snelly_amd import scenes, serve
     arrivals = scenes.served_jobs(n_jobs, target=(0.10, 0.25))
-    srv = serve.Server(ctx, quantum=(2, 16))
+    srv = serve.Server(ctx, quantum=(2, 256), tick_samples=TICK_SAMPLES)
     t = time.perf_counter()
     done, left = srv.run(arrivals, time_limit=90.0)
     wall = time.perf_counter() - t
     rep = serve.Server.report(done, wall, srv.samples)
     rep.update({"workload": "served_poisson20_mixed_sdf", "jobs": len(arrivals), "unfinished": len(left),
+                "unfinished_scenes": sorted({j.scene.name for j in left}),
+                "quantum_frames": [2, 256], "tick_samples": TICK_SAMPLES,
                 "arrival_rate_per_s": 20.0, "sides": [64, 512], "target_rrmse": [0.10, 0.25], "wall_s": round(wall, 2),
                 "ticks": srv.ticks, "gpu_busy_frac": round(srv.gpu_ms * 1e-3 / wall, 3),
                 "error_estimate": "spread of 2-32 independently seeded chains per job, reduced on the GPU"})

@@ -10,8 +10,12 @@ Each tick the server
   1. admits waiting jobs in arrival order while the device has room for their
      states (snl_ctx_mem_info, snl_state_bytes) and the active set is not full;
   2. renders one quantum of frames for every active job in one
-     snl_render_batch call -- a job's quantum is sized from its error trend,
-     frames_needed ~ frames_done * (error / target)^2;
+     snl_render_batch call.  A job wants the frames its error trend asks for,
+     frames_needed ~ frames_done * (error / target)^2, within (smallest,
+     largest); every job gets the smallest quantum, and the rest of the tick's
+     path-sample budget goes to the jobs with the least remaining work first
+     (so a cheap job that needs many frames takes them in a few ticks, and a
+     job whose error hardly falls does not hold back the others);
   3. retires the jobs whose estimate is at or below their target, recording
      the time from arrival to completion.
 Pixels never read each other's state, so a job's images do not depend on
@@ -59,23 +63,50 @@ class Job:
 class Server:
     """min_frames: frames per state before an error estimate counts;
     quantum: (smallest, largest) frames per state per tick;
+    tick_samples: path samples per tick beyond every job's smallest quantum
+    (None: no budget, each job takes what it wants);
     reserve: device bytes kept free; max_active: cap on concurrent jobs."""
 
     def __init__(self, ctx, flags=native.BUILD_FAST, min_frames=4, quantum=(2, 64), reserve=8 << 30,
-                 max_active=256, clock=time.perf_counter, keep_images=False):
+                 max_active=256, clock=time.perf_counter, keep_images=False, tick_samples=None):
         self.ctx, self.flags, self.keep_images = ctx, flags, keep_images
         self.min_frames, self.quantum, self.reserve, self.max_active = min_frames, quantum, reserve, max_active
+        self.tick_samples = tick_samples
         self.clock = clock
         self.ticks = 0
         self.samples = 0
         self.gpu_ms = 0.0
 
+    def _need(self, job):
+        """frames the job still needs by its error trend"""
+        if job.frames < self.min_frames:
+            return self.min_frames - job.frames
+        need = job.frames * (job.error / job.target) ** 2 if job.error < math.inf else 2 * job.frames
+        return max(0, math.ceil(need - job.frames))
+
     def _quantum(self, job):
+        """the frames the job wants this tick"""
         lo, hi = self.quantum
         if job.frames < self.min_frames:
             return max(lo, self.min_frames - job.frames)
-        need = job.frames * (job.error / job.target) ** 2 if job.error < math.inf else 2 * job.frames
-        return int(max(lo, min(hi, math.ceil(need - job.frames))))
+        return int(max(lo, min(hi, self._need(job))))
+
+    def _plan(self, jobs, samples_per_frame):
+        """frames per job for one tick: each job's wish (_quantum) when there is
+        no budget; otherwise the smallest quantum for every job (or its first
+        min_frames), then the budget to the least remaining work first"""
+        want = [self._quantum(j) for j in jobs]
+        if self.tick_samples is None:
+            return want
+        lo = self.quantum[0]
+        q = [w if j.frames < self.min_frames else min(lo, w) for j, w in zip(jobs, want)]
+        budget = self.tick_samples
+        for i in sorted(range(len(jobs)), key=lambda i: max(self._need(jobs[i]), 1) * samples_per_frame[i]):
+            extra = min(want[i] - q[i], budget // samples_per_frame[i])
+            if extra > 0:
+                q[i] += extra
+                budget -= extra * samples_per_frame[i]
+        return q
 
     def run(self, arrivals, time_limit=None):
         """arrivals: [(t, scene, target)] with t in seconds from the start,
@@ -107,11 +138,8 @@ class Server:
                 if waiting:
                     time.sleep(max(0.0, min(waiting[-1].arrival - (self.clock() - t0), 0.05)))
                 continue
-            batch, counts = [], []
-            for job in active:
-                q = self._quantum(job)
-                counts.append(q)
-                batch.append((job.a.prog, job.a.state, q))
+            counts = self._plan(active, [job.pixels * job.chains for job in active])
+            batch = [(job.a.prog, job.a.state, q) for job, q in zip(active, counts)]
             st = native.render_batch(batch)
             self.ticks += 1
             self.gpu_ms += st.ms_total

@@ -21,6 +21,21 @@ def test_quantum_follows_error_trend():
     assert s._quantum(FakeJob(64, 1.0, 0.1)) == 64             # capped
 
 
+def test_plan_budget_goes_to_least_remaining_work():
+    s = serve.Server.__new__(serve.Server)
+    s.min_frames, s.quantum, s.tick_samples = 4, (2, 256), 1000
+    new = FakeJob(0, math.inf, 0.1)          # before its first estimate: its min_frames
+    near = FakeJob(8, 0.11, 0.1)             # wants 2 frames (8 * 1.21 - 8 -> 2)
+    cheap = FakeJob(100, 0.2, 0.1)           # wants 256 (capped); remaining 300 x 1 samples
+    costly = FakeJob(100, 0.2, 0.1)          # same frames, 10 samples per frame: remaining 3000
+    jobs, spf = [new, near, cheap, costly], [10, 10, 1, 10]
+    q = s._plan(jobs, spf)
+    # base: 4 + 2 + 2 + 2 frames; cheap takes 254 more (254 samples), costly the rest (746 // 10 = 74)
+    assert q == [4, 2, 256, 76]
+    s.tick_samples = None
+    assert s._plan(jobs, spf) == [4, 2, 256, 256]
+
+
 def test_served_workload():
     a = scenes.served_jobs(50, seed=3)
     b = scenes.served_jobs(50, seed=3)
@@ -81,11 +96,13 @@ def test_chain_zero_is_the_single_chain_render(ctx):
 
 
 @pytest.mark.gpu
-def test_served_jobs_finish_and_match_alone(ctx):
+@pytest.mark.parametrize("tick_samples", [None, 1 << 16], ids=["no_budget", "budget"])
+def test_served_jobs_finish_and_match_alone(ctx, tick_samples):
     """a few small jobs served together reach their targets, and each job's
     images are what the job renders alone with the same seeds and frames"""
     arr = scenes.served_jobs(6, seed=11, rate=200.0, min_side=24, max_side=48, target=(0.2, 0.35))
-    srv = serve.Server(ctx, flags=native.BUILD_PRECISE, reserve=1 << 30, keep_images=True)
+    srv = serve.Server(ctx, flags=native.BUILD_PRECISE, reserve=1 << 30, keep_images=True, quantum=(2, 256),
+                       tick_samples=tick_samples)
     done, left = srv.run(arr, time_limit=120.0)
     assert not left and len(done) == 6
     for job in done:

@@ -37,8 +37,10 @@ FP32_PEAK_TFLOPS = 157.3
 # (profiles/<kernel>_menger4_1024_traffic_valu.md)
 TRAFFIC_B_PER_PX = {"snl_persist": 67.1, "snl_mega": 67.1}
 # served: path samples per tick beyond each job's smallest quantum, given to the
-# jobs with the least remaining work first
-TICK_SAMPLES = 1 << 23
+# jobs with the least remaining work first; chains per job: the fewest (up to
+# 128) that give it 2^18 lanes
+TICK_SAMPLES = 1 << 24
+CHAIN_LANES, MAX_CHAINS = 1 << 18, 128
 
 
 def unbiased(rad):
@@ -122,17 +124,19 @@ def served(ctx, n_jobs):
     reaches its target (10-25%), by the continuous-batching server."""
     from snelly_amd import scenes, serve
     arrivals = scenes.served_jobs(n_jobs, target=(0.10, 0.25))
-    srv = serve.Server(ctx, quantum=(2, 256), tick_samples=TICK_SAMPLES)
+    srv = serve.Server(ctx, quantum=(2, 256), tick_samples=TICK_SAMPLES, chain_lanes=CHAIN_LANES,
+                       max_chains=MAX_CHAINS)
     t = time.perf_counter()
     done, left = srv.run(arrivals, time_limit=90.0)
     wall = time.perf_counter() - t
     rep = serve.Server.report(done, wall, srv.samples)
     rep.update({"workload": "served_poisson20_mixed_sdf", "jobs": len(arrivals), "unfinished": len(left),
                 "unfinished_scenes": sorted({j.scene.name for j in left}),
-                "quantum_frames": [2, 256], "tick_samples": TICK_SAMPLES,
+                "quantum_frames": [2, 256], "tick_samples": TICK_SAMPLES, "chain_lanes": CHAIN_LANES,
+                "max_chains": MAX_CHAINS,
                 "arrival_rate_per_s": 20.0, "sides": [64, 512], "target_rrmse": [0.10, 0.25], "wall_s": round(wall, 2),
                 "ticks": srv.ticks, "gpu_busy_frac": round(srv.gpu_ms * 1e-3 / wall, 3),
-                "error_estimate": "spread of 2-32 independently seeded chains per job, reduced on the GPU"})
+                "error_estimate": "spread of 2-128 independently seeded chains per job, reduced on the GPU"})
     return rep
 
 

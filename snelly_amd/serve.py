@@ -43,15 +43,16 @@ class Job:
         return self.scene.width * self.scene.height
 
     @staticmethod
-    def chains_for(pixels, lanes=1 << 16):
+    def chains_for(pixels, lanes=1 << 16, most=32):
+        """the fewest chains (a power of two, 2..most) that give `lanes` lanes"""
         k = 2
-        while k < 32 and k * pixels < lanes:
+        while k < most and k * pixels < lanes:
             k *= 2
         return k
 
-    def open(self, ctx, flags, seed):
+    def open(self, ctx, flags, seed, lanes=1 << 16, most=32):
         s = self.scene
-        self.chains = self.chains_for(self.pixels)
+        self.chains = self.chains_for(self.pixels, lanes, most)
         self.a = host.Session(s, s.width, s.height, ctx, seed=seed, flags=flags, chains=self.chains)
 
     def close(self):
@@ -65,13 +66,18 @@ class Server:
     quantum: (smallest, largest) frames per state per tick;
     tick_samples: path samples per tick beyond every job's smallest quantum
     (None: no budget, each job takes what it wants);
+    chain_lanes, max_chains: a job gets the fewest chains (up to max_chains)
+    that give it chain_lanes lanes -- a chain's frames run one after another
+    in a lane, so more chains cut the time a job needs for many frames;
     reserve: device bytes kept free; max_active: cap on concurrent jobs."""
 
     def __init__(self, ctx, flags=native.BUILD_FAST, min_frames=4, quantum=(2, 64), reserve=8 << 30,
-                 max_active=256, clock=time.perf_counter, keep_images=False, tick_samples=None):
+                 max_active=256, clock=time.perf_counter, keep_images=False, tick_samples=None,
+                 chain_lanes=1 << 16, max_chains=32):
         self.ctx, self.flags, self.keep_images = ctx, flags, keep_images
         self.min_frames, self.quantum, self.reserve, self.max_active = min_frames, quantum, reserve, max_active
         self.tick_samples = tick_samples
+        self.chain_lanes, self.max_chains = chain_lanes, max_chains
         self.clock = clock
         self.ticks = 0
         self.samples = 0
@@ -126,11 +132,12 @@ class Server:
             free, _ = native.mem_info(self.ctx)
             while queue and len(active) < self.max_active:
                 job = queue[0]
-                need = native.state_bytes(job.scene.width, job.scene.height, Job.chains_for(job.pixels))
+                k = Job.chains_for(job.pixels, self.chain_lanes, self.max_chains)
+                need = native.state_bytes(job.scene.width, job.scene.height, k)
                 if need + self.reserve > free:
                     break
                 queue.pop(0)
-                job.open(self.ctx, self.flags, 1000003 * (job.id + 1))
+                job.open(self.ctx, self.flags, 1000003 * (job.id + 1), self.chain_lanes, self.max_chains)
                 job.admitted = self.clock() - t0
                 free -= need
                 active.append(job)

@@ -9,6 +9,7 @@
 #include <hip/hiprtc.h>
 
 #include <dlfcn.h>
+#include <elf.h>
 #include <sys/stat.h>
 
 #include <unistd.h>
@@ -169,6 +170,88 @@ struct Compiled {
   std::set<std::string> defines;
 };
 
+// Compile `source` with `opts` through hipRTC, or fetch the code object from
+// the on-disk cache (keyed by the source, the options and the embedded headers).
+static int compileCached(snl_ctx ctx, const std::string& source, const std::vector<std::string>& opts,
+                         std::vector<char>& code, char* log, int loglen) {
+  std::string key = source;
+  for (auto& o : opts) key += "\n" + o;
+  for (int i = 0; i < kNumEmbedded; ++i) key += kEmbeddedSources[i];
+  char name[64];
+  snprintf(name, sizeof name, "%016llx.hsaco", fnv1a(key));
+  std::string path = cacheDir() + "/" + name;
+  {
+    std::ifstream f(path, std::ios::binary);
+    if (f) {
+      code.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+      if (!code.empty()) return SNL_OK;
+    }
+  }
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, source.c_str(), "snl_program.hip", kNumEmbedded, kEmbeddedSources,
+                          kEmbeddedNames) != HIPRTC_SUCCESS)
+    return fail(ctx, SNL_E_COMPILE, "hiprtcCreateProgram failed");
+  std::vector<const char*> copts;
+  for (auto& o : opts) copts.push_back(o.c_str());
+  hiprtcResult r = hiprtcCompileProgram(prog, (int)copts.size(), copts.data());
+  size_t ls = 0;
+  hiprtcGetProgramLogSize(prog, &ls);
+  std::string plog(ls + 1, '\0');
+  if (ls) hiprtcGetProgramLog(prog, &plog[0]);
+  if (r != HIPRTC_SUCCESS) {
+    if (log && loglen > 0) snprintf(log, loglen, "%s", plog.c_str());
+    hiprtcDestroyProgram(&prog);
+    std::string head = plog.substr(0, 1500);
+    return fail(ctx, SNL_E_COMPILE, "hipRTC compile failed:\n%s", head.c_str());
+  }
+  size_t cs = 0;
+  hiprtcGetCodeSize(prog, &cs);
+  code.resize(cs);
+  hiprtcGetCode(prog, code.data());
+  hiprtcDestroyProgram(&prog);
+  std::string tmp = path + ".tmp" + std::to_string(getpid());
+  {
+    std::ofstream f(tmp, std::ios::binary);
+    f.write(code.data(), (std::streamsize)code.size());
+  }
+  rename(tmp.c_str(), path.c_str());
+  return SNL_OK;
+}
+
+// Private-segment (scratch) bytes per lane of a kernel in an AMDGPU code
+// object: the second field of its kernel descriptor, the symbol `<kernel>.kd`.
+// -1 when the object or the symbol cannot be read.
+static long kernelPrivateBytes(const std::vector<char>& code, const char* kdName) {
+  if (code.size() < sizeof(Elf64_Ehdr)) return -1;
+  const char* base = code.data();
+  Elf64_Ehdr eh;
+  memcpy(&eh, base, sizeof eh);
+  if (memcmp(eh.e_ident, ELFMAG, SELFMAG) != 0 || eh.e_ident[EI_CLASS] != ELFCLASS64) return -1;
+  if (eh.e_shoff + (size_t)eh.e_shnum * sizeof(Elf64_Shdr) > code.size()) return -1;
+  std::vector<Elf64_Shdr> sh(eh.e_shnum);
+  memcpy(sh.data(), base + eh.e_shoff, sh.size() * sizeof(Elf64_Shdr));
+  for (const Elf64_Shdr& t : sh) {
+    if ((t.sh_type != SHT_SYMTAB && t.sh_type != SHT_DYNSYM) || t.sh_link >= sh.size()) continue;
+    const Elf64_Shdr& str = sh[t.sh_link];
+    if (t.sh_offset + t.sh_size > code.size() || str.sh_offset + str.sh_size > code.size()) continue;
+    size_t n = t.sh_size / sizeof(Elf64_Sym);
+    for (size_t i = 0; i < n; ++i) {
+      Elf64_Sym sym;
+      memcpy(&sym, base + t.sh_offset + i * sizeof(Elf64_Sym), sizeof sym);
+      if (sym.st_name >= str.sh_size || sym.st_shndx == SHN_UNDEF || sym.st_shndx >= sh.size()) continue;
+      const char* nm = base + str.sh_offset + sym.st_name;
+      if (strncmp(nm, kdName, str.sh_size - sym.st_name) != 0) continue;
+      const Elf64_Shdr& sec = sh[sym.st_shndx];
+      size_t off = sec.sh_offset + (sym.st_value - sec.sh_addr) + 4;
+      if (sym.st_value < sec.sh_addr || off + 4 > code.size()) return -1;
+      uint32_t priv;
+      memcpy(&priv, base + off, 4);
+      return (long)priv;
+    }
+  }
+  return -1;
+}
+
 // Assemble the hipRTC source, compile (or fetch from the on-disk cache).
 int compileProgram(snl_ctx ctx, const char* shader, const char* ior, const char* defines, int maxMarch,
                    int maxBounces, int maxAtm, int spf, unsigned flags, Compiled& out, char* log, int loglen) {
@@ -230,48 +313,15 @@ int compileProgram(snl_ctx ctx, const char* shader, const char* ior, const char*
   if (mode == SNL_BUILD_PTSIMPLE) opts.push_back("-DSNL_MODE_PTSIMPLE");
   if (flags & SNL_BUILD_PRECISE) opts.push_back("-ffp-contract=off");
   if (fastDiv) opts.push_back("-fno-hip-fp32-correctly-rounded-divide-sqrt");
-  std::string key = source;
-  for (auto& o : opts) key += "\n" + o;
-  for (int i = 0; i < kNumEmbedded; ++i) key += kEmbeddedSources[i];
-  char name[64];
-  snprintf(name, sizeof name, "%016llx.hsaco", fnv1a(key));
-  std::string path = cacheDir() + "/" + name;
-  {
-    std::ifstream f(path, std::ios::binary);
-    if (f) {
-      out.code.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
-      if (!out.code.empty()) return SNL_OK;
-    }
+  int rc = compileCached(ctx, source, opts, out.code, log, loglen);
+  if (rc) return rc;
+  // The megakernel asks for four waves per SIMD (at most 128 VGPRs); where that
+  // makes it spill to scratch it is built again without the bound.
+  if (kernelPrivateBytes(out.code, "snl_mega.kd") > 0) {
+    opts.push_back("-DSNL_MEGA_WAVES=1");
+    rc = compileCached(ctx, source, opts, out.code, log, loglen);
   }
-  hiprtcProgram prog;
-  if (hiprtcCreateProgram(&prog, source.c_str(), "snl_program.hip", kNumEmbedded, kEmbeddedSources,
-                          kEmbeddedNames) != HIPRTC_SUCCESS)
-    return fail(ctx, SNL_E_COMPILE, "hiprtcCreateProgram failed");
-  std::vector<const char*> copts;
-  for (auto& o : opts) copts.push_back(o.c_str());
-  hiprtcResult r = hiprtcCompileProgram(prog, (int)copts.size(), copts.data());
-  size_t ls = 0;
-  hiprtcGetProgramLogSize(prog, &ls);
-  std::string plog(ls + 1, '\0');
-  if (ls) hiprtcGetProgramLog(prog, &plog[0]);
-  if (r != HIPRTC_SUCCESS) {
-    if (log && loglen > 0) snprintf(log, loglen, "%s", plog.c_str());
-    hiprtcDestroyProgram(&prog);
-    std::string head = plog.substr(0, 1500);
-    return fail(ctx, SNL_E_COMPILE, "hipRTC compile failed:\n%s", head.c_str());
-  }
-  size_t cs = 0;
-  hiprtcGetCodeSize(prog, &cs);
-  out.code.resize(cs);
-  hiprtcGetCode(prog, out.code.data());
-  hiprtcDestroyProgram(&prog);
-  std::string tmp = path + ".tmp" + std::to_string(getpid());
-  {
-    std::ofstream f(tmp, std::ios::binary);
-    f.write(out.code.data(), (std::streamsize)out.code.size());
-  }
-  rename(tmp.c_str(), path.c_str());
-  return SNL_OK;
+  return rc;
 }
 
 void fillArgs(snl_prog p, snl_state st, SnlKArgs& a) {

@@ -611,6 +611,12 @@ __device__ inline void mega_body(const SnlKArgs& a) {
   if (lane == 0 && ntot) atomicAdd(reinterpret_cast<unsigned long long*>(&a.cnt[C_NORMALS]), ntot);
 }
 
-extern "C" __global__ void __launch_bounds__(256) snl_mega(SnlKArgs a) { mega_body<SnlScene>(a); }
+// four waves per SIMD (at most 128 VGPRs) unless the runtime asks for fewer:
+// it builds the program again with SNL_MEGA_WAVES=1 when four make this kernel
+// spill to scratch
+#ifndef SNL_MEGA_WAVES
+#define SNL_MEGA_WAVES 4
+#endif
+extern "C" __global__ void __launch_bounds__(256, SNL_MEGA_WAVES) snl_mega(SnlKArgs a) { mega_body<SnlScene>(a); }
 
 }  // namespace glsl

@@ -37,7 +37,8 @@ extern "C" {
 /* program build flags */
 #define SNL_BUILD_PRECISE 1  /* no FMA contraction (-ffp-contract=off) */
 #define SNL_BUILD_FAST 2     /* approximate (1-ulp) float divide and sqrt outside the RNG; the scene
-                                code divides by multiplying with a 1-ulp reciprocal */
+                                code divides by multiplying with a 1-ulp reciprocal; subnormal
+                                results flush to zero */
 /* Renderer.renderMode (js/renderer.js:334-352): the program renders with the
  * ambient-occlusion integrator (shaders/ao-fragment-shader.glsl) or the
  * simplified path tracer (shaders/simplepathtracer-fragment-shader.glsl)

@@ -312,7 +312,12 @@ int compileProgram(snl_ctx ctx, const char* shader, const char* ior, const char*
   if (mode == SNL_BUILD_AO) opts.push_back("-DSNL_MODE_AO");
   if (mode == SNL_BUILD_PTSIMPLE) opts.push_back("-DSNL_MODE_PTSIMPLE");
   if (flags & SNL_BUILD_PRECISE) opts.push_back("-ffp-contract=off");
-  if (fastDiv) opts.push_back("-fno-hip-fp32-correctly-rounded-divide-sqrt");
+  if (fastDiv) {
+    opts.push_back("-fno-hip-fp32-correctly-rounded-divide-sqrt");
+    // subnormal results flush to zero (GLSL ES 3.00 lets highp flush them): no
+    // scaling around sqrt, reciprocals and divisions
+    opts.push_back("-fgpu-flush-denormals-to-zero");
+  }
   int rc = compileCached(ctx, source, opts, out.code, log, loglen);
   if (rc) return rc;
   // The megakernel asks for four waves per SIMD (at most 128 VGPRs); where that

@@ -35,7 +35,7 @@ FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per pixel per launch of the queue kernels, from the FETCH_SIZE /
 # WRITE_SIZE counter runs scaled by the copy yardstick
 # (profiles/<kernel>_menger4_1024_traffic_valu.md)
-TRAFFIC_B_PER_PX = {"snl_persist": 67.1, "snl_mega": 67.1}
+TRAFFIC_B_PER_PX = {"snl_persist": 67.1, "snl_mega": 67.3}
 # served: path samples per tick beyond each job's smallest quantum, given to the
 # jobs with the least remaining work first; chains per job: the fewest (up to
 # 128) that give it 2^18 lanes

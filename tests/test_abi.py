@@ -48,3 +48,46 @@ def test_scene_without_geometry_fails_like_the_reference():
     s = scenes.sphere_plane(8, 8, 1)
     with pytest.raises(native.SnellyError):
         host.assemble("float foo(vec3 X) { return 1.0; }", s.renderer, s.materials)
+
+
+def kernel_private_bytes(path, kd):
+    """the private-segment (scratch) bytes per lane in the kernel descriptor
+    `kd` of an AMDGPU code object (ELF64: the descriptor's second u32)"""
+    import struct
+    b = open(path, "rb").read()
+    assert b[:4] == b"\x7fELF" and b[4] == 2
+    shoff, = struct.unpack_from("<Q", b, 0x28)
+    shentsize, shnum = struct.unpack_from("<HH", b, 0x3A)
+    sh = [struct.unpack_from("<IIQQQQIIQQ", b, shoff + i * shentsize) for i in range(shnum)]
+    for name, typ, flags, addr, off, size, link, info, align, entsize in sh:
+        if typ not in (2, 11):  # SHT_SYMTAB, SHT_DYNSYM
+            continue
+        stroff = sh[link][4]
+        for k in range(size // 24):
+            st_name, st_info, st_other, st_shndx, st_value, st_size = struct.unpack_from("<IBBHQQ", b, off + 24 * k)
+            end = b.index(b"\0", stroff + st_name)
+            if b[stroff + st_name:end].decode() == kd:
+                sec = sh[st_shndx]
+                priv, = struct.unpack_from("<I", b, sec[4] + (st_value - sec[3]) + 4)
+                return priv
+    raise KeyError(kd)
+
+
+def test_megakernel_rebuilt_without_occupancy_bound_where_it_spills(tmp_path, monkeypatch):
+    """the megakernel asks for four waves per SIMD; a program whose megakernel
+    then spills (a Menger-4 dielectric batch job) is built again without the bound,
+    and the program keeps that scratch-free build; one that does not spill (the
+    Menger-4 benchmark scene) keeps the first build"""
+    import numpy as np
+    from snelly_amd import warm
+    monkeypatch.setenv("SNL_KCACHE", str(tmp_path / "spills"))
+    spills = scenes.batch_job("menger4", "DIELECTRIC", 32, 4, 6, np.random.RandomState(0))
+    native.precompile(*warm.program_key(spills, native.BUILD_FAST))
+    objs = sorted(tmp_path.glob("spills/*.hsaco"))
+    assert len(objs) == 2
+    assert sorted(kernel_private_bytes(str(p), "snl_mega.kd") for p in objs)[0] == 0
+    assert max(kernel_private_bytes(str(p), "snl_mega.kd") for p in objs) > 0
+    monkeypatch.setenv("SNL_KCACHE", str(tmp_path / "clean"))
+    native.precompile(*warm.program_key(scenes.menger_metal(64, 64, 4), native.BUILD_FAST))
+    objs = list(tmp_path.glob("clean/*.hsaco"))
+    assert len(objs) == 1 and kernel_private_bytes(str(objs[0]), "snl_mega.kd") == 0

@@ -139,10 +139,12 @@ int snl_set_profiling(snl_prog prog, int on);
  * per-bounce barrier) and a pixel's frames back to back; tiles run on separate
  * streams.  MEGAKERNEL: the same launch shape, every lane running cameraPath
  * as written, shadow and probe rays traced in place.  All draw the same random
- * numbers per path.  Programs with the homogeneous atmosphere (HAS_ATMOSPHERE)
- * always render with the megakernel; those whose surface has sub-surface
- * scattering use the wavefront engine unless the megakernel is chosen (and the
- * megakernel in snl_render_batch).  0 keeps a knob. */
+ * numbers per path.  A program starts with the megakernel.  Programs with the
+ * homogeneous atmosphere (HAS_ATMOSPHERE) always render with the megakernel;
+ * those whose surface has sub-surface scattering use the wavefront engine when
+ * the persistent one is chosen.  snl_render_batch runs the megakernel, or the
+ * persistent kernel where that is chosen and the program can use it.  0 keeps
+ * a knob. */
 #define SNL_ENGINE_WAVEFRONT 0
 #define SNL_ENGINE_PERSISTENT 1
 #define SNL_ENGINE_MEGAKERNEL 2
@@ -153,7 +155,7 @@ int snl_set_march_burst(snl_prog prog, int steps);
 
 /* Batched rendering: frames[j] frames of every job j = (progs[j], states[j]),
  * each with its program's current uniforms, submitted together -- the jobs'
- * persistent launches are spread over the context's batch streams
+ * launches are spread over the context's batch streams
  * (SNL_BATCH_STREAMS, default 16), longest first, so that many small images
  * fill the device at once; they run concurrently up to the process's hardware
  * queue count (GPU_MAX_HW_QUEUES).  The reference renders

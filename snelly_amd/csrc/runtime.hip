@@ -79,7 +79,7 @@ struct snl_prog_s {
   hipFunction_t k_reset, k_rotate, k_camera, k_trace, k_bin, k_miss, k_surf, k_metal, k_diele, k_accum, k_normals,
       k_pick, k_tonemap, k_persist, k_persist_reset, k_mega, k_mode = nullptr;
   unsigned mode = 0;  // SNL_BUILD_AO or SNL_BUILD_PTSIMPLE, else 0 (the path tracer)
-  int engine = SNL_ENGINE_PERSISTENT;
+  int engine = SNL_ENGINE_MEGAKERNEL;
   int tiles = 2, persistBlocks = 0, shadeThresh = 32, persistChunk = 64, marchBurst = 4;
   bool hasSurface = false, hasMetal = false, hasDiele = false, hasAtmosphere = false;
   int megaBlocks = 0, modeBlocks = 0;
@@ -1016,17 +1016,18 @@ unsigned long long snl_state_bytes(int width, int height, int chains) {
          (unsigned long long)(SNL_MAX_TILES + 1) * C_COUNT * 4ull;
 }
 
-// batched jobs run the persistent kernel, or the megakernel where that one
-// lacks a feature (the sub-surface walk, the atmosphere) or it is asked for;
-// AO and simplified programs their own kernel
+// batched jobs run the megakernel, or the persistent kernel where that is
+// chosen and the program needs neither the sub-surface walk nor the atmosphere
+// (the wavefront engine's per-bounce launches are not batched: its jobs run the
+// megakernel); AO and simplified programs their own kernel
 static hipFunction_t queueKernel(snl_prog p, int& blocks) {
   if (p->mode) { blocks = p->modeBlocks; return p->k_mode; }
-  if (p->hasAtmosphere || p->engine == SNL_ENGINE_MEGAKERNEL || p->hostP.subsurfaceMFP > 0.0f) {
-    blocks = p->megaBlocks;
-    return p->k_mega;
+  if (p->engine == SNL_ENGINE_PERSISTENT && !p->hasAtmosphere && !(p->hostP.subsurfaceMFP > 0.0f)) {
+    blocks = p->persistBlocks;
+    return p->k_persist;
   }
-  blocks = p->persistBlocks;
-  return p->k_persist;
+  blocks = p->megaBlocks;
+  return p->k_mega;
 }
 
 int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, const int* frames,

@@ -472,19 +472,32 @@ class Camera:
         self.focalDistance = 1.0
 
     def basis(self):
-        """camDir / camX / camY as js/renderer.js:520-527 derives them from the
-        camera's lookAt frame: z = normalize(pos - target), x = normalize(up x z),
-        y = z x x, camDir = -z, camY = y, camX = camY x camDir."""
+        """camDir / camX / camY as js/renderer.js:520-527 derives them (in
+        doubles, as JavaScript does).  The camera's rotation R = [x y z] is the
+        lookAt frame of three.js r79 (js/thirdparty/three/three.js:5105-5150):
+        z = normalize(pos - target), x = normalize(up x z), y = z x x.
+        camDir = getWorldDirection() = R (0, 0, -1) = -z; camY is `up` carried
+        through R and normalised (Vector3.transformDirection, 2292-2306), which
+        is y only for up = (0, 1, 0); camX = camY x camDir."""
+        def unit(v):
+            # Vector3.normalize (2460-2464): times 1 / length, NaN for a zero
+            # vector -- so three.js's zero-length fallbacks below never fire
+            n = math.sqrt(float(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]))
+            return v * (1.0 / n if n > 0.0 else math.inf)
         p = np.asarray(self.position, np.float64)
         t = np.asarray(self.target, np.float64)
         up = np.asarray(self.up, np.float64)
-        z = p - t
-        z /= np.linalg.norm(z)
-        x = np.cross(up, z)
-        x /= np.linalg.norm(x)
+        with np.errstate(invalid="ignore"):
+            z = unit(p - t)
+            if float(z @ z) == 0.0:
+                z[2] = 1.0
+            x = unit(np.cross(up, z))
+            if float(x @ x) == 0.0:
+                z[2] += 0.0001
+                x = unit(np.cross(up, z))
         y = np.cross(z, x)
         camDir = -z
-        camY = y
+        camY = unit(up[0] * x + up[1] * y + up[2] * z)
         camX = np.cross(camY, camDir)
         return p, camDir, camX, camY
 

@@ -49,16 +49,30 @@ class PerspectiveCamera {
   }
   lookAt(v) { this.target.copy(v); }
   updateProjectionMatrix() {}
-  // the lookAt frame (three.js Matrix4.lookAt): z = normalize(pos - target),
-  // x = normalize(up x z), y = z x x; the renderer uses camDir = -z, camY = y
-  // and camX = camY x camDir (js/renderer.js:520-527)
+  // The camera's rotation R = [x y z] is three.js r79's lookAt frame
+  // (js/thirdparty/three/three.js:5105-5150): z = normalize(pos - target),
+  // x = normalize(up x z), y = z x x.  The renderer (js/renderer.js:520-527) takes
+  // camDir = R (0, 0, -1) = -z, camY = `up` carried through R and normalised
+  // (Vector3.transformDirection), which is y only for up = (0, 1, 0), and
+  // camX = camY x camDir.
   basis() {
-    const z = this.position.clone().sub(this.target).normalize();
-    const x = new Vector3().crossVectors(this.up, z).normalize();
+    // Vector3.normalize of three.js r79: times 1 / length, NaN for a zero vector
+    // (so its zero-length fallbacks in lookAt never fire)
+    const unit = (v) => v.multiplyScalar(1 / v.length());
+    const z = unit(this.position.clone().sub(this.target));
+    if (z.dot(z) === 0) z.z = 1;
+    let x = unit(new Vector3().crossVectors(this.up, z));
+    if (x.dot(x) === 0) {
+      z.z += 0.0001;
+      x = unit(new Vector3().crossVectors(this.up, z));
+    }
     const y = new Vector3().crossVectors(z, x);
     const dir = z.clone().multiplyScalar(-1);
-    const camX = new Vector3().crossVectors(y, dir);
-    return { pos: this.position.clone(), dir, camX, camY: y };
+    const u = this.up;
+    const camY = unit(new Vector3(u.x * x.x + u.y * y.x + u.z * z.x, u.x * x.y + u.y * y.y + u.z * z.y,
+                                  u.x * x.z + u.y * y.z + u.z * z.z));
+    const camX = new Vector3().crossVectors(camY, dir);
+    return { pos: this.position.clone(), dir, camX, camY };
   }
   getWorldDirection() { return this.basis().dir; }
 }

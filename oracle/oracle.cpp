@@ -1828,7 +1828,42 @@ int oracle_pick(const float* xy, int n, float* out) {
   return 0;
 }
 
-// tonemapper (tonemapper-fragment-shader.glsl:36-97) without the hue shift
+// GLSL ES 3.00 builtins as defined: step(edge, x) = x < edge ? 0 : 1,
+// mix(x, y, a) = x (1 - a) + y a, fract(x) = x - floor(x), min(x, y) = y < x ? y : x
+static float glStep(float edge, float x) { return x < edge ? 0.0f : 1.0f; }
+static float glMix(float x, float y, float a) { return x * (1.0f - a) + y * a; }
+static float glFract(float x) { return x - std::floor(x); }
+static float glMin(float x, float y) { return y < x ? y : x; }
+
+// rgb2hsv (tonemapper-fragment-shader.glsl:19-27)
+static void rgb2hsv(const float c[3], float hsv[3]) {
+  const float K[4] = {0.0f, -1.0f / 3.0f, 2.0f / 3.0f, -1.0f};
+  float t = glStep(c[2], c[1]);
+  const float a[4] = {c[2], c[1], K[3], K[2]}, b[4] = {c[1], c[2], K[0], K[1]};
+  float p[4];
+  for (int k = 0; k < 4; ++k) p[k] = glMix(a[k], b[k], t);
+  float u = glStep(p[0], c[0]);
+  const float a2[4] = {p[0], p[1], p[3], c[0]}, b2[4] = {c[0], p[1], p[2], p[0]};
+  float q[4];
+  for (int k = 0; k < 4; ++k) q[k] = glMix(a2[k], b2[k], u);
+  float d = q[0] - glMin(q[3], q[1]);
+  float e = 1.0e-10f;
+  hsv[0] = std::fabs(q[2] + (q[3] - q[1]) / (6.0f * d + e));
+  hsv[1] = d / (q[0] + e);
+  hsv[2] = q[0];
+}
+
+// hsv2rgb (tonemapper-fragment-shader.glsl:29-34)
+static void hsv2rgb(const float c[3], float rgb[3]) {
+  const float K[4] = {1.0f, 2.0f / 3.0f, 1.0f / 3.0f, 3.0f};
+  for (int k = 0; k < 3; ++k) {
+    float p = std::fabs(glFract(c[0] + K[k]) * 6.0f - K[3]);
+    float cl = std::fmin(std::fmax(p - K[0], 0.0f), 1.0f);
+    rgb[k] = c[2] * glMix(K[0], cl, c[1]);
+  }
+}
+
+// tonemapper (tonemapper-fragment-shader.glsl:36-97)
 int oracle_tonemap(const float* radiance, long long npix, float* rgb) {
   auto tm = [](float L) { return L / (1.0f + L); };
   for (long long i = 0; i < npix; ++i) {
@@ -1843,6 +1878,13 @@ int oracle_tonemap(const float* radiance, long long npix, float* rgb) {
     float mean = (c[0] + c[1] + c[2]) / 3.0f;
     for (float& v : c) { float dv = v - mean; v = mean + fsign(dv) * std::pow(std::fabs(dv), 1.0f / P.saturation); }
     for (float& v : c) { float dv = v - 0.5f; v = 0.5f + fsign(dv) * std::pow(std::fabs(dv), 1.0f / P.contrast); }
+    if (P.hueShift > 0.0f) {  // (85-91)
+      float hsv[3];
+      rgb2hsv(c, hsv);
+      float h = hsv[0] + P.hueShift;
+      hsv[0] = h - 1.0f * std::floor(h / 1.0f);  // mod(h, 1.0)
+      hsv2rgb(hsv, c);
+    }
     rgb[3 * i] = c[0]; rgb[3 * i + 1] = c[1]; rgb[3 * i + 2] = c[2];
   }
   return 0;

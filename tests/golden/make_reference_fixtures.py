@@ -342,10 +342,11 @@ def uniform_lines(u):
     return out
 
 
-def run_reference(scene, W, H, frames, seed, td, name="case", program="pathtracer"):
+def run_reference(scene, W, H, frames, seed, td, name="case", program="pathtracer", radiance=None):
     """render `frames` frames of `scene` with the reference's `program`
-    (pathtracer or normals), or its pick program once per pixel; returns
-    (radiance, rng, gl_info, seconds)"""
+    (pathtracer or normals), or its pick program once per pixel, from a fresh
+    state or from `radiance` (the tonemapper reads the pathtracer's result);
+    returns (radiance, rng, gl_info, seconds)"""
     scene.camera.aspect = W / H
     u = host.frame_uniforms(scene.renderer, scene.camera, scene.scales, scene.materials, W, H)
     env = getattr(scene, "env_map", None)
@@ -359,7 +360,7 @@ def run_reference(scene, W, H, frames, seed, td, name="case", program="pathtrace
     paths = {k: os.path.join(td, f"{name}_{k}") for k in ("vs", "fs", "rad", "rng", "rad_out", "rng_out", "spec")}
     open(paths["vs"], "w").write(vs)
     open(paths["fs"], "w").write(fs)
-    rad = np.tile(np.float32([0, 0, 0, 1]), (W * H, 1))
+    rad = np.tile(np.float32([0, 0, 0, 1]), (W * H, 1)) if radiance is None else np.float32(radiance)
     rng = oracle_py.Oracle.seed(W * H, seed).astype(np.float32)
     rad.tofile(paths["rad"])
     rng.tofile(paths["rng"])
@@ -394,6 +395,13 @@ def run_reference(scene, W, H, frames, seed, td, name="case", program="pathtrace
     return out_rad, out_rng, (gl_info[0][4:] if gl_info else ""), time.time() - t0
 
 
+# tonemapper settings (Renderer's exposure, gamma, contrast, saturation, hueShift),
+# one per case in turn: the defaults and two others
+TONEMAP_SETTINGS = [dict(exposure=0.0, gamma=2.2, contrast=1.0, saturation=1.0, hueShift=0.0),
+                    dict(exposure=0.8, gamma=2.0, contrast=1.3, saturation=1.4, hueShift=0.25),
+                    dict(exposure=-1.5, gamma=2.4, contrast=0.8, saturation=0.6, hueShift=-0.4)]
+
+
 def make(name, td):
     scene, W, H, frames, seed = reference_cases.build(name)
     # Renderer.compileShaders' program per renderMode (js/renderer.js:334-352)
@@ -402,11 +410,18 @@ def make(name, td):
     # first-hit passes: the pick program per pixel, one frame of the normals program
     pick, _, _, _ = run_reference(scene, W, H, 1, seed, td, name, "pick")
     nrm_rad, nrm_rng, _, _ = run_reference(scene, W, H, 1, seed, td, name, "normals")
+    # the tonemapper over the final radiance (js/renderer.js:622-642), read back
+    # as floats before the canvas's 8-bit store
+    tm = TONEMAP_SETTINGS[list(reference_cases.CASES).index(name) % len(TONEMAP_SETTINGS)]
+    for k, v in tm.items():
+        setattr(scene.renderer, k, v)
+    tone, _, _, _ = run_reference(scene, W, H, 1, seed, td, name, "tonemapper", radiance=out_rad)
     meta = dict(case=name, width=W, height=H, frames=frames, seed=seed, gl=gl, seconds=round(secs, 2),
                 shaders=[f"shaders/{program}-fragment-shader.glsl", "shaders/pick-fragment-shader.glsl",
-                         "shaders/normals-fragment-shader.glsl"])
+                         "shaders/normals-fragment-shader.glsl", "shaders/tonemapper-fragment-shader.glsl"],
+                tonemap=tm)
     np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), radiance=out_rad, rng=out_rng,
-                        pick=pick[:, :2].copy(), normals=nrm_rad, normals_rng=nrm_rng,
+                        pick=pick[:, :2].copy(), normals=nrm_rad, normals_rng=nrm_rng, tonemap=tone.copy(),
                         meta=np.frombuffer(json.dumps(meta).encode(), np.uint8))
     print(json.dumps(meta))
 

@@ -169,3 +169,51 @@ def test_gpu_first_hit_matches_reference(ctx, name):
     nrng = sess.state.read(1).reshape(-1, 4)
     sess.close()
     check_normals(name, nrad, nrng, ref)
+
+
+def tonemap_scene(name, meta):
+    """the case's scene with the tonemapper settings its fixture was made with"""
+    s, W, H, _, _ = reference_cases.build(name)
+    for k, v in meta["tonemap"].items():
+        setattr(s.renderer, k, v)
+    return s, W, H
+
+
+def check_tonemap(rgb, rgba8, ref):
+    """float output within 1e-5 of the pixel's largest channel (the hue
+    rotation mixes channels) plus 1e-6 (a channel that the out-of-gamut shift
+    sets to 0 stays 0 here and comes out 2^-25 from the reference's inexact
+    pow(x, 1.0)); the 8-bit store off by at most one step, and only where the
+    float sits at a rounding boundary"""
+    t = ref["tonemap"][:, :3]
+    scale = np.abs(t).max(axis=1, keepdims=True)
+    assert np.all(np.abs(rgb - t) <= 1e-5 * scale + 1e-6)
+    if rgba8 is not None:
+        want = np.floor(np.clip(t, 0.0, 1.0) * 255.0 + 0.5)
+        diff = np.abs(rgba8.astype(np.int32) - want)
+        assert diff.max() <= 1 and np.mean(diff > 0) <= 1e-3
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_oracle_tonemap_matches_reference(name):
+    """the oracle's tonemap over the fixture's radiance against the reference's
+    tonemapper-fragment-shader.glsl (three settings of exposure, gamma,
+    contrast, saturation and hue shift across the cases)"""
+    import oracle_py
+    ref, meta = fixture(name)
+    s, W, H = tonemap_scene(name, meta)
+    o = oracle_py.Oracle(s, W, H)
+    check_tonemap(o.tonemap(ref["radiance"]), None, ref)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", CASES)
+def test_gpu_tonemap_matches_reference(ctx, name):
+    from snelly_amd import host, native
+    ref, meta = fixture(name)
+    s, W, H = tonemap_scene(name, meta)
+    sess = host.Session(s, W, H, ctx, seed=1, flags=native.BUILD_PRECISE)
+    sess.state.write(0, ref["radiance"])
+    rgb, rgba8 = sess.prog.tonemap(sess.state)
+    sess.close()
+    check_tonemap(rgb.reshape(-1, 3), rgba8[..., :3].reshape(-1, 3), ref)

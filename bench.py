@@ -79,10 +79,17 @@ def batched(ctx, steps, warmup):
     """The batched workload (64 seeded scenes, scenes.batch_jobs): the same jobs
     rendered plain -- one scene per launch and one launch per frame, job after
     job, as the reference draws -- and as one snl_render_batch call."""
-    from snelly_amd import host, native, scenes
+    from snelly_amd import autotune, host, native, scenes
     jobs = scenes.batch_jobs()
     sessions = [host.Session(s, s.width, s.height, ctx, seed=500 + j, flags=native.BUILD_FAST)
                 for j, s in enumerate(jobs)]
+    # each job's engine from a short trial of both alone (untimed); the plain
+    # and the batched runs then use the same programs and engines
+    chosen = {}
+    for j, (se, s) in enumerate(zip(sessions, jobs)):
+        e = autotune.pick_engine(se, min(4, s.spp))
+        chosen[autotune.ENGINE_NAMES[e]] = chosen.get(autotune.ENGINE_NAMES[e], 0) + 1
+        se.state.seed(500 + j)
     samples = sum(s.width * s.height * s.spp for s in jobs)
     ctx.sync()
     t = time.perf_counter()
@@ -115,7 +122,8 @@ def batched(ctx, steps, warmup):
             "plain_path_samples_per_s": round(samples / plain_s, 1), "plain_s": round(plain_s, 3),
             "batched_path_samples_per_s": round(samples / batch_s, 1), "batched_s": round(batch_s, 3),
             "batched_gpu_ms": round(sum(gpu_ms) / len(gpu_ms), 2), "speedup_vs_plain": round(plain_s / batch_s, 2),
-            "build": "fast", "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")), "jobs_by_scene": fams}
+            "build": "fast", "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")), "engines": chosen,
+            "jobs_by_scene": fams}
 
 
 def served(ctx, n_jobs):

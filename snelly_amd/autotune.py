@@ -62,6 +62,23 @@ def tune(sess, frames=4, cu=256, verbose=False, engines=(native.ENGINE_PERSISTEN
     return best, best_t, trials
 
 
+def pick_engine(sess, frames=4, engines=(native.ENGINE_MEGAKERNEL, native.ENGINE_PERSISTENT)):
+    """the engine alone (default launch shape) that renders `frames` frames of
+    the session's job fastest; set on its program and returned.  For the many
+    jobs of a batch, where a full tune() per job would cost more than the
+    batch: heavy fractal metals favour the persistent kernel, most other jobs
+    the megakernel.  The caller re-seeds the state afterwards."""
+    best, best_t = engines[0], None
+    for e in engines:
+        sess.prog.engine(e)
+        time_render(sess, 1)  # load the kernel, warm the caches
+        t = time_render(sess, frames)
+        if best_t is None or t < best_t:
+            best, best_t = e, t
+    sess.prog.engine(best)
+    return best
+
+
 ENGINE_NAMES = {native.ENGINE_WAVEFRONT: "wavefront", native.ENGINE_PERSISTENT: "persistent",
                 native.ENGINE_MEGAKERNEL: "megakernel"}
 KERNEL_NAMES = {native.ENGINE_PERSISTENT: "snl_persist", native.ENGINE_MEGAKERNEL: "snl_mega"}

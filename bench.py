@@ -84,11 +84,15 @@ def batched(ctx, steps, warmup):
     sessions = [host.Session(s, s.width, s.height, ctx, seed=500 + j, flags=native.BUILD_FAST)
                 for j, s in enumerate(jobs)]
     # each job's engine from a short trial of both alone (untimed); the plain
-    # and the batched runs then use the same programs and engines
-    chosen = {}
+    # and the batched runs then use the same programs and engines.  (Tuning the
+    # whole launch shape of the heaviest jobs as well sped the plain run up more
+    # than the batch.)
     for j, (se, s) in enumerate(zip(sessions, jobs)):
-        e = autotune.pick_engine(se, min(4, s.spp))
-        chosen[autotune.ENGINE_NAMES[e]] = chosen.get(autotune.ENGINE_NAMES[e], 0) + 1
+        autotune.pick_engine(se, min(4, s.spp))
+    chosen = {}
+    for j, se in enumerate(sessions):
+        name = autotune.ENGINE_NAMES[se.prog.engine_id]
+        chosen[name] = chosen.get(name, 0) + 1
         se.state.seed(500 + j)
     samples = sum(s.width * s.height * s.spp for s in jobs)
     ctx.sync()

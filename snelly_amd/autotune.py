@@ -25,11 +25,14 @@ def time_render(sess, frames):
     return time.perf_counter() - t
 
 
-def tune(sess, frames=4, cu=256, verbose=False, engines=(native.ENGINE_PERSISTENT, native.ENGINE_MEGAKERNEL)):
+def tune(sess, frames=4, cu=256, verbose=False, engines=(native.ENGINE_PERSISTENT, native.ENGINE_MEGAKERNEL),
+         tiles=(1, 2, 4)):
+    """coordinate descent over the knobs; `tiles` (1) for jobs rendered by
+    snl_render_batch, which launches one kernel per job"""
     prog = sess.prog
     space = {
         "engine": list(engines),
-        "tiles": [1, 2, 4],
+        "tiles": list(tiles),
         "blocks": [cu * 2, cu * 3, cu * 4, cu * 6, cu * 8],
         "thresh": [8, 16, 24, 32, 48],
         "chunk": [32, 64, 128, 256],
@@ -64,7 +67,7 @@ def tune(sess, frames=4, cu=256, verbose=False, engines=(native.ENGINE_PERSISTEN
 
 def pick_engine(sess, frames=4, engines=(native.ENGINE_MEGAKERNEL, native.ENGINE_PERSISTENT)):
     """the engine alone (default launch shape) that renders `frames` frames of
-    the session's job fastest; set on its program and returned.  For the many
+    the session's job fastest; set on its program and returned with its time.  For the many
     jobs of a batch, where a full tune() per job would cost more than the
     batch: heavy fractal metals favour the persistent kernel, most other jobs
     the megakernel.  The caller re-seeds the state afterwards."""
@@ -76,7 +79,7 @@ def pick_engine(sess, frames=4, engines=(native.ENGINE_MEGAKERNEL, native.ENGINE
         if best_t is None or t < best_t:
             best, best_t = e, t
     sess.prog.engine(best)
-    return best
+    return best, best_t
 
 
 ENGINE_NAMES = {native.ENGINE_WAVEFRONT: "wavefront", native.ENGINE_PERSISTENT: "persistent",

@@ -232,6 +232,7 @@ class Program:
             raise SnellyError(f"program build failed ({rc}): {lib().snl_last_error(ctx.h).decode()}\n"
                               f"{log.value.decode()[:4000]}")
         self.h = h
+        self.engine_id = ENGINE_MEGAKERNEL  # a program starts with the megakernel
 
     def index(self, name):
         return lib().snl_uniform_index(self.h, name.encode())
@@ -289,6 +290,7 @@ class Program:
 
     def engine(self, engine, tiles=0, persist_blocks=0, shade_thresh=0, chunk=0):
         self.ctx.check(lib().snl_set_engine(self.h, engine, tiles, persist_blocks, shade_thresh, chunk))
+        self.engine_id = engine
 
     def profiling(self, on):
         self.ctx.check(lib().snl_set_profiling(self.h, int(on)))

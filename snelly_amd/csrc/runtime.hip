@@ -71,6 +71,8 @@ struct snl_state_s {
   int* cnt = nullptr;  // C_COUNT ints
   int* tileCnt = nullptr;  // SNL_MAX_TILES * C_COUNT ints (persistent engine)
   float* radPrev = nullptr;  // radiance as of the frame start, for samplers left on texture unit 0
+  int* tileCost = nullptr;   // per 8x8 tile: march steps of the render's first frame, then the
+                             // cost-bucket histogram and the tile order (ordered megakernel renders)
 };
 
 struct snl_prog_s {
@@ -80,6 +82,7 @@ struct snl_prog_s {
       k_pick, k_tonemap, k_persist, k_persist_reset, k_mega, k_mode = nullptr;
   unsigned mode = 0;  // SNL_BUILD_AO or SNL_BUILD_PTSIMPLE, else 0 (the path tracer)
   int engine = SNL_ENGINE_MEGAKERNEL;
+  bool orderTiles = true;  // megakernel renders take tiles costliest first (SNL_TILE_ORDER=0: queue order)
   int tiles = 2, persistBlocks = 0, shadeThresh = 32, persistChunk = 64, marchBurst = 4;
   bool hasSurface = false, hasMetal = false, hasDiele = false, hasAtmosphere = false;
   int megaBlocks = 0, modeBlocks = 0;
@@ -508,6 +511,7 @@ int snl_program_build(snl_ctx ctx, const char* shader, const char* ior, const ch
   }
   HIPCHK(ctx, hipMalloc((void**)&p->dP, sizeof(SnlParams)));
   HIPCHK(ctx, hipMalloc(&p->dU, p->hostU.size()));
+  if (const char* e = getenv("SNL_TILE_ORDER")) p->orderTiles = atoi(e) != 0;
   p->traceBlocks = ctx->numCU * 8;
   p->traceChunk = 64;
   p->shadeBlocks = ctx->numCU * 8;
@@ -694,7 +698,7 @@ int snl_state_destroy(snl_state s) {
   hipSetDevice(s->ctx->device);
   hipStreamSynchronize(s->ctx->stream);
   for (void* p : {(void*)s->radiance, (void*)s->rng, (void*)s->pf, (void*)s->pi, (void*)s->q, (void*)s->cnt,
-                  (void*)s->tileCnt, (void*)s->radPrev})
+                  (void*)s->tileCnt, (void*)s->radPrev, (void*)s->tileCost})
     if (p) hipFree(p);
   delete s;
   return SNL_OK;
@@ -905,6 +909,41 @@ static int renderWavefront(snl_prog p, snl_state st, int frames) {
 // tiles run on their own streams so one tile's tail overlaps another's work.
 // The megakernel (snl_mega) and the AO / simplified integrators
 // (snl_mode_kernel) launch the same way.
+// Tile order of a megakernel render: its first frame runs as a launch of its
+// own that adds each pixel's march steps to its 8x8 tile; the tiles are then
+// sorted by that cost, most first (a counting sort over log-spaced buckets,
+// four per doubling), and the remaining frames take them in that order, so the
+// longest pixels start first and the launch does not end on a few of them.
+// Each pixel still runs its frames in order in one lane: results do not change.
+constexpr int kCostBuckets = 128;
+__device__ inline int costBucket(int c) {
+  int b = (int)(log2f((float)c + 1.0f) * 4.0f);
+  b = b < 0 ? 0 : (b >= kCostBuckets ? kCostBuckets - 1 : b);
+  return kCostBuckets - 1 - b;  // costliest first
+}
+__global__ void tile_zero_kernel(int* p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+__global__ void tile_hist_kernel(const int* cost, int n, int* hist) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(&hist[costBucket(cost[i])], 1);
+}
+__global__ void tile_scan_kernel(int* hist) {  // exclusive prefix sum, in place
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    int run = 0;
+    for (int b = 0; b < kCostBuckets; ++b) {
+      int h = hist[b];
+      hist[b] = run;
+      run += h;
+    }
+  }
+}
+__global__ void tile_scatter_kernel(const int* cost, int n, int* cursor, int* order) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) order[atomicAdd(&cursor[costBucket(cost[i])], 1)] = i;
+}
+
 static int renderPersistent(snl_prog p, snl_state st, int frames, hipFunction_t kern, int blocksAll) {
   snl_ctx ctx = p->ctx;
   bool unit0 = false;
@@ -930,6 +969,14 @@ static int renderPersistent(snl_prog p, snl_state st, int frames, hipFunction_t 
   HIPCHK(ctx, hipEventRecord(p->ev0, ctx->stream));
   HIPCHK(ctx, hipEventRecord(ctx->joinEvent, ctx->stream));
   int blocks = std::max(1, blocksAll / K);
+  // megakernel renders of one 8x8-tiled image on one stream take their tiles
+  // costliest first after the first frame (see tile_scatter_kernel)
+  const bool ordered = kern == p->k_mega && K == 1 && !unit0 && frames >= 2 && st->chains == 1 &&
+                       st->w % 8 == 0 && st->h % 8 == 0 && p->orderTiles;
+  if (ordered && !st->tileCost) {
+    size_t nt = (size_t)(st->w >> 3) * (st->h >> 3);
+    HIPCHK(ctx, hipMalloc((void**)&st->tileCost, (2 * nt + kCostBuckets) * sizeof(int)));
+  }
   for (int k = 0; k < K; ++k) {
     hipStream_t ts = ctx->tileStream[k];
     HIPCHK(ctx, hipStreamWaitEvent(ts, ctx->joinEvent, 0));
@@ -952,6 +999,27 @@ static int renderPersistent(snl_prog p, snl_state st, int frames, hipFunction_t 
         HIPCHK(ctx, hipModuleLaunchKernel(p->k_persist_reset, 1, 1, 1, 64, 1, 1, 0, ts, args, nullptr));
         HIPCHK(ctx, hipModuleLaunchKernel(kern, blocks, 1, 1, 256, 1, 1, 0, ts, args, nullptr));
       }
+    } else if (ordered) {
+      // frame 1 records the tiles' costs; the rest take the costliest tiles first
+      const int nt = (st->w >> 3) * (st->h >> 3), g = (nt + 255) / 256;
+      int* cost = st->tileCost;
+      int* hist = cost + nt;
+      int* order = hist + kCostBuckets;
+      hipLaunchKernelGGL(tile_zero_kernel, dim3(g), dim3(256), 0, ts, cost, nt);
+      hipLaunchKernelGGL(tile_zero_kernel, dim3(1), dim3(kCostBuckets), 0, ts, hist, kCostBuckets);
+      a.frames = 1;
+      a.tileCost = cost;
+      HIPCHK(ctx, hipModuleLaunchKernel(p->k_persist_reset, 1, 1, 1, 64, 1, 1, 0, ts, args, nullptr));
+      HIPCHK(ctx, hipModuleLaunchKernel(kern, blocks, 1, 1, 256, 1, 1, 0, ts, args, nullptr));
+      hipLaunchKernelGGL(tile_hist_kernel, dim3(g), dim3(256), 0, ts, cost, nt, hist);
+      hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1), 0, ts, hist);
+      hipLaunchKernelGGL(tile_scatter_kernel, dim3(g), dim3(256), 0, ts, cost, nt, hist, order);
+      HIPCHK(ctx, hipGetLastError());
+      a.frames = frames - 1;
+      a.tileCost = nullptr;
+      a.tileOrder = order;
+      HIPCHK(ctx, hipModuleLaunchKernel(p->k_persist_reset, 1, 1, 1, 64, 1, 1, 0, ts, args, nullptr));
+      HIPCHK(ctx, hipModuleLaunchKernel(kern, blocks, 1, 1, 256, 1, 1, 0, ts, args, nullptr));
     } else if (frames > 0) {
       // every pixel runs all its frames in one lane: one launch per tile
       a.frames = frames;
@@ -979,7 +1047,7 @@ static int renderPersistent(snl_prog p, snl_state st, int frames, hipFunction_t 
   p->stats.ms_trace = ms;  // the persistent kernel is the whole frame
   p->stats.trace_steps = steps;
   p->stats.paths = (long long)frames * p->spf * st->npix;
-  p->stats.trace_launches = unit0 ? frames * K : K;
+  p->stats.trace_launches = unit0 ? frames * K : (ordered ? 2 : K);
   return SNL_OK;
 }
 

@@ -29,11 +29,13 @@ __device__ inline int snl_tile_item(const SnlKArgs& a, int j) {
 // The pixel a queue item stands for.  Where the image divides into 8x8 tiles,
 // 64 consecutive items cover one tile, so a wave's lanes start neighbouring
 // rays; otherwise items are pixels in row order.  Chains of a multi-chain
-// state follow one another.
+// state follow one another.  With a tile order, queue position t takes tile
+// tileOrder[t] (a render's later launches take the costliest tiles first).
 __device__ inline int snl_item_pixel(const SnlKArgs& a, int item) {
   if ((a.W & 7) || (a.H & 7)) return item;
   int chain = item / a.imagePix, i = item - chain * a.imagePix;
   int tile = i >> 6, j = i & 63, tw = a.W >> 3;
+  if (a.tileOrder) tile = a.tileOrder[tile];
   int tr = tile / tw, tc = tile - tr * tw;
   return chain * a.imagePix + (tr * 8 + (j >> 3)) * a.W + tc * 8 + (j & 7);
 }

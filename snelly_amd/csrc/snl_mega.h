@@ -514,6 +514,7 @@ __device__ inline void mega_body(const SnlKArgs& a) {
   RT Lframe = rt1(0.0f);
   typename Mega<SC>::Path ps;
   long long normals = 0;
+  unsigned long long pixSteps = 0;  // pt.steps when the lane took its pixel
   while (true) {
     // ---- lanes without a pixel take the next ones of the range
     unsigned long long need = __ballot(!havePix);
@@ -533,6 +534,7 @@ __device__ inline void mega_body(const SnlKArgs& a) {
         frame = 0;
         sample = 0;
         frameOpen = false;
+        pixSteps = pt.steps;
         rnd = ld4(a.rngstate, pix);
         accum = ld4(a.radiance, pix);
         pixel = vec2(float(pix % a.W) + 0.5f, float((pix % a.imagePix) / a.W) + 0.5f);
@@ -551,6 +553,11 @@ __device__ inline void mega_body(const SnlKArgs& a) {
           st4(a.radiance, pix, accum);
           st4(a.rngstate, pix, rnd);
           havePix = false;
+          if (a.tileCost) {  // the pixel's march steps to its 8x8 tile
+            int i = pix % a.imagePix, y = i / a.W, x = i - y * a.W;
+            unsigned long long d = pt.steps - pixSteps;
+            atomicAdd(&a.tileCost[(y >> 3) * (a.W >> 3) + (x >> 3)], (int)(d < 0x7fffffffull ? d : 0x7fffffffull));
+          }
           break;
         }
 #ifdef INTERACTIVE_MODE

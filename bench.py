@@ -32,10 +32,12 @@ os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("SNL_BENCH_HW_QUEUES", "16")
 METRIC = "path samples/s (plain vs batched/served) at RMSE≤1%; p50/p99 job latency"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-# HBM bytes per pixel per launch of the queue kernels, from the FETCH_SIZE /
+# HBM bytes per pixel per render of the queue kernels, from the FETCH_SIZE /
 # WRITE_SIZE counter runs scaled by the copy yardstick
-# (profiles/<kernel>_menger4_1024_traffic_valu.md)
-TRAFFIC_B_PER_PX = {"snl_persist": 67.1, "snl_mega": 67.3}
+# (profiles/<kernel>_menger4_1024_traffic_valu.md): the persistent kernel's one
+# launch, the megakernel's two (frame 1 with the tile-cost atomics, then the
+# other frames in cost order)
+TRAFFIC_B_PER_PX = {"snl_persist": 67.1, "snl_mega": 143.4}
 # served: path samples per tick beyond each job's smallest quantum, given to the
 # jobs with the least remaining work first; chains per job: the fewest (up to
 # 128) that give it 2^18 lanes
@@ -236,15 +238,18 @@ def main():
     # central-difference normal (6 SDF evaluations); shading is not counted.
     flops = trace_steps * scene.flops_per_step + normal_evals * 6 * scene.sdf_flops
     achieved = flops / (trace_ms * 1e-3) / 1e12
-    pixels_per_launch = W * H * args.steps / launches
+    # per render (one step): a megakernel render is two launches (frame 1, then
+    # the other frames in cost order), each reading and writing every pixel once
+    renders = args.steps
+    per_render = launches / renders
+    kname = autotune.KERNEL_NAMES[best["engine"]]
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                "traffic": (None if TRAFFIC_B_PER_PX[autotune.KERNEL_NAMES[best["engine"]]] is None else
-                            round(TRAFFIC_B_PER_PX[autotune.KERNEL_NAMES[best["engine"]]] * pixels_per_launch)),
-                "traffic_algorithmic": 64 * pixels_per_launch,
-                "kernel": autotune.KERNEL_NAMES[best["engine"]], "launches": launches, "gpu_ms": round(trace_ms, 2),
-                "flops_per_launch": flops / launches, "flops_per_step": scene.flops_per_step,
-                "steps_per_launch": trace_steps / launches, "normal_evals_per_launch": normal_evals / launches,
+                "traffic": (None if TRAFFIC_B_PER_PX[kname] is None else round(TRAFFIC_B_PER_PX[kname] * W * H)),
+                "traffic_algorithmic": 64 * W * H * per_render, "per": "render",
+                "kernel": kname, "launches_per_render": per_render, "gpu_ms": round(trace_ms, 2),
+                "flops_per_render": flops / renders, "flops_per_step": scene.flops_per_step,
+                "steps_per_render": trace_steps / renders, "normal_evals_per_render": normal_evals / renders,
                 "sdf_flops": scene.sdf_flops,
                 "flops_counted": "SDF march steps (SDF + 8) and normals (6 SDF); shading excluded"}
     rr = None

@@ -553,10 +553,14 @@ __device__ inline void mega_body(const SnlKArgs& a) {
           st4(a.radiance, pix, accum);
           st4(a.rngstate, pix, rnd);
           havePix = false;
-          if (a.tileCost) {  // the pixel's march steps to its 8x8 tile
+          if (a.tileCost) {
+            // the pixel's march steps to its 8x8 tile; the 16 pixels with even
+            // coordinates stand for the tile (a quarter of the atomics)
             int i = pix % a.imagePix, y = i / a.W, x = i - y * a.W;
-            unsigned long long d = pt.steps - pixSteps;
-            atomicAdd(&a.tileCost[(y >> 3) * (a.W >> 3) + (x >> 3)], (int)(d < 0x7fffffffull ? d : 0x7fffffffull));
+            if (((x | y) & 1) == 0) {
+              unsigned long long d = pt.steps - pixSteps;
+              atomicAdd(&a.tileCost[(y >> 3) * (a.W >> 3) + (x >> 3)], (int)(d < 0x7fffffffull ? d : 0x7fffffffull));
+            }
           }
           break;
         }

@@ -37,7 +37,7 @@ FP32_PEAK_TFLOPS = 157.3
 # (profiles/<kernel>_menger4_1024_traffic_valu.md): the persistent kernel's one
 # launch, the megakernel's two (frame 1 with the tile-cost atomics, then the
 # other frames in cost order)
-TRAFFIC_B_PER_PX = {"snl_persist": 67.1, "snl_mega": 143.4}
+TRAFFIC_B_PER_PX = {"snl_persist": 67.1, "snl_mega": 144.3}
 # served: path samples per tick beyond each job's smallest quantum, given to the
 # jobs with the least remaining work first; chains per job: the fewest (up to
 # 128) that give it 2^18 lanes

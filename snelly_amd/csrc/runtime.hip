@@ -82,7 +82,7 @@ struct snl_prog_s {
       k_pick, k_tonemap, k_persist, k_persist_reset, k_mega, k_mode = nullptr;
   unsigned mode = 0;  // SNL_BUILD_AO or SNL_BUILD_PTSIMPLE, else 0 (the path tracer)
   int engine = SNL_ENGINE_MEGAKERNEL;
-  bool orderTiles = true;  // megakernel renders take tiles costliest first (SNL_TILE_ORDER=0: queue order)
+  bool orderTiles = true;  // queue-kernel renders take tiles costliest first (SNL_TILE_ORDER=0: queue order)
   int tiles = 2, persistBlocks = 0, shadeThresh = 32, persistChunk = 64, marchBurst = 4;
   bool hasSurface = false, hasMetal = false, hasDiele = false, hasAtmosphere = false;
   int megaBlocks = 0, modeBlocks = 0;
@@ -909,8 +909,9 @@ static int renderWavefront(snl_prog p, snl_state st, int frames) {
 // tiles run on their own streams so one tile's tail overlaps another's work.
 // The megakernel (snl_mega) and the AO / simplified integrators
 // (snl_mode_kernel) launch the same way.
-// Tile order of a megakernel render: its first frame runs as a launch of its
-// own that adds each pixel's march steps to its 8x8 tile; the tiles are then
+// Tile order of a queue-kernel render (megakernel or persistent): its first
+// frame runs as a launch of its own that adds pixels' march steps to their
+// 8x8 tiles; the tiles are then
 // sorted by that cost, most first (a counting sort over log-spaced buckets,
 // four per doubling), and the remaining frames take them in that order, so the
 // longest pixels start first and the launch does not end on a few of them.
@@ -944,6 +945,48 @@ __global__ void tile_scatter_kernel(const int* cost, int n, int* cursor, int* or
   if (i < n) order[atomicAdd(&cursor[costBucket(cost[i])], 1)] = i;
 }
 
+// whether a render of `frames` frames of `st` with `kern` takes its tiles in
+// cost order, and the state's tile buffers for it (allocated on first use)
+static bool tileOrdered(snl_prog p, snl_state st, hipFunction_t kern, int frames) {
+  return (kern == p->k_mega || kern == p->k_persist) && frames >= 2 && st->chains == 1 && st->w % 8 == 0 &&
+         st->h % 8 == 0 && p->orderTiles;
+}
+static int ensureTileBuffers(snl_ctx ctx, snl_state st) {
+  if (!st->tileCost) {
+    size_t nt = (size_t)(st->w >> 3) * (st->h >> 3);
+    HIPCHK(ctx, hipMalloc((void**)&st->tileCost, (2 * nt + kCostBuckets) * sizeof(int)));
+  }
+  return SNL_OK;
+}
+// frame 1 with tile costs, the tile order, then the other frames in that order,
+// all on stream ts (a is the launch's argument block, its queue counter reset
+// before each launch)
+static int launchOrdered(snl_ctx ctx, snl_prog p, snl_state st, hipFunction_t kern, int blocks, hipStream_t ts,
+                         SnlKArgs a, int frames) {
+  const int nt = (st->w >> 3) * (st->h >> 3), g = (nt + 255) / 256;
+  int* cost = st->tileCost;
+  int* hist = cost + nt;
+  int* order = hist + kCostBuckets;
+  void* args[] = {&a};
+  hipLaunchKernelGGL(tile_zero_kernel, dim3(g), dim3(256), 0, ts, cost, nt);
+  hipLaunchKernelGGL(tile_zero_kernel, dim3(1), dim3(kCostBuckets), 0, ts, hist, kCostBuckets);
+  a.frames = 1;
+  a.tileCost = cost;
+  a.tileOrder = nullptr;
+  HIPCHK(ctx, hipModuleLaunchKernel(p->k_persist_reset, 1, 1, 1, 64, 1, 1, 0, ts, args, nullptr));
+  HIPCHK(ctx, hipModuleLaunchKernel(kern, blocks, 1, 1, 256, 1, 1, 0, ts, args, nullptr));
+  hipLaunchKernelGGL(tile_hist_kernel, dim3(g), dim3(256), 0, ts, cost, nt, hist);
+  hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1), 0, ts, hist);
+  hipLaunchKernelGGL(tile_scatter_kernel, dim3(g), dim3(256), 0, ts, cost, nt, hist, order);
+  HIPCHK(ctx, hipGetLastError());
+  a.frames = frames - 1;
+  a.tileCost = nullptr;
+  a.tileOrder = order;
+  HIPCHK(ctx, hipModuleLaunchKernel(p->k_persist_reset, 1, 1, 1, 64, 1, 1, 0, ts, args, nullptr));
+  HIPCHK(ctx, hipModuleLaunchKernel(kern, blocks, 1, 1, 256, 1, 1, 0, ts, args, nullptr));
+  return SNL_OK;
+}
+
 static int renderPersistent(snl_prog p, snl_state st, int frames, hipFunction_t kern, int blocksAll) {
   snl_ctx ctx = p->ctx;
   bool unit0 = false;
@@ -969,14 +1012,10 @@ static int renderPersistent(snl_prog p, snl_state st, int frames, hipFunction_t 
   HIPCHK(ctx, hipEventRecord(p->ev0, ctx->stream));
   HIPCHK(ctx, hipEventRecord(ctx->joinEvent, ctx->stream));
   int blocks = std::max(1, blocksAll / K);
-  // megakernel renders of one 8x8-tiled image on one stream take their tiles
+  // queue-kernel renders of one 8x8-tiled image on one stream take their tiles
   // costliest first after the first frame (see tile_scatter_kernel)
-  const bool ordered = kern == p->k_mega && K == 1 && !unit0 && frames >= 2 && st->chains == 1 &&
-                       st->w % 8 == 0 && st->h % 8 == 0 && p->orderTiles;
-  if (ordered && !st->tileCost) {
-    size_t nt = (size_t)(st->w >> 3) * (st->h >> 3);
-    HIPCHK(ctx, hipMalloc((void**)&st->tileCost, (2 * nt + kCostBuckets) * sizeof(int)));
-  }
+  const bool ordered = K == 1 && !unit0 && tileOrdered(p, st, kern, frames);
+  if (ordered && (rc = ensureTileBuffers(ctx, st))) return rc;
   for (int k = 0; k < K; ++k) {
     hipStream_t ts = ctx->tileStream[k];
     HIPCHK(ctx, hipStreamWaitEvent(ts, ctx->joinEvent, 0));
@@ -1001,25 +1040,7 @@ static int renderPersistent(snl_prog p, snl_state st, int frames, hipFunction_t 
       }
     } else if (ordered) {
       // frame 1 records the tiles' costs; the rest take the costliest tiles first
-      const int nt = (st->w >> 3) * (st->h >> 3), g = (nt + 255) / 256;
-      int* cost = st->tileCost;
-      int* hist = cost + nt;
-      int* order = hist + kCostBuckets;
-      hipLaunchKernelGGL(tile_zero_kernel, dim3(g), dim3(256), 0, ts, cost, nt);
-      hipLaunchKernelGGL(tile_zero_kernel, dim3(1), dim3(kCostBuckets), 0, ts, hist, kCostBuckets);
-      a.frames = 1;
-      a.tileCost = cost;
-      HIPCHK(ctx, hipModuleLaunchKernel(p->k_persist_reset, 1, 1, 1, 64, 1, 1, 0, ts, args, nullptr));
-      HIPCHK(ctx, hipModuleLaunchKernel(kern, blocks, 1, 1, 256, 1, 1, 0, ts, args, nullptr));
-      hipLaunchKernelGGL(tile_hist_kernel, dim3(g), dim3(256), 0, ts, cost, nt, hist);
-      hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(1), 0, ts, hist);
-      hipLaunchKernelGGL(tile_scatter_kernel, dim3(g), dim3(256), 0, ts, cost, nt, hist, order);
-      HIPCHK(ctx, hipGetLastError());
-      a.frames = frames - 1;
-      a.tileCost = nullptr;
-      a.tileOrder = order;
-      HIPCHK(ctx, hipModuleLaunchKernel(p->k_persist_reset, 1, 1, 1, 64, 1, 1, 0, ts, args, nullptr));
-      HIPCHK(ctx, hipModuleLaunchKernel(kern, blocks, 1, 1, 256, 1, 1, 0, ts, args, nullptr));
+      if ((rc = launchOrdered(ctx, p, st, kern, blocks, ts, a, frames))) return rc;
     } else if (frames > 0) {
       // every pixel runs all its frames in one lane: one launch per tile
       a.frames = frames;
@@ -1136,6 +1157,14 @@ int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, 
       HIPCHK(ctx, hipEventCreateWithFlags(&ctx->batchEvent[k], hipEventDisableTiming));
     }
   }
+  // queue-kernel jobs of one 8x8-tiled image take their tiles in cost order;
+  // their tile buffers are allocated before anything is launched
+  std::vector<char> ordered(njobs, 0);
+  for (int j = 0; j < njobs; ++j) {
+    int resident = 0;
+    ordered[j] = !unit0[j] && tileOrdered(progs[j], states[j], queueKernel(progs[j], resident), frames[j]);
+    if (ordered[j] && (rc = ensureTileBuffers(ctx, states[j]))) return rc;
+  }
   // Longest job first onto the least loaded stream.  A job lasts about the
   // larger of its throughput time (all its path samples at a device-wide rate)
   // and its latency (its frames run back to back in one lane: frames times a
@@ -1173,6 +1202,10 @@ int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, 
     long long want = (st->npix + 255) / 256;
     int blocks = (int)std::max(1LL, std::min<long long>(resident, want));
     void* args[] = {&a};
+    if (ordered[j]) {  // the costliest tiles first after frame 1, as in snl_render
+      if ((rc = launchOrdered(ctx, p, st, kern, blocks, ts, a, frames[j]))) return rc;
+      continue;
+    }
     int launches = unit0[j] ? frames[j] : (frames[j] > 0 ? 1 : 0);
     a.frames = unit0[j] ? 1 : frames[j];
     for (int f = 0; f < launches; ++f) {
@@ -1200,7 +1233,7 @@ int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, 
       stats->trace_steps += sk;
       stats->normal_evals += nk;
       stats->paths += (long long)frames[j] * progs[j]->spf * states[j]->npix;
-      stats->trace_launches += unit0[j] ? frames[j] : 1;
+      stats->trace_launches += unit0[j] ? frames[j] : (ordered[j] ? 2 : 1);
     }
   }
   return SNL_OK;

@@ -58,6 +58,17 @@ __device__ inline void persist_body(const SnlKArgs& a) {
   float nden0 = 1.0f, nden1 = 1.0f, nden2 = 1.0f, ncos0 = 0.0f, ncos1 = 0.0f, ncos2 = 0.0f;
   float nmis0 = 0.0f, nmis1 = 0.0f, nmis2 = 0.0f;
   long long stepTotal = 0, normalTotal = 0;
+  long long pixSteps = 0;  // stepTotal when the lane took its pixel
+  // the pixel's march steps to its 8x8 tile (the 16 pixels with even
+  // coordinates stand for the tile), for a render's tile order
+  auto addTileCost = [&]() {
+    if (!a.tileCost) return;
+    int i = pix % a.imagePix, y = i / a.W, x = i - y * a.W;
+    if (((x | y) & 1) == 0) {
+      long long d = stepTotal - pixSteps;
+      atomicAdd(&a.tileCost[(y >> 3) * (a.W >> 3) + (x >> 3)], (int)(d < 0x7fffffffll ? d : 0x7fffffffll));
+    }
+  };
 
   auto startRay = [&](const vec3& dir, float len, bool isShadow) {
     d = dir;
@@ -99,6 +110,7 @@ __device__ inline void persist_body(const SnlKArgs& a) {
         pix = snl_item_pixel(a, snl_tile_item(a, rb + rank));
         sample = 0;
         frame = 0;
+        pixSteps = stepTotal;
         st = PS_CAMERA;
       }
       rb += min(avail, (int)__popcll(idleMask));
@@ -336,6 +348,7 @@ __device__ inline void persist_body(const SnlKArgs& a) {
         } else {
           st4(a.radiance, pix, accum);
           st4(a.rngstate, pix, rnd);
+          addTileCost();
           st = PS_IDLE;
         }
       }
@@ -374,6 +387,7 @@ __device__ inline void persist_body(const SnlKArgs& a) {
         } else {
           st4(a.radiance, pix, accum);
           st4(a.rngstate, pix, rnd);
+          addTileCost();
           st = PS_IDLE;
         }
       } else {

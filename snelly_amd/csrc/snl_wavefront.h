@@ -54,7 +54,7 @@ struct SnlKArgs {
   int maxBounces;         // __MAX_BOUNCES__: the last vertex index a path may reach
   int imagePix;           // W * H; a state of several chains holds npix = chains * imagePix
   int burst;              // persistent kernel: march steps per lane between shading checks
-  int* tileCost;          // megakernel: march steps per 8x8 pixel tile are added here (or null)
+  int* tileCost;          // queue kernels: march steps per 8x8 pixel tile are added here (or null)
   const int* tileOrder;   // queue position of a tile -> the tile it takes (or null: in order)
 };
 

@@ -83,7 +83,7 @@ struct snl_prog_s {
   unsigned mode = 0;  // SNL_BUILD_AO or SNL_BUILD_PTSIMPLE, else 0 (the path tracer)
   int engine = SNL_ENGINE_MEGAKERNEL;
   bool orderTiles = true;  // queue-kernel renders take tiles costliest first (SNL_TILE_ORDER=0: queue order)
-  int tiles = 2, persistBlocks = 0, shadeThresh = 32, persistChunk = 64, marchBurst = 4;
+  int tiles = 1, persistBlocks = 0, shadeThresh = 32, persistChunk = 64, marchBurst = 4;
   bool hasSurface = false, hasMetal = false, hasDiele = false, hasAtmosphere = false;
   int megaBlocks = 0, modeBlocks = 0;
   int maxBounces = 3, maxMarch = 256, spf = 1;

@@ -310,3 +310,22 @@ def test_render_modes(ctx, name):
     g, r, close, rng_same, _ = compare_render(ctx, s, 64, 48, 8)
     assert close >= 0.97 and rng_same >= 0.97, (close, rng_same)
     assert rrmse(unbiased(g), unbiased(r)) < 1e-2
+
+
+@pytest.mark.parametrize("engine", [native.ENGINE_MEGAKERNEL, native.ENGINE_PERSISTENT], ids=["megakernel", "persistent"])
+def test_tile_cost_order_does_not_change_results(ctx, engine, monkeypatch):
+    """a render of several frames takes its 8x8 tiles costliest first after
+    frame 1; each pixel still runs its frames in order in one lane, so the
+    radiance and RNG states are bit-identical to a render in queue order
+    (SNL_TILE_ORDER=0, read when the program is built)"""
+    s = scenes.menger_metal(64, 48, 8)
+    out = []
+    for order in ("1", "0"):
+        monkeypatch.setenv("SNL_TILE_ORDER", order)
+        sess = session(ctx, s, 64, 48, native.BUILD_PRECISE, engine)
+        sess.render(8)
+        out.append((sess.radiance().copy(), sess.state.read(1).copy(), sess.prog.stats().trace_launches))
+        sess.close()
+    (ra, ga, la), (rb, gb, lb) = out
+    assert la == 2 and lb == 1  # frame 1, then the rest in cost order; one launch in queue order
+    assert np.array_equal(ra, rb) and np.array_equal(ga, gb)

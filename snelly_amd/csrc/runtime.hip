@@ -948,8 +948,8 @@ __global__ void tile_scatter_kernel(const int* cost, int n, int* cursor, int* or
 // whether a render of `frames` frames of `st` with `kern` takes its tiles in
 // cost order, and the state's tile buffers for it (allocated on first use)
 static bool tileOrdered(snl_prog p, snl_state st, hipFunction_t kern, int frames) {
-  return (kern == p->k_mega || kern == p->k_persist) && frames >= 2 && st->chains == 1 && st->w % 8 == 0 &&
-         st->h % 8 == 0 && p->orderTiles;
+  return (kern == p->k_mega || kern == p->k_persist) && frames >= 2 && st->w % 8 == 0 && st->h % 8 == 0 &&
+         p->orderTiles;
 }
 static int ensureTileBuffers(snl_ctx ctx, snl_state st) {
   if (!st->tileCost) {
@@ -1012,8 +1012,8 @@ static int renderPersistent(snl_prog p, snl_state st, int frames, hipFunction_t 
   HIPCHK(ctx, hipEventRecord(p->ev0, ctx->stream));
   HIPCHK(ctx, hipEventRecord(ctx->joinEvent, ctx->stream));
   int blocks = std::max(1, blocksAll / K);
-  // queue-kernel renders of one 8x8-tiled image on one stream take their tiles
-  // costliest first after the first frame (see tile_scatter_kernel)
+  // queue-kernel renders of an 8x8-tiled image (all its chains) on one stream
+  // take their tiles costliest first after the first frame (see tile_scatter_kernel)
   const bool ordered = K == 1 && !unit0 && tileOrdered(p, st, kern, frames);
   if (ordered && (rc = ensureTileBuffers(ctx, st))) return rc;
   for (int k = 0; k < K; ++k) {
@@ -1157,7 +1157,7 @@ int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, 
       HIPCHK(ctx, hipEventCreateWithFlags(&ctx->batchEvent[k], hipEventDisableTiming));
     }
   }
-  // queue-kernel jobs of one 8x8-tiled image take their tiles in cost order;
+  // queue-kernel jobs of an 8x8-tiled image take their tiles in cost order;
   // their tile buffers are allocated before anything is launched
   std::vector<char> ordered(njobs, 0);
   for (int j = 0; j < njobs; ++j) {

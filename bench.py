@@ -264,6 +264,8 @@ def main():
         "config": {"workload": "menger4_metal_spectral", "width": W, "height": H, "spp": spp, "bounces": 8,
                    "mode": "plain", "engine": autotune.ENGINE_NAMES[best["engine"]], "build": "fast",
                    "launch_shape": {k: v for k, v in best.items() if k != "engine"},
+                   "tile_order": ("cost (frame 1, then the 8x8 tiles costliest first)"
+                                  if os.environ.get("SNL_TILE_ORDER", "1") != "0" and best["tiles"] == 1 else "queue"),
                    "path_samples_per_step": W * H * spp},
         "engines": {"wavefront": wavefront, "megakernel": megakernel},
         "roofline": roofline,

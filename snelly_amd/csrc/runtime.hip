@@ -493,16 +493,24 @@ int snl_program_build(snl_ctx ctx, const char* shader, const char* ior, const ch
   p->user = c.tr.uniforms;
   p->hostU.assign(c.tr.userBlockSize, 0);
   {
-    std::string text = std::string(shader) + "\n" + (ior ? ior : "");
-    auto mentions = [&](const std::string& name) {
-      for (size_t at = text.find(name); at != std::string::npos; at = text.find(name, at + 1)) {
-        bool l = at == 0 || !(isalnum((unsigned char)text[at - 1]) || text[at - 1] == '_');
+    // samplers the program can read: those the scene code names, and those of
+    // the materials' IOR code when something calls it -- the metal's
+    // IOR_METAL / K_METAL run only in the metal BSDF (under HAS_METAL) unless
+    // the scene code calls them itself, so a program without a metal never
+    // reads the default tabulated metal's kTex
+    std::string text = shader ? shader : "";
+    auto mentionsIn = [](const std::string& t, const std::string& name) {
+      for (size_t at = t.find(name); at != std::string::npos; at = t.find(name, at + 1)) {
+        bool l = at == 0 || !(isalnum((unsigned char)t[at - 1]) || t[at - 1] == '_');
         size_t e = at + name.size();
-        bool r = e >= text.size() || !(isalnum((unsigned char)text[e]) || text[e] == '_');
+        bool r = e >= t.size() || !(isalnum((unsigned char)t[e]) || t[e] == '_');
         if (l && r) return true;
       }
       return false;
     };
+    if (ior && (p->hasMetal || mentionsIn(text, "IOR_METAL") || mentionsIn(text, "K_METAL")))
+      text += std::string("\n") + ior;
+    auto mentions = [&](const std::string& name) { return mentionsIn(text, name); };
     const auto& b = builtins();
     for (int i = 0; i < (int)b.size(); ++i)
       if (b[i].type == SNL_T_SAMPLER2D && mentions(b[i].name)) p->sampled2D.push_back(i);

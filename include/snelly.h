@@ -108,6 +108,20 @@ int snl_state_write(snl_state st, int which, const float* in);
 /* Renderer.render's pathtracer draw (js/renderer.js:462-644) repeated for
  * `frames` frames: one path sample per pixel per frame per sample-per-frame. */
 int snl_render(snl_prog prog, snl_state st, int frames);
+/* snl_render without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The render's
+ * queue counters and its tile-order buffers live in a file-scope device array
+ * of the library (one per GPU, about 2.1 MB, held until the process ends)
+ * instead of the state's buffers; everything else is the caller's program and
+ * state.  Results are identical to snl_render(prog, st, frames).  Limits: the
+ * image has at most SNL_NOALLOC_MAX_TILES 8x8 tiles (4096 x 4096; larger ->
+ * SNL_E_ARG); programs that render with the wavefront engine or sample texture
+ * unit 0 (an unbound sampler2D) -> SNL_E_UNSUPPORTED.  Every call on one GPU
+ * shares that GPU's array, so calls that overlap on one GPU (from several
+ * contexts or threads) race; calls on one context are ordered. */
+#define SNL_NOALLOC_MAX_TILES 262144
+int snl_render_noalloc(snl_prog prog, snl_state st, int frames);
 /* the 'normals' render mode (normals-fragment-shader.glsl) */
 int snl_render_normals(snl_prog prog, snl_state st, int frames);
 /* Renderer.pick (js/renderer.js:398-460) for n pixel positions (gl_FragCoord

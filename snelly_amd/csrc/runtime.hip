@@ -51,6 +51,7 @@ struct snl_ctx_s {
   std::vector<hipEvent_t> batchEvent;
   std::string err;
   int numCU = 256;
+  int* scratch = nullptr;  // this device's copy of the allocation-free render's `scratch` array
 };
 
 struct snl_tex_s {
@@ -401,6 +402,36 @@ __global__ void seed_kernel(float* radiance, float* rngstate, long long npix, un
 
 int blocksFor(long long n, int bs = 256) { return (int)((n + bs - 1) / bs); }
 
+// ---------------------------------------------------------------- allocation-free render's scratch
+// snl_render_noalloc keeps the render's queue counters and its tile-order
+// buffers (costs, cost histogram, order) in this array instead of the state's
+// allocations.  Its size bounds the image: at most SNL_NOALLOC_MAX_TILES 8x8
+// tiles (a 4096 x 4096 image), 2.1 MB per GPU, held until the process ends.
+// On the host the array's name is no device address: set-up resolves each
+// device's copy with hipGetSymbolAddress and every host use goes through that.
+constexpr int kNoallocCostBuckets = 128;  // = kCostBuckets of the tile order
+constexpr size_t kScratchInts = C_COUNT + 2 * (size_t)SNL_NOALLOC_MAX_TILES + kNoallocCostBuckets;
+__device__ __align__(256) int scratch[kScratchInts];
+std::mutex g_scratchMu;
+std::map<int, int*> g_scratchPtr;  // device -> address of its copy of `scratch`
+
+// the address of `scratch` on `device`, resolved once per device with that
+// device current (the caller has made it current)
+int scratchFor(int device, int** out) {
+  std::lock_guard<std::mutex> lk(g_scratchMu);
+  auto it = g_scratchPtr.find(device);
+  if (it == g_scratchPtr.end()) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(nullptr, SNL_E_HIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", device,
+                  hipGetErrorString(e));
+    it = g_scratchPtr.emplace(device, static_cast<int*>(ptr)).first;
+  }
+  *out = it->second;
+  return SNL_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- API
@@ -430,6 +461,7 @@ int snl_ctx_create(int device, snl_ctx* out) {
     HIPCHK(nullptr, hipEventCreateWithFlags(&c->tileEvent[k], hipEventDisableTiming));
   }
   HIPCHK(nullptr, hipEventCreateWithFlags(&c->joinEvent, hipEventDisableTiming));
+  if (int rc = scratchFor(device, &c->scratch)) return rc;
   *out = c.release();
   return SNL_OK;
 }
@@ -925,6 +957,7 @@ static int renderWavefront(snl_prog p, snl_state st, int frames) {
 // longest pixels start first and the launch does not end on a few of them.
 // Each pixel still runs its frames in order in one lane: results do not change.
 constexpr int kCostBuckets = 128;
+static_assert(kCostBuckets == kNoallocCostBuckets, "the allocation-free render's scratch holds the cost histogram");
 __device__ inline int costBucket(int c) {
   int b = (int)(log2f((float)c + 1.0f) * 4.0f);
   b = b < 0 ? 0 : (b >= kCostBuckets ? kCostBuckets - 1 : b);
@@ -968,11 +1001,11 @@ static int ensureTileBuffers(snl_ctx ctx, snl_state st) {
 }
 // frame 1 with tile costs, the tile order, then the other frames in that order,
 // all on stream ts (a is the launch's argument block, its queue counter reset
-// before each launch)
+// before each launch); tileBuf holds 2 * tiles + kCostBuckets ints
 static int launchOrdered(snl_ctx ctx, snl_prog p, snl_state st, hipFunction_t kern, int blocks, hipStream_t ts,
-                         SnlKArgs a, int frames) {
+                         SnlKArgs a, int frames, int* tileBuf) {
   const int nt = (st->w >> 3) * (st->h >> 3), g = (nt + 255) / 256;
-  int* cost = st->tileCost;
+  int* cost = tileBuf;
   int* hist = cost + nt;
   int* order = hist + kCostBuckets;
   void* args[] = {&a};
@@ -995,7 +1028,11 @@ static int launchOrdered(snl_ctx ctx, snl_prog p, snl_state st, hipFunction_t ke
   return SNL_OK;
 }
 
-static int renderPersistent(snl_prog p, snl_state st, int frames, hipFunction_t kern, int blocksAll) {
+// scratch (or null): the allocation-free render's counters (C_COUNT ints) and
+// tile buffers, used in place of the state's; the render then runs one tile
+// set on one stream
+static int renderPersistent(snl_prog p, snl_state st, int frames, hipFunction_t kern, int blocksAll,
+                            int* scratch = nullptr) {
   snl_ctx ctx = p->ctx;
   bool unit0 = false;
   int rc = prepareUnit0(p, st, unit0);
@@ -1004,7 +1041,8 @@ static int renderPersistent(snl_prog p, snl_state st, int frames, hipFunction_t 
   int K = std::max(1, std::min(p->tiles, SNL_MAX_TILES));
   // a frame must see the whole previous frame through the unit-0 snapshot,
   // so such programs run their frames in order on one stream
-  if ((long long)K > st->npix || unit0) K = 1;
+  if ((long long)K > st->npix || unit0 || scratch) K = 1;
+  int* counters = scratch ? scratch : st->tileCnt;
   p->stats = snl_stats{};
   SnlKArgs base;
   fillArgs(p, st, base);
@@ -1014,7 +1052,7 @@ static int renderPersistent(snl_prog p, snl_state st, int frames, hipFunction_t 
   SnlKArgs z = base;
   z.vertex = -2;
   for (int k = 0; k < K; ++k) {
-    z.cnt = st->tileCnt + k * C_COUNT;
+    z.cnt = counters + k * C_COUNT;
     if ((rc = launch(ctx, p->k_reset, 1, 64, z))) return rc;
   }
   HIPCHK(ctx, hipEventRecord(p->ev0, ctx->stream));
@@ -1023,12 +1061,16 @@ static int renderPersistent(snl_prog p, snl_state st, int frames, hipFunction_t 
   // queue-kernel renders of an 8x8-tiled image (all its chains) on one stream
   // take their tiles costliest first after the first frame (see tile_scatter_kernel)
   const bool ordered = K == 1 && !unit0 && tileOrdered(p, st, kern, frames);
-  if (ordered && (rc = ensureTileBuffers(ctx, st))) return rc;
+  int* tileBuf = scratch ? scratch + C_COUNT : nullptr;
+  if (ordered && !scratch) {
+    if ((rc = ensureTileBuffers(ctx, st))) return rc;
+    tileBuf = st->tileCost;
+  }
   for (int k = 0; k < K; ++k) {
     hipStream_t ts = ctx->tileStream[k];
     HIPCHK(ctx, hipStreamWaitEvent(ts, ctx->joinEvent, 0));
     SnlKArgs a = base;
-    a.cnt = st->tileCnt + k * C_COUNT;
+    a.cnt = counters + k * C_COUNT;
     // tile k: the 64-pixel stripes s with s % K == k (the last one may be short)
     long long stripes = (st->npix + 63) / 64, last = stripes - 1;
     long long mine = stripes / K + (k < stripes % K ? 1 : 0);
@@ -1048,7 +1090,7 @@ static int renderPersistent(snl_prog p, snl_state st, int frames, hipFunction_t 
       }
     } else if (ordered) {
       // frame 1 records the tiles' costs; the rest take the costliest tiles first
-      if ((rc = launchOrdered(ctx, p, st, kern, blocks, ts, a, frames))) return rc;
+      if ((rc = launchOrdered(ctx, p, st, kern, blocks, ts, a, frames, tileBuf))) return rc;
     } else if (frames > 0) {
       // every pixel runs all its frames in one lane: one launch per tile
       a.frames = frames;
@@ -1065,8 +1107,8 @@ static int renderPersistent(snl_prog p, snl_state st, int frames, hipFunction_t 
   long long steps = 0, normals = 0;
   for (int k = 0; k < K; ++k) {
     long long sk = 0, nk = 0;
-    HIPCHK(ctx, hipMemcpyAsync(&sk, st->tileCnt + k * C_COUNT + C_TRACE_STEPS, 8, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipMemcpyAsync(&nk, st->tileCnt + k * C_COUNT + C_NORMALS, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(&sk, counters + k * C_COUNT + C_TRACE_STEPS, 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(&nk, counters + k * C_COUNT + C_NORMALS, 8, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     steps += sk;
     normals += nk;
@@ -1211,7 +1253,7 @@ int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, 
     int blocks = (int)std::max(1LL, std::min<long long>(resident, want));
     void* args[] = {&a};
     if (ordered[j]) {  // the costliest tiles first after frame 1, as in snl_render
-      if ((rc = launchOrdered(ctx, p, st, kern, blocks, ts, a, frames[j]))) return rc;
+      if ((rc = launchOrdered(ctx, p, st, kern, blocks, ts, a, frames[j], st->tileCost))) return rc;
       continue;
     }
     int launches = unit0[j] ? frames[j] : (frames[j] > 0 ? 1 : 0);
@@ -1259,6 +1301,34 @@ int snl_render(snl_prog p, snl_state st, int frames) {
   if (p->engine == SNL_ENGINE_PERSISTENT && !needWavefront)
     return renderPersistent(p, st, frames, p->k_persist, p->persistBlocks);
   return renderWavefront(p, st, frames);
+}
+
+int snl_render_noalloc(snl_prog p, snl_state st, int frames) {
+  if (!p || !st || frames < 0) return fail(p ? p->ctx : nullptr, SNL_E_ARG, "bad argument");
+  snl_ctx ctx = p->ctx;
+  if (st->ctx != ctx) return fail(ctx, SNL_E_ARG, "program and state belong to different contexts");
+  const long long tiles = (long long)(st->w >> 3) * (st->h >> 3);
+  if (tiles > SNL_NOALLOC_MAX_TILES)
+    return fail(ctx, SNL_E_ARG,
+                "snl_render_noalloc: a %d x %d image has %lld 8x8 tiles, its scratch holds at most %d "
+                "(a 4096 x 4096 image); render it with snl_render",
+                st->w, st->h, tiles, SNL_NOALLOC_MAX_TILES);
+  for (int idx : p->sampled2D)
+    if (!p->bound.count(idx))
+      return fail(ctx, SNL_E_UNSUPPORTED,
+                  "snl_render_noalloc: the program samples texture unit 0 (an unbound sampler2D), whose "
+                  "radiance snapshot is an allocation; render it with snl_render");
+  if (!ctx->scratch) return fail(ctx, SNL_E_HIP, "snl_render_noalloc: scratch address not resolved");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  // the kernel snl_render picks; the wavefront engine's path buffers are allocations
+  if (p->mode) return renderPersistent(p, st, frames, p->k_mode, p->modeBlocks, ctx->scratch);
+  if (p->hasAtmosphere || p->engine == SNL_ENGINE_MEGAKERNEL)
+    return renderPersistent(p, st, frames, p->k_mega, p->megaBlocks, ctx->scratch);
+  if (p->engine == SNL_ENGINE_PERSISTENT && !(p->hostP.subsurfaceMFP > 0.0f))
+    return renderPersistent(p, st, frames, p->k_persist, p->persistBlocks, ctx->scratch);
+  return fail(ctx, SNL_E_UNSUPPORTED,
+              "snl_render_noalloc: this program renders with the wavefront engine, whose path buffers are "
+              "allocations; choose the megakernel or the persistent engine");
 }
 
 int snl_set_engine(snl_prog p, int engine, int tiles, int persist_blocks, int shade_thresh, int chunk) {

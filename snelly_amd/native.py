@@ -26,6 +26,7 @@ EXPORTS = [
     "snl_state_write", "snl_render", "snl_render_normals", "snl_pick", "snl_tonemap", "snl_sync",
     "snl_get_stats", "snl_set_tuning", "snl_set_profiling", "snl_set_engine", "snl_render_batch",
     "snl_state_create_chains", "snl_state_error", "snl_ctx_mem_info", "snl_state_bytes", "snl_set_march_burst",
+    "snl_render_noalloc",
 ]
 ENGINE_WAVEFRONT, ENGINE_PERSISTENT, ENGINE_MEGAKERNEL = 0, 1, 2
 
@@ -71,6 +72,7 @@ def lib():
         "snl_state_read": (ip, [vp, ip, fp]),
         "snl_state_write": (ip, [vp, ip, fp]),
         "snl_render": (ip, [vp, vp, ip]),
+        "snl_render_noalloc": (ip, [vp, vp, ip]),
         "snl_render_normals": (ip, [vp, vp, ip]),
         "snl_pick": (ip, [vp, fp, ip, fp]),
         "snl_tonemap": (ip, [vp, vp, fp, ctypes.POINTER(ctypes.c_ubyte)]),

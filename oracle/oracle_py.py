@@ -61,6 +61,18 @@ def lib():
         L.oracle_normals.argtypes = [fp, fp] + [ctypes.c_int] * 4
         L.oracle_pick.argtypes = [fp, ctypes.c_int, fp]
         L.oracle_tonemap.argtypes = [fp, ctypes.c_longlong, fp]
+        L.oracle_medium.argtypes = [ctypes.c_int] + [ctypes.c_float] * 4
+        for n in ["oracle_value_noise", "oracle_fbm3", "oracle_medium_density", "oracle_medium_bound"]:
+            getattr(L, n).argtypes = [fp]
+            getattr(L, n).restype = ctypes.c_float
+        L.oracle_lattice_value.argtypes = [ctypes.c_int]
+        L.oracle_lattice_value.restype = ctypes.c_float
+        for n in ["oracle_medium_free_flight", "oracle_medium_transmittance"]:
+            getattr(L, n).argtypes = [fp, fp, ctypes.c_float, fp]
+            getattr(L, n).restype = ctypes.c_float
+        L.oracle_sample_hg.argtypes = [fp, ctypes.c_float, fp, fp, fp]
+        L.oracle_phase_function.argtypes = [ctypes.c_float, ctypes.c_float]
+        L.oracle_phase_function.restype = ctypes.c_float
         _lib = L
     return _lib
 
@@ -96,8 +108,13 @@ class Oracle:
         L.oracle_config(int(r.dispersive), int(r.interactive), int(round(r.maxBounces)),
                         int(round(r.maxMarchSteps)), int(round(r.maxSamplesPerFrame)), sphere_light)
         v = mats.volume
-        has_atm = any(c > 0.0 for c in v.scatteringColor) or any(c > 0.0 for c in v.absorptionColor)
+        # a scene with SDF_VOLUME has the heterogeneous medium instead of the atmosphere
+        has_medium = host.has_hook(scene.glsl, "SDF_VOLUME")
+        has_atm = not has_medium and (any(c > 0.0 for c in v.scatteringColor) or any(c > 0.0 for c in v.absorptionColor))
         L.oracle_volume(int(has_atm), int(bool(v.fogEnable)), int(round(r.maxAtmosphereScatters)))
+        medium = getattr(scene, "oracle_medium", None)
+        assert has_medium == (medium is not None), "a scene with SDF_VOLUME needs its oracle restatement"
+        L.oracle_medium(*(medium or (0, 0.0, 0.0, 0.0, 0.0)))
         for (m, kind, a, b, c, d) in scene.oracle_sdfs:
             L.oracle_sdf(m, kind, a, b, c, d)
         L.oracle_albedo_hook(scene.oracle_albedo_hook)

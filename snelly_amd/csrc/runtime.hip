@@ -86,6 +86,10 @@ struct snl_prog_s {
   bool orderTiles = true;  // queue-kernel renders take tiles costliest first (SNL_TILE_ORDER=0: queue order)
   int tiles = 1, persistBlocks = 0, shadeThresh = 32, persistChunk = 64, marchBurst = 4;
   bool hasSurface = false, hasMetal = false, hasDiele = false, hasAtmosphere = false;
+  // programs only the megakernel renders: the homogeneous atmosphere (a vertex
+  // traces up to five extra rays between random draws) and the heterogeneous
+  // medium (delta and ratio tracking inside the vertex)
+  bool megaOnly = false;
   int megaBlocks = 0, modeBlocks = 0;
   int maxBounces = 3, maxMarch = 256, spf = 1;
   SnlParams hostP;
@@ -518,6 +522,7 @@ int snl_program_build(snl_ctx ctx, const char* shader, const char* ior, const ch
   p->hasMetal = c.defines.count("HAS_METAL") > 0;
   p->hasDiele = c.defines.count("HAS_DIELECTRIC") > 0;
   p->hasAtmosphere = c.defines.count("HAS_ATMOSPHERE") > 0;
+  p->megaOnly = p->hasAtmosphere || c.defines.count("HAS_VOLUME") > 0;
   p->maxBounces = maxBounces;
   p->maxMarch = maxMarch;
   p->spf = spf;
@@ -1161,7 +1166,7 @@ unsigned long long snl_state_bytes(int width, int height, int chains) {
 // megakernel); AO and simplified programs their own kernel
 static hipFunction_t queueKernel(snl_prog p, int& blocks) {
   if (p->mode) { blocks = p->modeBlocks; return p->k_mode; }
-  if (p->engine == SNL_ENGINE_PERSISTENT && !p->hasAtmosphere && !(p->hostP.subsurfaceMFP > 0.0f)) {
+  if (p->engine == SNL_ENGINE_PERSISTENT && !p->megaOnly && !(p->hostP.subsurfaceMFP > 0.0f)) {
     blocks = p->persistBlocks;
     return p->k_persist;
   }
@@ -1296,7 +1301,7 @@ int snl_render(snl_prog p, snl_state st, int frames) {
   // run the sub-surface walk, such programs use the wavefront unless the
   // megakernel is asked for
   if (p->mode) return renderPersistent(p, st, frames, p->k_mode, p->modeBlocks);
-  if (p->hasAtmosphere || p->engine == SNL_ENGINE_MEGAKERNEL) return renderPersistent(p, st, frames, p->k_mega, p->megaBlocks);
+  if (p->megaOnly || p->engine == SNL_ENGINE_MEGAKERNEL) return renderPersistent(p, st, frames, p->k_mega, p->megaBlocks);
   bool needWavefront = p->hostP.subsurfaceMFP > 0.0f;
   if (p->engine == SNL_ENGINE_PERSISTENT && !needWavefront)
     return renderPersistent(p, st, frames, p->k_persist, p->persistBlocks);
@@ -1322,7 +1327,7 @@ int snl_render_noalloc(snl_prog p, snl_state st, int frames) {
   HIPCHK(ctx, hipSetDevice(ctx->device));
   // the kernel snl_render picks; the wavefront engine's path buffers are allocations
   if (p->mode) return renderPersistent(p, st, frames, p->k_mode, p->modeBlocks, ctx->scratch);
-  if (p->hasAtmosphere || p->engine == SNL_ENGINE_MEGAKERNEL)
+  if (p->megaOnly || p->engine == SNL_ENGINE_MEGAKERNEL)
     return renderPersistent(p, st, frames, p->k_mega, p->megaBlocks, ctx->scratch);
   if (p->engine == SNL_ENGINE_PERSISTENT && !(p->hostP.subsurfaceMFP > 0.0f))
     return renderPersistent(p, st, frames, p->k_persist, p->persistBlocks, ctx->scratch);

@@ -702,7 +702,7 @@ __device__ inline void pick_body(const SnlKArgs& a, const float* xy, float* out,
   vec3 primaryStart = P.camPos;
   vec3 pW;
   int hitMaterial;
-  bool hit = pt.traceDistance(primaryStart, primaryDir, P.maxLengthScale, pW, hitMaterial);
+  bool hit = pt.traceDistance(primaryStart, primaryDir, P.maxLengthScale, pW, hitMaterial, true);
   float d = P.maxLengthScale;
   if (hit) d = length(pW - primaryStart);
   out[2 * k] = d;

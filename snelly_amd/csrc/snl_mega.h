@@ -20,12 +20,17 @@ struct Mega {
   const SnlParams& P;
   SNL_HD Mega(PT<SC>& t, const SnlParams& p) : pt(t), P(p) {}
 
-  // transmittanceOverSegment (1042-1050)
-  SNL_HD RT transmittanceOverSegment(const vec3& pW, const vec3& rayDir, float segmentLength, const vec3& rgb) {
+  // transmittanceOverSegment (1042-1050); with the heterogeneous medium also
+  // its ratio-tracking estimate over the segment (random draws)
+  SNL_HD RT transmittanceOverSegment(const vec3& pW, const vec3& rayDir, float segmentLength, const vec3& rgb,
+                                     vec4& rnd) {
     vec3 pHit;
     int hm;
     if (pt.traceRay(pW, rayDir, pHit, hm, segmentLength)) return rt1(0.0f);
-    return pt.transmittanceOverFreeSegment(pW, rayDir, segmentLength, rgb);
+    RT Tr = pt.transmittanceOverFreeSegment(pW, rayDir, segmentLength, rgb);
+    if constexpr (SC::kHasVolume) Tr *= pt.volumeTransmittance(pW, rayDir, segmentLength, rnd);
+    (void)rnd;
+    return Tr;
   }
 
   // directSurfaceLighting (1221-1295).  A BSDF value of exactly zero makes a
@@ -41,7 +46,7 @@ struct Mega {
       if (averageComponent(Li) > RADIANCE_EPSILON) {
         RT f = pt.evaluateBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true);
         if (anyNonzero(f)) {
-          RT Tr = transmittanceOverSegment(pW, woutputW, P.maxLengthScale, rgb);
+          RT Tr = transmittanceOverSegment(pW, woutputW, P.maxLengthScale, rgb, rnd);
           Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
           if (averageComponent(Li) > RADIANCE_EPSILON) {
             float bsdfPdf = max(PDF_EPSILON, pt.pdfBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true));
@@ -56,7 +61,7 @@ struct Mega {
       if (averageComponent(Li) > RADIANCE_EPSILON) {
         RT f = pt.evaluateBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true);
         if (anyNonzero(f)) {
-          RT Tr = transmittanceOverSegment(pW, woutputW, P.maxLengthScale, rgb);
+          RT Tr = transmittanceOverSegment(pW, woutputW, P.maxLengthScale, rgb, rnd);
           Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
           if (averageComponent(Li) > RADIANCE_EPSILON) {
             float bsdfPdf = max(PDF_EPSILON, pt.pdfBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true));
@@ -76,7 +81,7 @@ struct Mega {
         woutputL = worldToLocal(woutputW, basis);
         RT f = pt.evaluateBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true);
         if (anyNonzero(f)) {
-          RT Tr = transmittanceOverSegment(pW, woutputW, hitDist, rgb);
+          RT Tr = transmittanceOverSegment(pW, woutputW, hitDist, rgb, rnd);
           Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
           if (averageComponent(Li) > RADIANCE_EPSILON) {
             float bsdfPdf = max(PDF_EPSILON, pt.pdfBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true));
@@ -98,7 +103,7 @@ struct Mega {
       float skyPdf = 0.0f;
       RT Li = pt.sampleSkyInVolume(rgb, rnd, woutputW, skyPdf);
       if (averageComponent(Li) > RADIANCE_EPSILON) {
-        RT Tr = transmittanceOverSegment(pW, woutputW, P.maxLengthScale, rgb);
+        RT Tr = transmittanceOverSegment(pW, woutputW, P.maxLengthScale, rgb, rnd);
         Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
         if (averageComponent(Li) > RADIANCE_EPSILON) {
           float PF = phaseFunction(dot(-rayDir, -woutputW), P.atmosphereAnisotropy);
@@ -111,7 +116,7 @@ struct Mega {
       float sunPdf = 0.0f;
       RT Li = pt.sampleSunInVolume(rgb, rnd, woutputW, sunPdf);
       if (averageComponent(Li) > RADIANCE_EPSILON) {
-        RT Tr = transmittanceOverSegment(pW, woutputW, P.maxLengthScale, rgb);
+        RT Tr = transmittanceOverSegment(pW, woutputW, P.maxLengthScale, rgb, rnd);
         Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
         if (averageComponent(Li) > RADIANCE_EPSILON) {
           float PF = phaseFunction(dot(-rayDir, -woutputW), P.atmosphereAnisotropy);
@@ -127,11 +132,64 @@ struct Mega {
       if (averageComponent(Li) > RADIANCE_EPSILON) {
         float hitDist = max(0.0f, length(pHit - pW) - 3.0f * P.minLengthScale);
         vec3 woutputW = normalize(pHit - pW);
-        RT Tr = transmittanceOverSegment(pW, woutputW, hitDist, rgb);
+        RT Tr = transmittanceOverSegment(pW, woutputW, hitDist, rgb, rnd);
         Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
         if (averageComponent(Li) > RADIANCE_EPSILON) {
           float PF = phaseFunction(dot(-rayDir, -woutputW), P.atmosphereAnisotropy);
           Ldirect += PF * Li / max(PDF_EPSILON, sphPdf);
+        }
+      }
+    }
+#endif
+    return min(rt1(P.radianceClamp), Ldirect);
+  }
+
+  // Direct light at a scattering point of the heterogeneous medium: one sample
+  // of each light (the sky uniformly over the sphere, the sun's cone, the
+  // sphere light's visible cap), shadowed by geometry and weighted by the
+  // medium's ratio-tracking transmittance, combined with the phase-function
+  // sample of the continuation by the balance heuristic.  The light pdfs go
+  // back in skyPdf / sunPdf / sphPdf for the continuation's MIS weights.
+  SNL_HD RT directMediumLighting(const vec3& pW, const vec3& rayDir, float g, const vec3& rgb, vec4& rnd,
+                                 float& skyPdf, float& sunPdf, float& sphPdf) {
+    RT Ldirect = rt1(0.0f);
+    if (P.skyPower > RADIANCE_EPSILON) {
+      vec3 woutputW = sampleSphereUnit(rnd);
+      skyPdf = 1.0f / (4.0f * float(M_PI));
+      RT Li = pt.environmentRadiance(woutputW, rgb);
+      if (averageComponent(Li) > RADIANCE_EPSILON) {
+        RT Tr = transmittanceOverSegment(pW, woutputW, P.maxLengthScale, rgb, rnd);
+        Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
+        if (averageComponent(Li) > RADIANCE_EPSILON) {
+          float PF = phaseFunction(dot(rayDir, woutputW), g);
+          Ldirect += PF * Li / skyPdf * powerHeuristic(skyPdf, PF);
+        }
+      }
+    }
+    if (P.sunPower > RADIANCE_EPSILON) {
+      vec3 woutputW(0.0f);
+      RT Li = pt.sampleSunInVolume(rgb, rnd, woutputW, sunPdf);
+      if (averageComponent(Li) > RADIANCE_EPSILON) {
+        RT Tr = transmittanceOverSegment(pW, woutputW, P.maxLengthScale, rgb, rnd);
+        Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
+        if (averageComponent(Li) > RADIANCE_EPSILON) {
+          float PF = phaseFunction(dot(rayDir, woutputW), g);
+          Ldirect += PF * Li / max(PDF_EPSILON, sunPdf) * powerHeuristic(sunPdf, PF);
+        }
+      }
+    }
+#ifdef HAS_SPHERE_LIGHT
+    if (P.sphereLightPower > RADIANCE_EPSILON) {
+      vec3 pHit;
+      RT Li = pt.sampleSphereLight(rgb, rnd, pW, pHit, sphPdf);
+      if (averageComponent(Li) > RADIANCE_EPSILON) {
+        float hitDist = max(0.0f, length(pHit - pW) - 3.0f * P.minLengthScale);
+        vec3 woutputW = normalize(pHit - pW);
+        RT Tr = transmittanceOverSegment(pW, woutputW, hitDist, rgb, rnd);
+        Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
+        if (averageComponent(Li) > RADIANCE_EPSILON) {
+          float PF = phaseFunction(dot(rayDir, woutputW), g);
+          Ldirect += PF * Li / max(PDF_EPSILON, sphPdf) * powerHeuristic(sphPdf, PF);
         }
       }
     }
@@ -231,7 +289,7 @@ struct Mega {
     if (P.skyPower > RADIANCE_EPSILON) {
       RT Li = pt.sampleSkyAtSurface(basis, rgb, rnd, woutputL, woutputW, skyPdf);
       if (averageComponent(Li) > RADIANCE_EPSILON) {
-        RT Tr = transmittanceOverSegment(pW + dPw, woutputW, P.maxLengthScale, rgb);
+        RT Tr = transmittanceOverSegment(pW + dPw, woutputW, P.maxLengthScale, rgb, rnd);
         Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
         if (averageComponent(Li) > RADIANCE_EPSILON) {
           float misWeight = powerHeuristic(skyPdf, pdfHemisphereCosineWeighted(woutputL));
@@ -242,7 +300,7 @@ struct Mega {
     if (P.sunPower > RADIANCE_EPSILON) {
       RT Li = pt.sampleSunAtSurface(basis, rgb, rnd, woutputL, woutputW, sunPdf);
       if (averageComponent(Li) > RADIANCE_EPSILON) {
-        RT Tr = transmittanceOverSegment(pW + dPw, woutputW, P.maxLengthScale, rgb);
+        RT Tr = transmittanceOverSegment(pW + dPw, woutputW, P.maxLengthScale, rgb, rnd);
         Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
         if (averageComponent(Li) > RADIANCE_EPSILON) {
           float misWeight = powerHeuristic(sunPdf, pdfHemisphereCosineWeighted(woutputL));
@@ -258,7 +316,7 @@ struct Mega {
         float hitDist = max(0.0f, length(pHit - pW) - 3.0f * P.minLengthScale);
         woutputW = normalize(pHit - pW);
         woutputL = worldToLocal(woutputW, basis);
-        RT Tr = transmittanceOverSegment(pW + dPw, woutputW, hitDist, rgb);
+        RT Tr = transmittanceOverSegment(pW + dPw, woutputW, hitDist, rgb, rnd);
         Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
         if (averageComponent(Li) > RADIANCE_EPSILON) {
           float misWeight = powerHeuristic(sphPdf, pdfHemisphereCosineWeighted(woutputL));
@@ -294,6 +352,39 @@ struct Mega {
   // the loop's increment and condition: whether the path goes on
   SNL_HD static bool advance(Path& ps, int maxBounces) { return ++ps.vertex <= maxBounces; }
 
+  // A real collision of the ray (ps.pW, ps.rayDir) with the heterogeneous
+  // medium at distance tc, before the surface it would hit: the vertex
+  // scatters there.  The sphere light, if the ray passes it first, adds its
+  // emission as on the way to a surface hit (1721-1735); Beer absorption inside
+  // a dielectric applies over tc (1739-1745); the throughput takes the
+  // albedo; direct light as in a surface vertex (not inside a dielectric,
+  // 1846-1850); the continuation follows the phase function.
+  SNL_HD bool mediumVertex(Path& ps, float tc, const vec3& rgb, vec4& rnd, int maxBounces) {
+#ifdef HAS_SPHERE_LIGHT
+    {
+      vec3 pLightHit;
+      RT LiSph = (P.sphereLightPower < RADIANCE_EPSILON) ? rt1(0.0f) : pt.sphereLightRadiance(ps.pW, ps.rayDir, rgb, pLightHit);
+      if (averageComponent(LiSph) > RADIANCE_EPSILON && length(pLightHit - ps.pW) < tc)
+        ps.L += ps.throughput * ps.misWeightSph * LiSph;
+    }
+#endif
+    if constexpr (SC::kHasDielectric) {
+      if (ps.inDielectric) ps.throughput *= exp(-tc * pt.DIELECTRIC_ABSORPTION_EVAL(ps.pW, rgb));
+    }
+    ps.pW += tc * ps.rayDir;
+    ps.throughput *= pt.volumeAlbedo(ps.pW, rgb);
+    float g = pt.volumeAnisotropy(ps.pW);
+    float skyPdf = 0.0f, sunPdf = 0.0f, sphPdf = 0.0f;
+    if (!ps.inDielectric) ps.L += ps.throughput * directMediumLighting(ps.pW, ps.rayDir, g, rgb, rnd, skyPdf, sunPdf, sphPdf);
+    float phasePdf = 0.0f;
+    ps.rayDir = sampleHenyeyGreenstein(ps.rayDir, g, rnd, phasePdf);
+    ps.misWeightSky = powerHeuristic(phasePdf, skyPdf);
+    ps.misWeightSun = powerHeuristic(phasePdf, sunPdf);
+    ps.misWeightSph = powerHeuristic(phasePdf, sphPdf);
+    if (maxComponent(ps.throughput) < THROUGHPUT_EPSILON) return false;
+    return advance(ps, maxBounces);
+  }
+
   // the body of cameraPath's vertex loop for vertex ps.vertex: `break` ends the
   // path (false), `continue` and the end of the body go to the next vertex
   SNL_HD bool vertexStep(Path& ps, float wl, const vec3& rgb, vec4& rnd, int maxBounces, long long& normals) {
@@ -315,9 +406,13 @@ struct Mega {
     int hitMaterial;
     bool hit = pt.traceRay(pW, rayDir, pW_next, hitMaterial, P.maxLengthScale);
     float rayLength = P.maxLengthScale;
-    if (hit) {
-      rayLength = length(pW_next - pW);
-    } else {
+    if (hit) rayLength = length(pW_next - pW);
+    if constexpr (SC::kHasVolume) {
+      // delta tracking up to the surface hit (or maxLengthScale on a miss)
+      float tc = pt.volumeFreeFlight(pW, rayDir, rayLength, rnd);
+      if (tc < rayLength) return mediumVertex(ps, tc, rgb, rnd, maxBounces);
+    }
+    if (!hit) {
       float lightSegmentLength = P.maxLengthScale;
       RT Tr = rt1(1.0f);
       bool litSph = false;

@@ -102,6 +102,7 @@
   X(vec3, SNL_T_VEC3, fogTint)                                    \
   X(float, SNL_T_FLOAT, volumeEmission)                           \
   X(vec3, SNL_T_VEC3, volumeEmissionColorRGB)                     \
+  X(float, SNL_T_FLOAT, volumeExtinction)                         \
   X(vec2, SNL_T_VEC2, mousePick)                                  \
   X(float, SNL_T_FLOAT, exposure)                                 \
   X(float, SNL_T_FLOAT, invGamma)                                 \

@@ -15,6 +15,7 @@
 #define MAT_DIELE 0
 #define MAT_METAL 1
 #define MAT_SURFA 2
+#define MAT_VOLUM 3  // the pick program's medium bound (pick-fragment-shader.glsl:24)
 
 namespace glsl {
 // Hachisuka's combined multiplicative LCG evaluated with Schrage's method in

@@ -143,6 +143,34 @@ SNL_HD vec3 samplePhaseFunction(const vec3& rayDir, float g, vec4& rnd) {
   Basis basis = makeBasis(rayDir);
   return costheta * rayDir + sintheta * (cos(phi) * basis.tW + sin(phi) * basis.bW);
 }
+// Henyey-Greenstein sampling by inversion of its CDF in cos(theta) (the
+// published form, e.g. Pharr, Jakob & Humphreys, Physically Based Rendering,
+// 3rd ed., 15.2.3): the heterogeneous medium's phase sampling.  Returns wo
+// with cos(theta) = dot(dir, wo) distributed as phaseFunction(., g), and that
+// density in pdf.  Draws: cos(theta), then the azimuth.
+SNL_HD vec3 sampleHenyeyGreenstein(const vec3& dir, float g, vec4& rnd, float& pdf) {
+  float xi = snl_rand(rnd);
+  float cosTheta;
+  if (abs(g) < 1.0e-3f) cosTheta = 1.0f - 2.0f * xi;
+  else {
+    float sq = (1.0f - g * g) / (1.0f - g + 2.0f * g * xi);
+    cosTheta = (1.0f + g * g - sq * sq) / (2.0f * g);
+  }
+  cosTheta = clamp(cosTheta, -1.0f, 1.0f);
+  float sinTheta = sqrt(max(0.0f, 1.0f - cosTheta * cosTheta));
+  float phi = 2.0f * float(M_PI) * snl_rand(rnd);
+  Basis b = makeBasis(dir);
+  pdf = phaseFunction(cosTheta, g);
+  return localToWorld(vec3(sinTheta * cos(phi), sinTheta * sin(phi), cosTheta), b);
+}
+// a uniformly distributed unit direction (density 1 / 4 pi): the sky sample of
+// the heterogeneous medium's direct lighting
+SNL_HD vec3 sampleSphereUnit(vec4& rnd) {
+  float z = 1.0f - 2.0f * snl_rand(rnd);
+  float r = sqrt(max(0.0f, 1.0f - z * z));
+  float phi = 2.0f * float(M_PI) * snl_rand(rnd);
+  return vec3(r * cos(phi), r * sin(phi), z);
+}
 
 // ------------------------------------------------------------ Beckmann microfacet (379-441)
 SNL_HD float microfacetEval(const vec3& m, float roughness) {
@@ -271,11 +299,15 @@ struct PT {
 
   // First-hit marcher of the normals and pick programs (traceDistance,
   // normals-fragment-shader.glsl:70-113): t starts at 0 and the step precedes
-  // the evaluation; `iters` counts non-hit evaluations.
-  SNL_HD bool traceDistance(const vec3& start, const vec3& dir, float maxDist, vec3& hit, int& material) {
+  // the evaluation; `iters` counts non-hit evaluations.  `volume`: the pick
+  // program's HAS_VOLUME branch (pick-fragment-shader.glsl:74-100), where the
+  // medium's bound SDF_VOLUME is hit last in priority as MAT_VOLUM.
+  SNL_HD bool traceDistance(const vec3& start, const vec3& dir, float maxDist, vec3& hit, int& material,
+                            bool volume = false) {
     material = MAT_INVAL;
     float sdf;
     sdfStep(start, false, sdf);
+    if constexpr (SC::kHasVolume) if (volume) sdf = min(sdf, abs(S.SDF_VOLUME(start)));
     float InitialSign = sign(sdf);
     float t = 0.0f;
     int iters = 0;
@@ -285,6 +317,13 @@ struct PT {
       vec3 pW = start + t * dir;
       int m = sdfStep(pW, true, sdf);
       if (m != MAT_INVAL) { material = m; break; }
+      if constexpr (SC::kHasVolume) {
+        if (volume) {
+          float sv = abs(S.SDF_VOLUME(pW));
+          if (sv < P.minLengthScale) { material = MAT_VOLUM; break; }
+          sdf = min(sdf, sv);
+        }
+      }
       iters++;
     }
     hit = start + t * dir;
@@ -645,6 +684,94 @@ struct PT {
       return albedo_norm * (1.0f - averageComponent(Tr));
     }
     return 0.0f;
+  }
+
+  // ---------------- heterogeneous medium (HAS_VOLUME)
+  // The reference's scene API declares a medium's hooks -- SDF_VOLUME, and
+  // VOLUME_EXTINCTION(_MAX), VOLUME_SCATTERING/ABSORPTION_COLOR and
+  // VOLUME_ANISOTROPY with their injected defaults (js/sceneAPI.js:124-131,
+  // js/renderer.js:258-263) -- but its path tracer never calls them.  Here a
+  // scene that defines SDF_VOLUME gets that medium, in place of the
+  // homogeneous atmosphere:
+  //   * it lies where SDF_VOLUME(X) <= 0, and SDF_VOLUME(X) = d > 0 means no
+  //     medium within distance d of X;
+  //   * grey extinction sigma(X) = VOLUME_EXTINCTION(volumeExtinction, X) /
+  //     lengthScale, zero outside, bounded by the majorant sigmaMax =
+  //     VOLUME_EXTINCTION_MAX(volumeExtinction) / lengthScale;
+  //   * single-scattering albedo s / (s + a) per channel, with s and a the
+  //     VOLUME_SCATTERING_COLOR / VOLUME_ABSORPTION_COLOR of the Volume
+  //     material's scatteringColor / absorptionColor;
+  //   * Henyey-Greenstein phase function of g = VOLUME_ANISOTROPY(anisotropy, X).
+  // Free flights are sampled by delta tracking and shadow rays weighted by
+  // ratio tracking, both against sigmaMax.  Where SDF_VOLUME exceeds
+  // kVolumeSkip / sigmaMax the walk jumps that distance: nothing there can
+  // collide, and the exponential's memorylessness lets tracking restart after
+  // the jump.  At most kVolumeSteps steps per segment.
+  static constexpr int kVolumeSteps = 1024;
+  static constexpr float kVolumeSkip = 0.0625f;
+  SNL_HD float volumeSigmaMax() {
+    if constexpr (SC::kHasVolume) return S.VOLUME_EXTINCTION_MAX(P.volumeExtinction) / P.lengthScale;
+    return 0.0f;
+  }
+  SNL_HD float volumeSigma(const vec3& X) {
+    if constexpr (SC::kHasVolume) return S.VOLUME_EXTINCTION(P.volumeExtinction, X) / P.lengthScale;
+    return 0.0f;
+  }
+  SNL_HD float volumeBound(const vec3& X) {
+    if constexpr (SC::kHasVolume) return S.SDF_VOLUME(X);
+    return 1.0e20f;
+  }
+  SNL_HD RT volumeAlbedo(const vec3& X, const vec3& rgb) {
+    if constexpr (SC::kHasVolume) {
+      vec3 sc = S.VOLUME_SCATTERING_COLOR(P.atmosphereScatteringCoeffRGB, X);
+      vec3 ab = S.VOLUME_ABSORPTION_COLOR(P.atmosphereExtinctionCoeffRGB - P.atmosphereScatteringCoeffRGB, X);
+      return rgbToAlbedo(sc / max(sc + ab, vec3(DENOM_TOLERANCE)), rgb);
+    }
+    return rt1(0.0f);
+  }
+  SNL_HD float volumeAnisotropy(const vec3& X) {
+    if constexpr (SC::kHasVolume) return S.VOLUME_ANISOTROPY(P.atmosphereAnisotropy, X);
+    return 0.0f;
+  }
+  // delta tracking over [0, tmax) of the ray (o, d): the distance of the first
+  // real collision, or tmax if none
+  SNL_HD float volumeFreeFlight(const vec3& o, const vec3& d, float tmax, vec4& rnd) {
+    float sigmaMax = volumeSigmaMax();
+    if (!(sigmaMax > 0.0f)) return tmax;
+    float skip = kVolumeSkip / sigmaMax;
+    float t = 0.0f;
+    for (int n = 0; n < kVolumeSteps; ++n) {
+      float dv = volumeBound(o + t * d);
+      if (dv > skip) {
+        t += dv;
+        if (t >= tmax) return tmax;
+        continue;
+      }
+      t -= log(1.0f - snl_rand(rnd)) / sigmaMax;
+      if (t >= tmax) return tmax;
+      if (snl_rand(rnd) * sigmaMax < volumeSigma(o + t * d)) return t;
+    }
+    return tmax;
+  }
+  // ratio tracking over [0, tmax): an unbiased estimate of the medium's
+  // transmittance along the segment
+  SNL_HD float volumeTransmittance(const vec3& o, const vec3& d, float tmax, vec4& rnd) {
+    float sigmaMax = volumeSigmaMax();
+    if (!(sigmaMax > 0.0f)) return 1.0f;
+    float skip = kVolumeSkip / sigmaMax;
+    float T = 1.0f, t = 0.0f;
+    for (int n = 0; n < kVolumeSteps; ++n) {
+      float dv = volumeBound(o + t * d);
+      if (dv > skip) {
+        t += dv;
+        if (t >= tmax) break;
+        continue;
+      }
+      t -= log(1.0f - snl_rand(rnd)) / sigmaMax;
+      if (t >= tmax) break;
+      T *= 1.0f - volumeSigma(o + t * d) / sigmaMax;
+    }
+    return T;
   }
 
   // ---------------- lights (1051-1212)

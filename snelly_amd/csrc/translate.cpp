@@ -290,6 +290,7 @@ class Translator {
     flag("kHasAtmosphere", "HAS_ATMOSPHERE");
     flag("kHasFog", "HAS_FOG");
     flag("kHasVolumeEmission", "HAS_VOLUME_EMISSION");
+    flag("kHasVolume", "HAS_VOLUME");
     flag("kHasNormalMap", "HAS_NORMALMAP");
     flag("kHasSurfaceNM", "HAS_SURFACE_NORMALMAP");
     flag("kHasMetalNM", "HAS_METAL_NORMALMAP");

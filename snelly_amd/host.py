@@ -171,7 +171,11 @@ class Surface:
 
 
 class Volume:
-    """js/materials.js:193-275"""
+    """js/materials.js:193-275.  `extinction` is the grey extinction of a
+    scene's SDF_VOLUME medium (the property the reference's volume scenes set,
+    e.g. exampleScenes/volumes/pyroclastic.html: volume.extinction), passed to
+    VOLUME_EXTINCTION / VOLUME_EXTINCTION_MAX as the uniform volumeExtinction;
+    scatteringColor / absorptionColor / anisotropy then describe that medium."""
 
     def __init__(self):
         self.atmosphereMinX, self.atmosphereMaxX = -10.0, 10.0
@@ -188,6 +192,7 @@ class Volume:
         self.fogMaxOpticalDepth = 5.0
         self.fogEmission = [1.0, 1.0, 1.0]
         self.fogTint = [0.5, 0.5, 0.5]
+        self.extinction = 1.0
 
     def uniforms(self, s):
         ext = [a + b for a, b in zip(self.absorptionColor, self.scatteringColor)]
@@ -199,7 +204,7 @@ class Volume:
                 "volumeEmission": self.emission, "volumeEmissionColorRGB": self.emissionColor,
                 "fogEnable": int(bool(self.fogEnable)), "fogMFP": 10.0 ** self.fogMFP,
                 "fogMaxOpticalDepth": self.fogMaxOpticalDepth, "fogEmission": self.fogEmission,
-                "fogTint": self.fogTint}
+                "fogTint": self.fogTint, "volumeExtinction": self.extinction}
 
 
 class Metal:
@@ -616,13 +621,19 @@ def assemble(scene_glsl, r, mats):
         shader += "\n vec3 VOLUME_EMISSION(vec3 emission, vec3 X) { return emission; }\n"
     has_sphere_light = r.sphereLightPower > -7.0
     has_geometry = has_surface or has_metal or has_diele
-    if not has_geometry:
-        raise native.SnellyError("Scene must define at least one of: SDF_SURFACE, SDF_METAL, SDF_DIELECTRIC")
+    # the reference tests `hasGeometry || hasVolume` without declaring hasVolume
+    # (js/renderer.js:267-271); here a scene that defines SDF_VOLUME has the
+    # heterogeneous medium (HAS_VOLUME), which replaces the homogeneous
+    # atmosphere: the Volume material's colours then describe the medium
+    has_volume = has_hook(shader, "SDF_VOLUME")
+    if not (has_geometry or has_volume):
+        raise native.SnellyError("Scene must define at least one of: SDF_SURFACE, SDF_METAL, SDF_DIELECTRIC, or SDF_VOLUME")
     v = mats.volume
-    has_atm = any(c > 0.0 for c in v.scatteringColor) or any(c > 0.0 for c in v.absorptionColor)
+    has_atm = not has_volume and (any(c > 0.0 for c in v.scatteringColor) or any(c > 0.0 for c in v.absorptionColor))
     defines = ""
     flags = [(custom_cam, "HAS_CUSTOM_CAMERA"), (has_surface, "HAS_SURFACE"), (has_metal, "HAS_METAL"),
-             (has_diele, "HAS_DIELECTRIC"), (has_atm, "HAS_ATMOSPHERE"), (v.fogEnable, "HAS_FOG"),
+             (has_diele, "HAS_DIELECTRIC"), (has_atm, "HAS_ATMOSPHERE"), (has_volume, "HAS_VOLUME"),
+             (v.fogEnable, "HAS_FOG"),
              (has_vol_emission, "HAS_VOLUME_EMISSION"), (has_geometry, "HAS_GEOMETRY"),
              (surface_nm or metal_nm or diele_nm, "HAS_NORMALMAP"), (surface_nm, "HAS_SURFACE_NORMALMAP"),
              (metal_nm, "HAS_METAL_NORMALMAP"), (diele_nm, "HAS_DIELECTRIC_NORMALMAP"),

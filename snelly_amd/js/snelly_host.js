@@ -473,6 +473,10 @@ class Volume extends Material {
     this.emission = 0.0; this.emissionColor = [1, 1, 1];
     this.fogEnable = false; this.fogMFP = 0.0; this.fogMaxOpticalDepth = 5.0;
     this.fogEmission = [1, 1, 1]; this.fogTint = [0.5, 0.5, 0.5];
+    // grey extinction of a scene's SDF_VOLUME medium (the property the
+    // reference's volume scenes set, e.g. exampleScenes/volumes/pyroclastic.html),
+    // passed to VOLUME_EXTINCTION / VOLUME_EXTINCTION_MAX
+    this.extinction = 1.0;
   }
   syncShader(s) {
     const ext = [0, 1, 2].map(c => this.absorptionColor[c] + this.scatteringColor[c]);
@@ -489,6 +493,7 @@ class Volume extends Material {
     s.uniformF('fogMaxOpticalDepth', this.fogMaxOpticalDepth);
     s.uniform3Fv('fogEmission', this.fogEmission);
     s.uniform3Fv('fogTint', this.fogTint);
+    s.uniformF('volumeExtinction', this.extinction);
   }
 }
 
@@ -780,13 +785,19 @@ class Renderer {
     if (!has('VOLUME_EMISSION')) { hasVolumeEmission = false; shader += '\n vec3 VOLUME_EMISSION(vec3 emission, vec3 X) { return emission; }\n'; }
     const hasSphereLight = this.sphereLightPower > -7.0;
     const hasGeometry = hasSurface || hasMetal || hasDielectric;
-    if (!hasGeometry) GLU.fail('Scene must define at least one of: SDF_SURFACE, SDF_METAL, SDF_DIELECTRIC, or SDF_VOLUME');
+    // the reference tests `hasGeometry || hasVolume` without declaring
+    // hasVolume (js/renderer.js:267-271); here a scene defining SDF_VOLUME has
+    // the heterogeneous medium (HAS_VOLUME) in place of the homogeneous
+    // atmosphere, described by the Volume material's colours and extinction
+    const hasVolume = has('SDF_VOLUME');
+    if (!(hasGeometry || hasVolume)) GLU.fail('Scene must define at least one of: SDF_SURFACE, SDF_METAL, SDF_DIELECTRIC, or SDF_VOLUME');
     const vol = this.snelly.getVolume();
     const nz = (c) => c[0] > 0 || c[1] > 0 || c[2] > 0;
-    const hasAtmosphere = nz(vol.scatteringColor) || nz(vol.absorptionColor);
+    const hasAtmosphere = !hasVolume && (nz(vol.scatteringColor) || nz(vol.absorptionColor));
     const hasNM = hasSurfaceNM || hasMetalNM || hasDielectricNM;
     const d = [[hasCustomCamera, 'HAS_CUSTOM_CAMERA'], [hasSurface, 'HAS_SURFACE'], [hasMetal, 'HAS_METAL'],
-      [hasDielectric, 'HAS_DIELECTRIC'], [hasAtmosphere, 'HAS_ATMOSPHERE'], [vol.fogEnable, 'HAS_FOG'],
+      [hasDielectric, 'HAS_DIELECTRIC'], [hasAtmosphere, 'HAS_ATMOSPHERE'], [hasVolume, 'HAS_VOLUME'],
+      [vol.fogEnable, 'HAS_FOG'],
       [hasVolumeEmission, 'HAS_VOLUME_EMISSION'], [hasGeometry, 'HAS_GEOMETRY'], [hasNM, 'HAS_NORMALMAP'],
       [hasSurfaceNM, 'HAS_SURFACE_NORMALMAP'], [hasMetalNM, 'HAS_METAL_NORMALMAP'],
       [hasDielectricNM, 'HAS_DIELECTRIC_NORMALMAP'], [this.interactive, 'INTERACTIVE_MODE'],

@@ -100,6 +100,96 @@ float SDF_@MAT@(vec3 X) { return length(X) - 1.0; }
 """
 
 
+# Heterogeneous media, written with the hooks the reference's scene API names
+# for them (SDF_VOLUME, VOLUME_EXTINCTION, VOLUME_EXTINCTION_MAX;
+# js/sceneAPI.js:124-131, js/renderer.js:258-263).  The oracle restates each
+# (oracle/oracle.cpp, kinds MEDIUM_*) with the same float operations.
+#
+# CLOUD_GLSL: density max(0, 1 - |p| + 0.3 fbm3(p)) in cloud coordinates
+# p = X / 4, with fbm3 five octaves of value noise -- lattice values from an
+# integer hash (nested quadratic maps mod the prime 32749; every product stays
+# below 2^31, so GLSL int arithmetic is exact) scaled by 2^-15 into [0, 1),
+# smoothstep-blended -- weighted 1/2, 1/4, ..., 1/32 at doubling frequencies
+# (base 2 per unit of p, offsets between octaves), so |fbm3| < 31/32: the
+# density stays below 1.3 (the majorant) and vanishes outside |p| < 1.3 (the
+# bound |X| < 5.2).  The extinction is volume.extinction (1/4) per world unit
+# times the density, the optical depth of the unit-scale cloud (about 1 along
+# a diameter).  At this scale the cloud envelops the Mandelbox of scale 2.5
+# (half-size 4.67); at unit scale it would sit in the fractal's hollow centre,
+# behind a shell that paths of 16 bounces through the glass do not cross.
+CLOUD_GLSL = """
+int latticeHash(int x)
+{
+    int h = x % 32749;
+    return (h * h + 7919) % 32749;
+}
+float latticeValue(int x)
+{
+    return float(latticeHash(x)) * (1.0 / 32768.0);
+}
+float valueNoise(vec3 p)
+{
+    vec3 i = floor(p);
+    vec3 f = p - i;
+    vec3 u = f * f * (3.0 - 2.0 * f);
+    ivec3 c = ivec3(i);
+    int hx0 = latticeHash(c.x & 4095);
+    int hx1 = latticeHash((c.x + 1) & 4095);
+    int y0 = c.y & 4095;
+    int y1 = (c.y + 1) & 4095;
+    int z0 = c.z & 4095;
+    int z1 = (c.z + 1) & 4095;
+    int h00 = latticeHash(hx0 + y0);
+    int h10 = latticeHash(hx1 + y0);
+    int h01 = latticeHash(hx0 + y1);
+    int h11 = latticeHash(hx1 + y1);
+    float x00 = mix(latticeValue(h00 + z0), latticeValue(h10 + z0), u.x);
+    float x10 = mix(latticeValue(h01 + z0), latticeValue(h11 + z0), u.x);
+    float x01 = mix(latticeValue(h00 + z1), latticeValue(h10 + z1), u.x);
+    float x11 = mix(latticeValue(h01 + z1), latticeValue(h11 + z1), u.x);
+    return mix(mix(x00, x10, u.y), mix(x01, x11, u.y), u.z);
+}
+float fbm3(vec3 p)
+{
+    float sum = 0.0;
+    float amp = 0.5;
+    vec3 q = 2.0 * p;
+    for (int i = 0; i < 5; i++)
+    {
+        sum += amp * (2.0 * valueNoise(q) - 1.0);
+        amp *= 0.5;
+        q = 2.0 * q + vec3(1.7, 9.2, 3.4);
+    }
+    return sum;
+}
+float SDF_VOLUME(vec3 X) { return length(X) - 5.2; }
+float VOLUME_EXTINCTION(float extinction_ui, vec3 X)
+{
+    vec3 p = 0.25 * X;
+    return extinction_ui * max(0.0, 1.0 - length(p) + 0.3 * fbm3(p));
+}
+float VOLUME_EXTINCTION_MAX(float extinction_ui) { return 1.3 * extinction_ui; }
+"""
+
+# density max(0, 1 - |X|) in the unit ball (its line integrals have a closed form)
+LINEAR_BALL_GLSL = """
+float SDF_VOLUME(vec3 X) { return length(X) - 1.0; }
+float VOLUME_EXTINCTION(float extinction_ui, vec3 X) { return extinction_ui * max(0.0, 1.0 - length(X)); }
+"""
+
+# density 1 in the box |X_k| < h_k
+BOX_MEDIUM_GLSL = """
+float sdBoxMedium(vec3 X, vec3 h)
+{
+    vec3 d = abs(X) - h;
+    return min(max(d.x, max(d.y, d.z)), 0.0) + length(max(d, 0.0));
+}
+float SDF_VOLUME(vec3 X) { return sdBoxMedium(X, vec3(@H@)); }
+float VOLUME_EXTINCTION(float extinction_ui, vec3 X) { return sdBoxMedium(X, vec3(@H@)) <= 0.0 ? extinction_ui : 0.0; }
+"""
+MEDIUM_CLOUD, MEDIUM_LINEAR_BALL, MEDIUM_BOX = 1, 2, 3
+
+
 # FLOPs of one Menger SDF evaluation: 10 for the box, 24 per iteration
 MENGER_SDF_FLOPS = {n: 10 + 24 * n for n in range(1, 8)}
 SPHERE_PLANE_SDF_FLOPS = 8
@@ -128,6 +218,8 @@ class Scene:
         self.oracle_sdfs = []
         self.oracle_albedo_hook = 0
         self.oracle_emission_hook = 0  # 1: the VOLUME_EMISSION of VOLUME_EMISSION_GLSL
+        # the SDF_VOLUME medium restated for the oracle: (kind, a, b, c, d) or None
+        self.oracle_medium = None
         # algorithmic FLOPs of one march step (SDF evaluations of the enabled
         # materials + 8 for p = o + t*d and t += sign*sdf), from the counting
         # build of the oracle (pinned by tests/test_flops.py); None = varies
@@ -242,6 +334,77 @@ def mandelbox_dielectric(width=1920, height=1080, spp=256, scale=2.5, iterations
     s.materials.surface.ior = 1.0
     s.user_uniforms = {"Scale": float(scale), "MinRad2": 0.25}
     s.oracle_sdfs = [(0, 4, float(scale), float(iterations), float(bailout), 0.25), (2, 1, 0.5, 0, 0, 0)]
+    return s
+
+
+def mandelbox_volume(width=1920, height=1080, spp=256):
+    """The Mandelbox dielectric scene of mandelbox_dielectric (scale 2.5, 12
+    iterations, bailout 10, Cauchy dispersion, Beer absorption, nested Surface
+    sphere of radius 0.5) in a heterogeneous medium: density
+    max(0, 1 - |p| + 0.3 fbm3(p)) (5 octaves) in cloud coordinates p = X / 4
+    (CLOUD_GLSL), albedo 0.9, Henyey-Greenstein g = 0.3; 1920 x 1080, 256 spp,
+    16 bounces, spectral."""
+    s = mandelbox_dielectric(width, height, spp)
+    s.name = "mandelbox_volume"
+    s.glsl = s.glsl + CLOUD_GLSL
+    v = s.materials.volume
+    v.extinction = 0.25
+    v.scatteringColor = [0.9, 0.9, 0.9]
+    v.absorptionColor = [0.1, 0.1, 0.1]
+    v.anisotropy = 0.3
+    s.oracle_medium = (MEDIUM_CLOUD, 5.2, 0.25, 0.0, 0.0)
+    return s
+
+
+def medium_ball(width=32, height=32, spp=16, albedo=0.0, extinction=2.0, anisotropy=0.3, dispersive=False,
+                bounces=24):
+    """A medium alone -- density `extinction` * max(0, 1 - |X|) in the unit
+    ball -- under a uniform white sky 1, pinhole camera at (0, 0, 4), fov 45,
+    no pixel jitter (every sample of a pixel takes the same ray).  albedo 0:
+    a pure absorber, whose pixels have the closed-form transmittance of their
+    ray as expectation; albedo 1: a white furnace, every pixel 1."""
+    s = Scene("medium_ball", LINEAR_BALL_GLSL, width, height, spp)
+    r = s.renderer
+    r.dispersive = dispersive
+    r.maxBounces = bounces
+    r.skyPower = 0.0
+    r.sunPower = -7.0
+    r.filterRadius = 0.0
+    r.radianceClamp = 3.0
+    s.camera.position = [0.0, 0.0, 4.0]
+    s.camera.fov = 45.0
+    s.camera.aperture = -math.inf
+    v = s.materials.volume
+    v.extinction = extinction
+    v.scatteringColor = [albedo] * 3
+    v.absorptionColor = [1.0 - albedo] * 3
+    v.anisotropy = anisotropy
+    s.oracle_medium = (MEDIUM_LINEAR_BALL, 0.0, 0.0, 0.0, 0.0)
+    return s
+
+
+def absorber_box(width=40, height=32, spp=16, half=1.5, absorption=0.5, medium=True):
+    """sphere_plane inside a grey absorbing box |X_k| < half of extinction
+    `absorption` and no scattering: as the heterogeneous medium (SDF_VOLUME,
+    medium=True) or as the reference's homogeneous atmosphere (medium=False,
+    atmosphere bounds +-half, mfp 1).  The two describe the same transport."""
+    s = sphere_plane(width, height, spp)
+    v = s.materials.volume
+    if medium:
+        s.name = "absorber_box_medium"
+        h = ", ".join([repr(float(half))] * 3)
+        s.glsl = s.glsl + BOX_MEDIUM_GLSL.replace("@H@", h)
+        v.extinction = absorption
+        v.scatteringColor = [0.0, 0.0, 0.0]
+        v.absorptionColor = [1.0, 1.0, 1.0]
+        s.oracle_medium = (MEDIUM_BOX, float(half), float(half), float(half), 0.0)
+    else:
+        s.name = "absorber_box_atmosphere"
+        v.atmosphereMinX = v.atmosphereMinY = v.atmosphereMinZ = -half
+        v.atmosphereMaxX = v.atmosphereMaxY = v.atmosphereMaxZ = half
+        v.lof10_mfp = 0.0
+        v.scatteringColor = [0.0, 0.0, 0.0]
+        v.absorptionColor = [absorption] * 3
     return s
 
 

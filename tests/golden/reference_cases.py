@@ -259,6 +259,11 @@ CASES = {
                                 tweak=_ptsimple_subsurface),
     "subsurface": dict(builder="sphere_plane", kwargs=dict(width=40, height=32, spp=4), frames=4, seed=24,
                        tweak=_subsurface),
+    # the homogeneous atmosphere as a pure grey absorber in a box around the
+    # sphere: the transport the heterogeneous medium's box of
+    # scenes.absorber_box(medium=True) must reproduce in expectation
+    "absorber_box": dict(builder="absorber_box", kwargs=dict(width=32, height=24, spp=64, medium=False), frames=64,
+                         seed=41),
 }
 
 

@@ -79,3 +79,35 @@ def test_mandelbox_dielectric_1920x1080_256spp_strided_pixels(ctx):
     d = (ug - ur).astype(np.float64)
     se = d.std(axis=0, ddof=1) / np.sqrt(len(idx))
     assert np.all(np.abs(d.mean(axis=0)) <= 4.0 * se + 1e-6 * np.abs(ur).mean(axis=0)), (d.mean(axis=0), se)
+
+
+def test_mandelbox_volume_1920x1080_256spp_strided_pixels(ctx):
+    """config 3 at its size -- 1920 x 1080, 256 spp, 16 bounces, the
+    heterogeneous medium in the glass -- in the precise build and in the fast
+    build the benchmark times: first hits per pixel on a strided subset, and
+    per build the radiance of every 4099th pixel against the oracle as paired
+    differences whose mean lies within 4 standard errors of zero"""
+    W, H = 1920, 1080
+    s = scenes.mandelbox_volume(W, H, 256)
+    idx = np.arange(11, W * H, 4099)
+    xy = np.stack([idx % W + 0.5, idx // W + 0.5], -1).astype(np.float32)
+    o, r, _ = oracle_subset(s, W, H, 256, 4099, 11)
+    ur = r[idx, :3] * (r[idx, 3:] / (r[idx, 3:] - 1.0))
+    for flags in (native.BUILD_PRECISE, native.BUILD_FAST):
+        sess = host.Session(s, W, H, ctx, flags=flags)
+        sess.render(256)
+        g = sess.radiance().reshape(-1, 4).copy()
+        pk = sess.prog.pick(xy)
+        sess.close()
+        assert np.array_equal(g[idx, 3], r[idx, 3])
+        if flags == native.BUILD_PRECISE:
+            rp = o.pick(xy)
+            same = pk[:, 1] == rp[:, 1]
+            assert same.mean() >= 0.995
+            assert np.mean(np.abs(pk[same, 0] - rp[same, 0]) <= 1e-4 * np.abs(rp[same, 0])) >= 0.995
+        ug = g[idx, :3] * (g[idx, 3:] / (g[idx, 3:] - 1.0))
+        d = (ug - ur).astype(np.float64)
+        se = d.std(axis=0, ddof=1) / np.sqrt(len(idx))
+        print(f"config 3 flags {flags}: paired mean {d.mean(axis=0)} se {se}")
+        assert np.all(np.abs(d.mean(axis=0)) <= 4.0 * se + 1e-6 * np.abs(ur).mean(axis=0)), (flags, d.mean(axis=0), se)
+

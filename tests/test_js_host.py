@@ -26,18 +26,21 @@ REF_EXAMPLES = "/root/reference/exampleScenes"
 needs_node = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON), reason="node or the addon is missing")
 
 # Example scenes the reference itself cannot run:
-#  - volume-only scenes: compileShaders tests an undeclared `hasVolume` when no
-#    surface SDF exists (js/renderer.js:267-271) and throws;
 #  - meshmenger uses `size`, which it never declares (fractals/IFS/meshmenger.html:429);
 #  - texture_rocks defines SURFACE_NORMAL_MAP(vec3) while the shader calls it
 #    with (X, N) (pathtracer-fragment-shader.glsl:948);
 #  - the font scenes construct TextSDF, a browser font rasteriser that is not
 #    part of the reference.
 REFERENCE_FAILS = {
-    "physics/orbitals.html", "volumes/pyroclastic.html", "volumes/volumeBunny.html",
-    "volumes/volumeCheckerboard.html", "fractals/IFS/meshmenger.html", "textures/texture_rocks.html",
+    "fractals/IFS/meshmenger.html", "textures/texture_rocks.html",
     "fonts/font_sdf.html", "fonts/font_sdf2.html", "textures/texture_flesh.html",
 }
+# Volume-only scenes: the reference's compileShaders tests an undeclared
+# `hasVolume` when no surface SDF exists (js/renderer.js:267-271) and throws.
+# This host declares it (SDF_VOLUME defines the heterogeneous medium), so they
+# compile here.
+VOLUME_ONLY = {"physics/orbitals.html", "volumes/pyroclastic.html", "volumes/volumeBunny.html",
+               "volumes/volumeCheckerboard.html"}
 
 
 def run_node(args, timeout=300, env=None):
@@ -136,6 +139,7 @@ def test_reference_example_scenes_compile():
         results = list(ex.map(one, names))
     unexpected = [(n, err) for n, rc, err in results if (rc != 0) != (n in REFERENCE_FAILS)]
     assert not unexpected, unexpected
+    assert VOLUME_ONLY <= set(names) and not (VOLUME_ONLY & REFERENCE_FAILS)
 
 
 @needs_node

@@ -1,18 +1,29 @@
 """Benchmark: path samples/s of Snelly's spectral SDF path tracer on one MI355X.
 
-Workload (one step): the Menger-sponge metal scene -- 4 iterations, half-size 1,
-analytic complex IOR n = 0.2 + 0.8 t, k = 3.5 - 1.5 t, Beckmann roughness 0.05,
-spectral (hero wavelength), 8 bounces -- rendered at 1024 x 1024 with 64 path
-samples per pixel from a freshly seeded accumulation state, i.e. 67,108,864
-path samples.  Scene data are synthetic (an analytic SDF); nothing is read from
-disk.  Prints one JSON line.
+Headline (`value`), config 3: the Mandelbox dielectric (scale 2.5, 12
+iterations, bailout 10; Cauchy dispersion n = 1.5 + 0.0042/lambda^2 um, Beer
+absorption (0.1, 0.3, 0.5)) with a nested Surface sphere of radius 0.5, in a
+heterogeneous medium (density max(0, 1 - |p| + 0.3 fbm3(p)), 5 octaves,
+albedo 0.9, Henyey-Greenstein g = 0.3), 1920 x 1080, 256 spp, 16 bounces,
+spectral (hero wavelength), rendered from a freshly seeded state: 530,841,600
+path samples per step.  Scene data are synthetic (analytic SDFs and noise);
+nothing is read from disk.  Prints one JSON line.
 
-Also measured: the 64-scene batched workload rendered plain (one launch per
-scene and frame) and as one batch ("batched"), and a served run -- jobs
-arriving at 20/s, rendered to a target error by continuous batching, with
-p50/p99 time from arrival to target ("served").
+Also measured, in the same line:
+  secondary  config 2: Menger-4 metal with analytic complex IOR, 1024^2 x 64
+             spp, 8 bounces, and its first-hit depth and normal passes;
+  batched    config 4: 64 seeded scenes rendered by the plain path (each
+             job's render, job after job; and one launch per frame, as the
+             reference draws) and as one batch;
+  served     config 5: 2000 jobs arriving at 20/s, sides 64..2048, target
+             relative RMSE 0.5..5%, served by continuous batching for a fixed
+             time; p50/p99 time to target, unfinished jobs censored;
+  cpu_baseline  the oracle (the CPU restatement) on the host's cores.
 
-  python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--cpu-seconds 15] [--no-batch] [--served-jobs 200]
+  python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--cpu-seconds 12] [--no-batch]
+                  [--served-jobs 2000] [--served-seconds 100] [--no-secondary]
+  python bench.py --copy-yardstick [MiB]   a plain device copy, for scaling
+                                           HBM traffic counters
 """
 import argparse
 import json
@@ -31,17 +42,16 @@ os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("SNL_BENCH_HW_QUEUES", "16")
 
 METRIC = "path samples/s (plain vs batched/served) at RMSE≤1%; p50/p99 job latency"
 HBM_PEAK_GBS = 8000.0
-FP32_PEAK_TFLOPS = 157.3
-# HBM bytes per pixel per render of the queue kernels, from the FETCH_SIZE /
-# WRITE_SIZE counter runs scaled by the copy yardstick
-# (profiles/<kernel>_menger4_1024_traffic_valu.md): the persistent kernel's one
-# launch, the megakernel's two (frame 1 with the tile-cost atomics, then the
-# other frames in cost order)
-TRAFFIC_B_PER_PX = {"snl_persist": 67.1, "snl_mega": 144.3}
-# served: path samples per tick beyond each job's smallest quantum, given to the
-# jobs with the least remaining work first; chains per job: the fewest (up to
-# 128) that give it 2^18 lanes
-TICK_SAMPLES = 1 << 24
+FP32_PEAK_TFLOPS = 157.3           # MI355X FP32 vector, packed (dual-issue) FP32
+FP32_UNPACKED_TFLOPS = 78.6        # without packed FP32, which every code object here disables
+# HBM bytes per pixel per render of the megakernel, from the FETCH_SIZE /
+# WRITE_SIZE counter runs scaled by the copy yardstick (profiles/*_traffic*.md):
+# two launches per render (frame 1 with the tile-cost atomics, then the other
+# frames in cost order), each reading and writing the pixel's radiance and RNG
+TRAFFIC_B_PER_PX = {"mandelbox_volume": 140.6, "menger_metal": 144.3}
+# served: path samples per tick (every job's smallest quantum included, in
+# round-robin order); chains per job: the fewest (up to 128) that give it 2^18 lanes
+TICK_SAMPLES = 1 << 26
 CHAIN_LANES, MAX_CHAINS = 1 << 18, 128
 
 
@@ -50,17 +60,28 @@ def unbiased(rad):
     return rad[:, :3] * (rad[:, 3:4] / (rad[:, 3:4] - 1.0))
 
 
+def host_threads():
+    """every core this process may run on (the box's CPU share), unless
+    OMP_NUM_THREADS says otherwise"""
+    n = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return n or avail, avail, os.cpu_count() or 1
+
+
 def cpu_baseline(scene, width, height, spp, budget_s):
     """The CPU oracle (a C++ restatement of the reference shaders, OpenMP) on a
-    bounded, strided sample of the same workload's pixels."""
+    bounded, strided sample of the same workload's pixels, on all host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    threads = min(threads, 16)
+    threads, avail, ncpu = host_threads()
     o = oracle_py.Oracle(scene, width, height)
     npix = width * height
-    # calibrate on a sparse sample, then size the stride to the time budget
-    stride = 4096
+    # calibrate on a sparse sample with enough pixels to keep every thread
+    # busy, then size the stride to the time budget
+    stride = max(1, npix // (256 * threads))
     rad, rng = o.new_state(99)
     t = time.perf_counter()
     o.render(rad, rng, spp, stride, 0, threads)
@@ -73,44 +94,177 @@ def cpu_baseline(scene, width, height, spp, budget_s):
     dt = time.perf_counter() - t
     pixels = math.ceil(npix / stride)
     return {"value": pixels * spp / dt, "unit": "path samples/s", "cores": threads, "kind": "port",
+            "nproc": avail, "cpu_count": ncpu,
             "sample": f"every {stride}th pixel of the {width}x{height} image ({pixels} pixels) x {spp} spp, "
                       f"{dt:.1f} s on {threads} threads"}
 
 
+def path_counts(scene, width, height, spp, budget_s):
+    """The oracle's counting build (every float operation of the restated
+    path counted) over a strided sample of the workload's pixels: FLOPs and
+    transcendentals per sphere-tracer SDF step, per normal, and per camera
+    path for the rest (camera, RNG, BSDFs, lights, MIS, medium tracking,
+    colour).  The GPU's own step, normal and path counts then price a render."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py
+    threads = host_threads()[0]
+    o = oracle_py.Oracle(scene, width, height, counting=True)
+    npix = width * height
+    stride = 16381
+    rad, rng = o.new_state(77)
+    o.count_reset()
+    t = time.perf_counter()
+    o.render(rad, rng, spp, stride, 3, threads)
+    dt = max(time.perf_counter() - t, 1e-3)
+    stride = max(1, int(math.ceil(npix * (dt / math.ceil(npix / stride)) / budget_s)))
+    rad, rng = o.new_state(77)
+    o.count_reset()
+    o.render(rad, rng, spp, stride, 3, threads)
+    c = o.counts()
+    f, tr = c["flops"], c["transcendentals"]
+    paths = max(c["paths"], 1)
+    return {"flops_per_step": f["march"] / max(c["steps"], 1),
+            "flops_per_normal": f["normal"] / max(c["normals"], 1),
+            "other_flops_per_path": (f["shade"] + f["medium"]) / paths,
+            "medium_flops_per_path": f["medium"] / paths,
+            "trans_per_step": tr["march"] / max(c["steps"], 1),
+            "trans_per_normal": tr["normal"] / max(c["normals"], 1),
+            "other_trans_per_path": (tr["shade"] + tr["medium"]) / paths,
+            "oracle_steps_per_path": c["steps"] / paths, "oracle_normals_per_path": c["normals"] / paths,
+            "sample": f"counting build over every {stride}th pixel x {spp} spp ({c['paths']} paths)"}
+
+
+def roofline(counts, stats, renders, gpu_ms, kname, width, height, traffic_key):
+    """FP32 VALU roofline of the render kernel: algorithmic FLOPs per render
+    (the GPU's sphere-tracer steps, normals and paths priced by the counting
+    build) over the kernel's GPU time per render (hipEvents on its stream)"""
+    steps, normals, paths = stats["steps"] / renders, stats["normals"] / renders, stats["paths"] / renders
+    flops = steps * counts["flops_per_step"] + normals * counts["flops_per_normal"] + \
+        paths * counts["other_flops_per_path"]
+    trans = steps * counts["trans_per_step"] + normals * counts["trans_per_normal"] + \
+        paths * counts["other_trans_per_path"]
+    ms = gpu_ms / renders
+    achieved = flops / (ms * 1e-3) / 1e12
+    per_render = stats["launches"] / renders
+    bpp = TRAFFIC_B_PER_PX.get(traffic_key)
+    return {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+            "frac_unpacked_peak": round(achieved / FP32_UNPACKED_TFLOPS, 4),
+            "traffic": None if bpp is None else round(bpp * width * height),
+            "traffic_algorithmic": 64 * width * height * per_render, "per": "render", "kernel": kname,
+            "launches_per_render": per_render, "gpu_ms_per_render": round(ms, 2),
+            "flops_per_render": flops, "transcendentals_per_render": trans,
+            "steps_per_render": steps, "normal_evals_per_render": normals, "paths_per_render": paths,
+            "flops_counted": "every + - * / sqrt of the restated path (counting build of the oracle): "
+                             "sphere-tracer steps, normals, camera, RNG, BSDFs, lights, MIS, medium tracking; "
+                             "transcendentals reported apart",
+            "counts": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in counts.items()}}
+
+
+def plain(ctx, scene, steps, warmup, engines, tune_frames):
+    """the plain one-scene renderer: the fast build, engine and launch shape
+    autotuned (untimed), then `steps` renders of scene.spp frames each from a
+    freshly seeded state, timed between device synchronisations"""
+    from snelly_amd import autotune, host, native
+    W, H, spp = scene.width, scene.height, scene.spp
+    sess = host.Session(scene, W, H, ctx, flags=native.BUILD_FAST)
+    best, _, _ = autotune.tune(sess, frames=tune_frames, engines=engines)
+    for w in range(warmup):
+        sess.state.seed(1000 + w)
+        sess.render(spp)
+    ctx.sync()
+    images, elapsed = [], 0.0
+    stats = {"steps": 0, "normals": 0, "paths": 0, "launches": 0}
+    gpu_ms = 0.0
+    for k in range(steps):
+        sess.state.seed(1234 + k)
+        ctx.sync()
+        ts = time.perf_counter()
+        sess.render(spp)
+        ctx.sync()
+        elapsed += time.perf_counter() - ts
+        st = sess.prog.stats()
+        gpu_ms += st.ms_total
+        stats["steps"] += st.trace_steps
+        stats["normals"] += st.normal_evals
+        stats["paths"] += st.paths
+        stats["launches"] += st.trace_launches
+        if k < 2:  # read back outside the timed region
+            images.append(unbiased(sess.radiance()))
+    rr = None
+    if len(images) == 2:
+        a, b = images
+        rr = float(np.linalg.norm(a - b) / math.sqrt(2.0) / np.linalg.norm(0.5 * (a + b)))
+    return sess, best, elapsed, gpu_ms, stats, rr
+
+
+def first_hit_pass(ctx, scene, reps=3):
+    """config 2's first-hit depth and normal passes over every pixel: the pick
+    program (depth, material) and one frame of the normals program"""
+    from snelly_amd import host, native
+    W, H = scene.width, scene.height
+    sess = host.Session(scene, W, H, ctx, flags=native.BUILD_FAST)
+    xy = np.stack(np.meshgrid(np.arange(W) + 0.5, np.arange(H) + 0.5), -1).reshape(-1, 2).astype(np.float32)
+    sess.prog.pick(xy)
+    sess.prog.render_normals(sess.state, 1)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sess.prog.pick(xy)
+    pick_s = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sess.prog.render_normals(sess.state, 1)
+    ctx.sync()
+    normals_s = (time.perf_counter() - t0) / reps
+    sess.close()
+    return {"pixels": W * H, "depth_ms": round(pick_s * 1e3, 2), "normals_ms": round(normals_s * 1e3, 2),
+            "depth_includes": "pixel positions up and hit distances back over PCIe (16 B/px)"}
+
+
 def batched(ctx, steps, warmup):
-    """The batched workload (64 seeded scenes, scenes.batch_jobs): the same jobs
-    rendered plain -- one scene per launch and one launch per frame, job after
-    job, as the reference draws -- and as one snl_render_batch call."""
+    """config 4 (64 seeded scenes, scenes.batch_jobs): rendered by the plain
+    path -- each job's own render of all its frames, job after job, and one
+    launch per frame as the reference draws -- and as one snl_render_batch"""
     from snelly_amd import autotune, host, native, scenes
     jobs = scenes.batch_jobs()
     sessions = [host.Session(s, s.width, s.height, ctx, seed=500 + j, flags=native.BUILD_FAST)
                 for j, s in enumerate(jobs)]
-    # each job's engine from a short trial of both alone (untimed); the plain
-    # and the batched runs then use the same programs and engines.  (Tuning the
-    # whole launch shape of the heaviest jobs as well sped the plain run up more
-    # than the batch.)
-    for j, (se, s) in enumerate(zip(sessions, jobs)):
+    # each job's engine from a short trial of both alone (untimed); all legs
+    # then use the same programs and engines
+    for se, s in zip(sessions, jobs):
         autotune.pick_engine(se, min(4, s.spp))
     chosen = {}
-    for j, se in enumerate(sessions):
+    for se in sessions:
         name = autotune.ENGINE_NAMES[se.prog.engine_id]
         chosen[name] = chosen.get(name, 0) + 1
-        se.state.seed(500 + j)
     samples = sum(s.width * s.height * s.spp for s in jobs)
+
+    def seed_all(base):
+        for j, se in enumerate(sessions):
+            se.state.seed(base + j)
+
+    seed_all(500)
+    ctx.sync()
+    t = time.perf_counter()
+    for se, s in zip(sessions, jobs):
+        se.render(s.spp)
+        ctx.sync()
+    plain_own_s = time.perf_counter() - t
+    seed_all(600)
     ctx.sync()
     t = time.perf_counter()
     for se, s in zip(sessions, jobs):
         for _ in range(s.spp):
             se.render(1)
     ctx.sync()
-    plain_s = time.perf_counter() - t
+    plain_frame_s = time.perf_counter() - t
     batch = [(se.prog, se.state, s.spp) for se, s in zip(sessions, jobs)]
     for w in range(warmup):
         native.render_batch(batch)
     times, gpu_ms = [], []
     for k in range(steps):
-        for j, se in enumerate(sessions):
-            se.state.seed(900 + 64 * k + j)
+        seed_all(900 + 64 * k)
         ctx.sync()
         t = time.perf_counter()
         st = native.render_batch(batch)
@@ -123,162 +277,148 @@ def batched(ctx, steps, warmup):
     fams = {}
     for s in jobs:
         fams[s.name] = fams.get(s.name, 0) + 1
-    return {"workload": "batch64_mixed_sdf", "jobs": len(jobs), "path_samples_per_batch": samples,
+    return {"workload": "batch64_mixed_sdf (config 4)", "jobs": len(jobs), "path_samples_per_batch": samples,
             "distinct_programs": len({(s.glsl, s.name) for s in jobs}),
-            "plain_path_samples_per_s": round(samples / plain_s, 1), "plain_s": round(plain_s, 3),
+            "plain_path_samples_per_s": round(samples / plain_own_s, 1), "plain_s": round(plain_own_s, 3),
+            "plain_per_frame_path_samples_per_s": round(samples / plain_frame_s, 1),
+            "plain_per_frame_s": round(plain_frame_s, 3),
             "batched_path_samples_per_s": round(samples / batch_s, 1), "batched_s": round(batch_s, 3),
-            "batched_gpu_ms": round(sum(gpu_ms) / len(gpu_ms), 2), "speedup_vs_plain": round(plain_s / batch_s, 2),
+            "batched_gpu_ms": round(sum(gpu_ms) / len(gpu_ms), 2),
+            "speedup_vs_plain": round(plain_own_s / batch_s, 2),
+            "speedup_vs_plain_per_frame": round(plain_frame_s / batch_s, 2),
+            "plain": "each job's snl_render of all its frames, job after job (tile cost order)",
+            "plain_per_frame": "one snl_render(1) per job and frame, as the reference draws one frame per canvas",
+            "mandelbox_scales": sorted(round(s.user_uniforms["Scale"], 3) for s in jobs if "Scale" in s.user_uniforms),
             "build": "fast", "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")), "engines": chosen,
             "jobs_by_scene": fams}
 
 
-def served(ctx, n_jobs):
-    """The served workload (scenes.served_jobs): jobs arriving as a Poisson
-    process at 20/s, each rendered until its estimated relative RMS error
-    reaches its target (10-25%), by the continuous-batching server."""
+def served(ctx, n_jobs, seconds):
+    """config 5 (scenes.served_jobs): jobs arriving as a Poisson process at
+    20/s, sides log-uniform 64..2048, each rendered until its estimated
+    relative RMS error reaches its target (0.5..5%), by the continuous-batching
+    server, for `seconds` of arrivals plus a fifth of that to drain"""
     from snelly_amd import scenes, serve
-    arrivals = scenes.served_jobs(n_jobs, target=(0.10, 0.25))
+    arrivals = [a for a in scenes.served_jobs(n_jobs) if a[0] <= seconds]
+    cutoff = 1.2 * seconds
     srv = serve.Server(ctx, quantum=(2, 256), tick_samples=TICK_SAMPLES, chain_lanes=CHAIN_LANES,
                        max_chains=MAX_CHAINS)
     t = time.perf_counter()
-    done, left = srv.run(arrivals, time_limit=90.0)
+    done, left = srv.run(arrivals, time_limit=cutoff)
     wall = time.perf_counter() - t
-    rep = serve.Server.report(done, wall, srv.samples)
-    rep.update({"workload": "served_poisson20_mixed_sdf", "jobs": len(arrivals), "unfinished": len(left),
-                "unfinished_scenes": sorted({j.scene.name for j in left}),
-                "quantum_frames": [2, 256], "tick_samples": TICK_SAMPLES, "chain_lanes": CHAIN_LANES,
-                "max_chains": MAX_CHAINS,
-                "arrival_rate_per_s": 20.0, "sides": [64, 512], "target_rrmse": [0.10, 0.25], "wall_s": round(wall, 2),
-                "ticks": srv.ticks, "gpu_busy_frac": round(srv.gpu_ms * 1e-3 / wall, 3),
-                "error_estimate": "spread of 2-128 independently seeded chains per job, reduced on the GPU"})
+    rep = serve.Server.report(done, left, wall, srv.samples, cutoff_s=cutoff)
+    # offered load: the path samples each job needs for its target, from its
+    # error trend (samples_needed ~ samples_done * (error / target)^2), over
+    # the arrival span; jobs never estimated are left out
+    need = 0.0
+    for j in done + left:
+        if j.frames > 0 and math.isfinite(j.error):
+            samples = j.frames * j.pixels * j.chains
+            need += samples * max(1.0, (j.error / j.target) ** 2)
+    span = max(a[0] for a in arrivals) if arrivals else 1.0
+    fams = {}
+    for j in done:
+        fams[j.scene.name] = fams.get(j.scene.name, 0) + 1
+    rep.update({"workload": "served_poisson20 (config 5)", "jobs_arrived": len(arrivals),
+                "arrival_rate_per_s": 20.0, "sides": [64, 2048], "target_rrmse": [0.005, 0.05],
+                "cutoff_s": round(cutoff, 1), "wall_s": round(wall, 2), "ticks": srv.ticks,
+                "gpu_busy_frac": round(srv.gpu_ms * 1e-3 / wall, 3),
+                "offered_path_samples_per_s_est": round(need / span, 1),
+                "tick_samples": TICK_SAMPLES, "chain_lanes": CHAIN_LANES, "max_chains": MAX_CHAINS,
+                "quantum_frames": [2, 256], "done_by_scene": fams,
+                "error_estimate": "spread of 2-128 independently seeded chains per job, reduced on the GPU",
+                "latency": "arrival to target; unfinished jobs censored at the cutoff"})
     return rep
+
+
+def copy_yardstick(mib):
+    """A plain device copy of known size, the yardstick for HBM traffic
+    counters: profiled under the same counters as a render, its FETCH_SIZE /
+    WRITE_SIZE per dispatch against the bytes it moves give the counters'
+    scale factors."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"copied {mib} MiB x 5: each dispatch reads {mib * 1024} KiB and writes {mib * 1024} KiB", flush=True)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--width", type=int, default=1024)
-    ap.add_argument("--height", type=int, default=1024)
-    ap.add_argument("--spp", type=int, default=64)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--count-seconds", type=float, default=6.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--served-jobs", type=int, default=200)
-    ap.add_argument("--engine", choices=["auto", "persistent", "megakernel"], default="auto",
-                    help="pixel-queue engine of the timed renders (auto: the faster one, tuned)")
+    ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--served-jobs", type=int, default=2000)
+    ap.add_argument("--served-seconds", type=float, default=100.0)
+    ap.add_argument("--copy-yardstick", type=int, nargs="?", const=1024, default=None, metavar="MIB")
     args = ap.parse_args()
+    if args.copy_yardstick is not None:
+        copy_yardstick(args.copy_yardstick)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the path is per-image data parallel "
               f"and is run on a single MI355X here", file=sys.stderr)
         sys.exit(2)
 
-    from snelly_amd import autotune, host, native, scenes
+    from snelly_amd import autotune, native, scenes
 
-    W, H, spp = args.width, args.height, args.spp
-    scene = scenes.menger_metal(W, H, spp)
     ctx = native.Context(0)
-    # FAST: FMA contraction and 1-ulp divide/sqrt outside the RNG (parity of this
-    # build with the oracle is statistical: tests/test_gpu_parity.py)
-    sess = host.Session(scene, W, H, ctx, flags=native.BUILD_FAST)
-    # the plain wavefront engine (one launch sequence per bounce), for comparison
-    sess.prog.engine(native.ENGINE_WAVEFRONT)
-    sess.prog.profiling(True)
-    sess.state.seed(7)
-    sess.render(spp)
-    wf = sess.prog.stats()
-    wavefront = {"path_samples_per_s": round(W * H * spp / (wf.ms_total * 1e-3), 1), "ms": round(wf.ms_total, 2),
-                 "trace_ms": round(wf.ms_trace, 2),
-                 "trace_tflops": round(wf.trace_steps * scene.flops_per_step / (wf.ms_trace * 1e-3) / 1e12, 3)}
-    # the megakernel (cameraPath per lane, rays traced in place), for comparison
-    sess.prog.engine(native.ENGINE_MEGAKERNEL)
-    sess.state.seed(7)
-    sess.render(spp)
-    mk = sess.prog.stats()
-    megakernel = {"path_samples_per_s": round(W * H * spp / (mk.ms_total * 1e-3), 1), "ms": round(mk.ms_total, 2)}
-    # the pixel-queue engine (persistent state machine or megakernel) and its
-    # launch shape, autotuned untimed
-    engines = {"auto": (native.ENGINE_PERSISTENT, native.ENGINE_MEGAKERNEL),
-               "persistent": (native.ENGINE_PERSISTENT,), "megakernel": (native.ENGINE_MEGAKERNEL,)}[args.engine]
-    best, _, _ = autotune.tune(sess, frames=8, engines=engines)
-    for w in range(args.warmup):
-        sess.state.seed(1000 + w)
-        sess.render(spp)
-    ctx.sync()
-    images = []
-    trace_ms = 0.0
-    trace_steps = 0
-    normal_evals = 0
-    launches = 0
-    step_ms = []
-    elapsed = 0.0
-    for k in range(args.steps):
-        ctx.sync()
-        ts = time.perf_counter()
-        sess.state.seed(1234 + k)
-        sess.render(spp)
-        ctx.sync()
-        dt = time.perf_counter() - ts
-        elapsed += dt
-        step_ms.append(dt * 1e3)
-        st = sess.prog.stats()
-        trace_ms += st.ms_trace
-        trace_steps += st.trace_steps
-        normal_evals += st.normal_evals
-        launches += st.trace_launches
-        if k < 2:  # read back outside the timed region
-            images.append(unbiased(sess.radiance()))
+    # ---- headline: config 3 (megakernel: the medium's tracking runs inside a vertex)
+    scene = scenes.mandelbox_volume(1920, 1080, 256)
+    W, H, spp = scene.width, scene.height, scene.spp
+    sess, best, elapsed, gpu_ms, stats, rr = plain(ctx, scene, args.steps, args.warmup,
+                                                   (native.ENGINE_MEGAKERNEL,), 4)
+    sess.close()
     samples = W * H * spp * args.steps
-    value = samples / elapsed
-
-    # the queue kernel is the whole render; its launches on the tile streams overlap,
-    # so the rate is total algorithmic FLOPs over the GPU time of the renders.
-    # Counted: every SDF march step (SDF + 8 for the step itself) and every
-    # central-difference normal (6 SDF evaluations); shading is not counted.
-    flops = trace_steps * scene.flops_per_step + normal_evals * 6 * scene.sdf_flops
-    achieved = flops / (trace_ms * 1e-3) / 1e12
-    # per render (one step): a megakernel render is two launches (frame 1, then
-    # the other frames in cost order), each reading and writing every pixel once
-    renders = args.steps
-    per_render = launches / renders
+    counts = path_counts(scenes.mandelbox_volume(W, H, spp), W, H, spp, args.count_seconds)
     kname = autotune.KERNEL_NAMES[best["engine"]]
-    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                "traffic": (None if TRAFFIC_B_PER_PX[kname] is None else round(TRAFFIC_B_PER_PX[kname] * W * H)),
-                "traffic_algorithmic": 64 * W * H * per_render, "per": "render",
-                "kernel": kname, "launches_per_render": per_render, "gpu_ms": round(trace_ms, 2),
-                "flops_per_render": flops / renders, "flops_per_step": scene.flops_per_step,
-                "steps_per_render": trace_steps / renders, "normal_evals_per_render": normal_evals / renders,
-                "sdf_flops": scene.sdf_flops,
-                "flops_counted": "SDF march steps (SDF + 8) and normals (6 SDF); shading excluded"}
-    rr = None
-    if len(images) == 2:
-        a, b = images
-        rr = float(np.linalg.norm(a - b) / math.sqrt(2.0) / np.linalg.norm(0.5 * (a + b)))
     out = {
-        "metric": METRIC, "value": round(value, 1), "unit": "path samples/s", "n_gpus": 1,
+        "metric": METRIC, "value": round(samples / elapsed, 1), "unit": "path samples/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(elapsed * 1e3 / args.steps, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (analytic SDF scene, PCG-seeded per-pixel RNG)",
-        "config": {"workload": "menger4_metal_spectral", "width": W, "height": H, "spp": spp, "bounces": 8,
-                   "mode": "plain", "engine": autotune.ENGINE_NAMES[best["engine"]], "build": "fast",
-                   "launch_shape": {k: v for k, v in best.items() if k != "engine"},
-                   "tile_order": ("cost (frame 1, then the 8x8 tiles costliest first)"
-                                  if os.environ.get("SNL_TILE_ORDER", "1") != "0" and best["tiles"] == 1 else "queue"),
+        "data": "synthetic (analytic SDFs and value noise, PCG-seeded per-pixel RNG)",
+        "config": {"workload": "mandelbox_dielectric_sphere_cloud (config 3)", "width": W, "height": H, "spp": spp,
+                   "bounces": 16, "mode": "plain", "engine": autotune.ENGINE_NAMES[best["engine"]],
+                   "build": "fast", "launch_shape": {k: v for k, v in best.items() if k != "engine"},
+                   "tile_order": "cost (frame 1, then the 8x8 tiles costliest first)",
                    "path_samples_per_step": W * H * spp},
-        "engines": {"wavefront": wavefront, "megakernel": megakernel},
-        "roofline": roofline,
+        "roofline": roofline(counts, stats, args.steps, gpu_ms, kname, W, H, "mandelbox_volume"),
         "rrmse_per_step": None if rr is None else round(rr, 5),
         "spp_for_rrmse_1pct": None if rr is None else int(math.ceil(spp * (rr / 0.01) ** 2)),
     }
-    sess.close()
+    # ---- config 2 and its first-hit pass
+    if not args.no_secondary:
+        m = scenes.menger_metal(1024, 1024, 64)
+        msess, mbest, mel, mgpu, mstats, mrr = plain(ctx, m, 3, 1, (native.ENGINE_PERSISTENT,
+                                                                   native.ENGINE_MEGAKERNEL), 8)
+        msess.close()
+        mcounts = path_counts(scenes.menger_metal(1024, 1024, 64), 1024, 1024, 64, args.count_seconds / 2)
+        mk = autotune.KERNEL_NAMES[mbest["engine"]]
+        out["secondary"] = {
+            "workload": "menger4_metal_spectral (config 2)", "width": 1024, "height": 1024, "spp": 64, "bounces": 8,
+            "path_samples_per_s": round(1024 * 1024 * 64 * 3 / mel, 1), "ms_per_step": round(mel * 1e3 / 3, 3),
+            "engine": autotune.ENGINE_NAMES[mbest["engine"]],
+            "launch_shape": {k: v for k, v in mbest.items() if k != "engine"},
+            "roofline": roofline(mcounts, mstats, 3, mgpu, mk, 1024, 1024, "menger_metal"),
+            "rrmse_per_step": None if mrr is None else round(mrr, 5),
+            "first_hit_pass": first_hit_pass(ctx, scenes.menger_metal(1024, 1024, 1))}
     if not args.no_batch:
         out["batched"] = batched(ctx, max(1, min(args.steps, 3)), min(args.warmup, 1))
     if args.served_jobs > 0:
-        out["served"] = served(ctx, args.served_jobs)
+        out["served"] = served(ctx, args.served_jobs, args.served_seconds)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(scenes.menger_metal(W, H, spp), W, H, spp, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(scenes.mandelbox_volume(W, H, spp), W, H, spp, args.cpu_seconds)
+        if not args.no_secondary:
+            out["secondary"]["cpu_baseline"] = cpu_baseline(scenes.menger_metal(1024, 1024, 64), 1024, 1024, 64,
+                                                            args.cpu_seconds / 2)
     print(json.dumps(out), flush=True)
 
 

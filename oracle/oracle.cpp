@@ -31,9 +31,48 @@
 #include <omp.h>
 #endif
 
+#ifdef ORACLE_COUNT
+// the counting build: every float below is a counting value (counting.h)
+#include "counting.h"
+#define COUNT_SCOPE(c) orccount::Scope count_scope_(orccount::c)
+#define COUNT_ADD(field) (++orccount::tc.field)
+#else
+#define COUNT_SCOPE(c)
+#define COUNT_ADD(field)
+#endif
+
 #include "vec.h"
 
 using namespace orc;
+
+#ifdef ORACLE_COUNT
+namespace orccount {
+thread_local Counters tc{};
+thread_local int cat = CAT_SHADE;
+Counters total{};
+// adds the calling thread's counts to the totals (inside each thread of a
+// parallel render, after its share of the pixels)
+void flush() {
+#ifdef _OPENMP
+#pragma omp critical(orccount_flush)
+#endif
+  {
+    for (int c = 0; c < CAT_COUNT; ++c) {
+      total.flops[c] += tc.flops[c];
+      total.trans[c] += tc.trans[c];
+      total.ops[c] += tc.ops[c];
+    }
+    total.steps += tc.steps;
+    total.normals += tc.normals;
+    total.paths += tc.paths;
+    tc = Counters{};
+  }
+}
+}  // namespace orccount
+#define COUNT_FLUSH() orccount::flush()
+#else
+#define COUNT_FLUSH()
+#endif
 
 namespace {
 
@@ -184,9 +223,14 @@ T mengerT(const T X[3], float size, float scale, int iter) {
 // per-scene constants are INIT() work, outside the per-evaluation count
 template <class T>
 T mandelboxT(const T pos[3], float Scale, int iters, float bailout, float MinRad2) {
-  float sc[4] = {Scale / MinRad2, Scale / MinRad2, Scale / MinRad2, std::fabs(Scale) / MinRad2};
-  float absScalem1 = std::fabs(Scale - 1.0f);
-  float AbsScaleRaisedTo1mIters = std::pow(std::fabs(Scale), float(1 - iters));
+  float sc[4], absScalem1, AbsScaleRaisedTo1mIters;
+  {
+    COUNT_SCOPE(CAT_INIT);
+    for (int k = 0; k < 3; ++k) sc[k] = Scale / MinRad2;
+    sc[3] = std::fabs(Scale) / MinRad2;
+    absScalem1 = std::fabs(Scale - 1.0f);
+    AbsScaleRaisedTo1mIters = std::pow(std::fabs(Scale), float(1 - iters));
+  }
   T p[4] = {pos[0], pos[1], pos[2], lit<T>(1.0f)}, p0[4] = {pos[0], pos[1], pos[2], lit<T>(1.0f)};
   for (int i = 0; i < iters; i++) {
     for (int k = 0; k < 3; ++k) p[k] = clamp_(p[k], -1.0f, 1.0f) * lit<T>(2.0f) - p[k];
@@ -601,14 +645,17 @@ struct Tracer {
   R zero() const { return rt(0.0f, R()); }
 
   bool traceRay(V3 start, V3 dir, V3& hit, int& material, float maxMarchDist) const {
+    COUNT_SCOPE(CAT_MARCH);
     material = MAT_INVAL;
     float minMarch = P.minLengthScale;
     float sdf = 1.0e20f;
+    COUNT_ADD(steps);
     for (int m : {MAT_SURFA, MAT_METAL, MAT_DIELE}) if (hasMat(m)) sdf = fmin_(sdf, std::fabs(evalSdf(sdfs[m], start)));
     float InitialSign = fsign(sdf);
     float t = InitialSign * sdf;
     if (std::fabs(t) >= maxMarchDist) return false;
     for (int n = 0; n < C.maxMarch; n++) {
+      COUNT_ADD(steps);
       V3 pW = start + t * dir;
       sdf = 1.0e20f;
       for (int m : {MAT_SURFA, MAT_METAL, MAT_DIELE}) {
@@ -624,6 +671,8 @@ struct Tracer {
     return P.maxStepsIsMiss == 0.0f;
   }
   V3 normal(V3 pW, int m, bool safe = true) const {
+    COUNT_SCOPE(CAT_NORMAL);
+    COUNT_ADD(normals);
     float e = 2.0f * P.minLengthScale;
     const SdfSpec& s = sdfs[m];
     V3 N = v3(evalSdf(s, pW + v3(e, 0, 0)) - evalSdf(s, pW - v3(e, 0, 0)),
@@ -955,6 +1004,7 @@ struct Tracer {
     return A(sc / vmax(sc + ab, DENOM_TOLERANCE));
   }
   float mediumFreeFlight(V3 o, V3 d, float tmax, float* rnd) const {
+    COUNT_SCOPE(CAT_MEDIUM);
     float sigmaMax = mediumSigmaMax();
     if (!(sigmaMax > 0.0f)) return tmax;
     float skip = kVolumeSkip / sigmaMax;
@@ -973,6 +1023,7 @@ struct Tracer {
     return tmax;
   }
   float mediumTransmittance(V3 o, V3 d, float tmax, float* rnd) const {
+    COUNT_SCOPE(CAT_MEDIUM);
     float sigmaMax = mediumSigmaMax();
     if (!(sigmaMax > 0.0f)) return 1.0f;
     float skip = kVolumeSkip / sigmaMax;
@@ -1507,6 +1558,7 @@ void pathtracePixel(int ix, int iy, float* rad, float* rnd) {
     float jy = 0.5f * P.filterRadius * sample_jitter(randf(rnd));
     V3 s, d;
     constructPrimaryRay(px + jx, py + jy, rnd, s, d, true);
+    COUNT_ADD(paths);
     L = L + T.cameraPath(s, d, rnd);
   }
   L = L / float(C.spf);
@@ -1946,6 +1998,24 @@ void oracle_sample_hg(const float* dir, float g, float* rnd4, float* out3, float
   out3[0] = w.x; out3[1] = w.y; out3[2] = w.z;
 }
 float oracle_phase_function(float mu, float g) { return phaseFunction(mu, g); }
+#ifdef ORACLE_COUNT
+// the counting build's totals since the last reset: FLOPs, transcendentals and
+// other operations per category (shade, march, normal, medium, init), then SDF
+// evaluations of the sphere tracer, normals and camera paths: 18 values
+void oracle_count_reset() { orccount::total = orccount::Counters{}; orccount::tc = orccount::Counters{}; }
+void oracle_count_get(long long* out) {
+  const orccount::Counters& t = orccount::total;
+  const int n = orccount::CAT_COUNT;
+  for (int c = 0; c < n; ++c) {
+    out[c] = t.flops[c];
+    out[n + c] = t.trans[c];
+    out[2 * n + c] = t.ops[c];
+  }
+  out[3 * n] = t.steps;
+  out[3 * n + 1] = t.normals;
+  out[3 * n + 2] = t.paths;
+}
+#endif
 void oracle_dielectric(int model, const float* coeffs, int n) { dieleModel = model; dieleC.assign(coeffs, coeffs + n); }
 void oracle_metal(int model, const float* n64, const float* k64) {
   metalModel = model;
@@ -2023,20 +2093,32 @@ int oracle_render(float* radiance, float* rng, int W, int H, int frames, int str
     for (int f = 0; f < frames; ++f) {
       unit0Radiance.assign(radiance, radiance + 4 * npix);
 #ifdef _OPENMP
-#pragma omp parallel for schedule(dynamic, 64)
+#pragma omp parallel
 #endif
-      for (long long i = offset; i < npix; i += stride) {
-        renderPixel((int)(i % W), (int)(i / W), radiance + 4 * i, rng + 4 * i);
+      {
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 64)
+#endif
+        for (long long i = offset; i < npix; i += stride) {
+          renderPixel((int)(i % W), (int)(i / W), radiance + 4 * i, rng + 4 * i);
+        }
+        COUNT_FLUSH();
       }
     }
     return 0;
   }
 #ifdef _OPENMP
-#pragma omp parallel for schedule(dynamic, 64)
+#pragma omp parallel
 #endif
-  for (long long i = offset; i < npix; i += stride) {
-    int ix = (int)(i % W), iy = (int)(i / W);
-    for (int f = 0; f < frames; ++f) renderPixel(ix, iy, radiance + 4 * i, rng + 4 * i);
+  {
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 64)
+#endif
+    for (long long i = offset; i < npix; i += stride) {
+      int ix = (int)(i % W), iy = (int)(i / W);
+      for (int f = 0; f < frames; ++f) renderPixel(ix, iy, radiance + 4 * i, rng + 4 * i);
+    }
+    COUNT_FLUSH();
   }
   return 0;
 }

@@ -11,7 +11,10 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "liboracle.so")
-_lib = None
+# the counting build of the same source (counting.h): identical results, plus
+# operation counts per category
+LIB_COUNT = os.path.join(HERE, "_build", "liboracle_count.so")
+_libs = {}
 
 DIELECTRIC_MODELS = {"AbbeDielectric": 0, "SellmeierDielectric": 1, "Sellmeier2Dielectric": 2,
                      "PolyanskiyDielectric": 3, "CauchyDielectric": 4}
@@ -21,12 +24,12 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
+def lib(counting=False):
+    if counting not in _libs:
+        path = LIB_COUNT if counting else LIB
+        if not os.path.exists(path):
             build()
-        L = ctypes.CDLL(LIB)
+        L = ctypes.CDLL(path)
         fp = ctypes.POINTER(ctypes.c_float)
         L.oracle_set.argtypes = [ctypes.c_char_p, fp, ctypes.c_int]
         L.oracle_config.argtypes = [ctypes.c_int] * 6
@@ -73,8 +76,10 @@ def lib():
         L.oracle_sample_hg.argtypes = [fp, ctypes.c_float, fp, fp, fp]
         L.oracle_phase_function.argtypes = [ctypes.c_float, ctypes.c_float]
         L.oracle_phase_function.restype = ctypes.c_float
-        _lib = L
-    return _lib
+        if counting:
+            L.oracle_count_get.argtypes = [ctypes.POINTER(ctypes.c_longlong)]
+        _libs[counting] = L
+    return _libs[counting]
 
 
 def fp(a):
@@ -88,10 +93,10 @@ def f3(v):
 class Oracle:
     """The oracle configured for one scene (a snelly_amd.scenes.Scene)."""
 
-    def __init__(self, scene, width, height):
+    def __init__(self, scene, width, height, counting=False):
         from snelly_amd import host  # plumbing: uniform values, LUTs
         self.scene, self.width, self.height = scene, width, height
-        L = lib()
+        self.L = L = lib(counting)
         L.oracle_reset()
         r, mats = scene.renderer, scene.materials
         scene.camera.aspect = width / height
@@ -139,6 +144,22 @@ class Oracle:
         lib().oracle_seed(fp(rng), npix, seed)
         return rng
 
+    # the counting build's totals (Oracle(..., counting=True)): FLOPs,
+    # transcendentals and other operations of each category, SDF evaluations of
+    # the sphere tracer, normals and camera paths since the last reset
+    CATEGORIES = ("shade", "march", "normal", "medium", "init")
+
+    def count_reset(self):
+        self.L.oracle_count_reset()
+
+    def counts(self):
+        out = (ctypes.c_longlong * 18)()
+        self.L.oracle_count_get(out)
+        v = list(out)
+        cats = self.CATEGORIES
+        return {"flops": dict(zip(cats, v[0:5])), "transcendentals": dict(zip(cats, v[5:10])),
+                "other_ops": dict(zip(cats, v[10:15])), "steps": v[15], "normals": v[16], "paths": v[17]}
+
     def new_state(self, seed=1234):
         npix = self.width * self.height
         rad = np.zeros((npix, 4), np.float32)
@@ -146,19 +167,19 @@ class Oracle:
         return rad, self.seed(npix, seed)
 
     def render(self, rad, rng, frames, stride=1, offset=0, threads=0):
-        lib().oracle_render(fp(rad), fp(rng), self.width, self.height, frames, stride, offset, threads)
+        self.L.oracle_render(fp(rad), fp(rng), self.width, self.height, frames, stride, offset, threads)
 
     def normals(self, rad, rng, frames, threads=0):
-        lib().oracle_normals(fp(rad), fp(rng), self.width, self.height, frames, threads)
+        self.L.oracle_normals(fp(rad), fp(rng), self.width, self.height, frames, threads)
 
     def pick(self, xy):
         xy = f3(xy).reshape(-1, 2)
         out = np.empty_like(xy)
-        lib().oracle_pick(fp(xy), xy.shape[0], fp(out))
+        self.L.oracle_pick(fp(xy), xy.shape[0], fp(out))
         return out
 
     def tonemap(self, rad):
         rad = f3(rad).reshape(-1, 4)
         out = np.empty((rad.shape[0], 3), np.float32)
-        lib().oracle_tonemap(fp(rad), rad.shape[0], fp(out))
+        self.L.oracle_tonemap(fp(rad), rad.shape[0], fp(out))
         return out

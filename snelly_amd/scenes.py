@@ -469,13 +469,12 @@ def batch_job(family, material, side, spp, bounces, rs):
         s.user_uniforms = {"size": 1.0, "scale": 1.0}
         s.oracle_sdfs = [(mid, 3, 1.0, 1.0, float(family[-1]), 0)]
     if family == "mandelbox":
-        # |scale| < 1.5 gives degenerate boxes; draw again
-        sc = 0.0
-        while abs(sc) < 1.5:
-            sc = float(rs.uniform(-2.0, 2.5))
+        # scale uniform in [-2, 2.5] as drawn; the camera distance follows the
+        # box's extent 2 (s + 1) / (s - 1) for s > 1, at most 6
+        sc = float(rs.uniform(-2.0, 2.5))
         s.user_uniforms = {"Scale": sc, "MinRad2": 0.25}
         s.oracle_sdfs = [(mid, 4, sc, 12.0, 10.0, 0.25)]
-        extent = 2.0 * (abs(sc) + 1.0) / (abs(sc) - 1.0) if sc > 0 else 2.0
+        extent = min(6.0, 2.0 * (sc + 1.0) / (sc - 1.0)) if sc > 1.0 else (2.0 if sc < 0.0 else 6.0)
         r.maxMarchSteps = 512
     d = 2.6 * extent
     s.camera.position = [0.55 * d, 0.45 * d, 0.75 * d]
@@ -516,11 +515,12 @@ def batch_jobs(n=64, seed=2024, min_side=128, max_side=1024):
     return jobs
 
 
-def served_jobs(n=200, seed=7, rate=20.0, min_side=64, max_side=512, target=(0.05, 0.20)):
-    """The served workload: n jobs arriving as a Poisson process of `rate` per
-    second, scenes drawn like batch_jobs (family, material, bounces), side
-    log-uniform in [min_side, max_side], target relative RMS error uniform in
-    `target`.  Returns [(arrival_s, scene, target)]."""
+def served_jobs(n=2000, seed=7, rate=20.0, min_side=64, max_side=2048, target=(0.005, 0.05)):
+    """The served workload (config 5 by default): n jobs arriving as a
+    Poisson process of `rate` per second, scenes drawn like batch_jobs
+    (family, material, bounces), side log-uniform in [min_side, max_side],
+    target relative RMS error uniform in `target`.
+    Returns [(arrival_s, scene, target)]."""
     rs = np.random.RandomState(seed)
     t, out = 0.0, []
     for _ in range(n):

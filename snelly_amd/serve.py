@@ -9,13 +9,16 @@ one launch tens of thousands of lanes; the chains' spread, reduced on the GPU
 Each tick the server
   1. admits waiting jobs in arrival order while the device has room for their
      states (snl_ctx_mem_info, snl_state_bytes) and the active set is not full;
-  2. renders one quantum of frames for every active job in one
+  2. renders one quantum of frames for the active jobs in one
      snl_render_batch call.  A job wants the frames its error trend asks for,
      frames_needed ~ frames_done * (error / target)^2, within (smallest,
-     largest); every job gets the smallest quantum, and the rest of the tick's
-     path-sample budget goes to the jobs with the least remaining work first
-     (so a cheap job that needs many frames takes them in a few ticks, and a
-     job whose error hardly falls does not hold back the others);
+     largest).  The tick's path-sample budget bounds the whole tick, so its
+     wall time and the wait of a job arriving meanwhile stay bounded: jobs
+     take their smallest quantum in round-robin order (the longest unserved
+     first) while the budget lasts -- at least one job per tick -- and what is
+     left goes to the jobs with the least remaining work first (so a cheap job
+     that needs many frames takes them in a few ticks, and a job whose error
+     hardly falls does not hold back the others);
   3. retires the jobs whose estimate is at or below their target, recording
      the time from arrival to completion.
 Pixels never read each other's state, so a job's images do not depend on
@@ -35,6 +38,7 @@ class Job:
         self.frames = 0
         self.error = math.inf
         self.admitted = self.finished = None
+        self.last_tick = -1  # the last tick that rendered the job
         self.chains = 2
         self.a = None
 
@@ -64,8 +68,8 @@ class Job:
 class Server:
     """min_frames: frames per state before an error estimate counts;
     quantum: (smallest, largest) frames per state per tick;
-    tick_samples: path samples per tick beyond every job's smallest quantum
-    (None: no budget, each job takes what it wants);
+    tick_samples: path samples per tick, smallest quanta included (None: no
+    budget, every job takes what it wants each tick);
     chain_lanes, max_chains: a job gets the fewest chains (up to max_chains)
     that give it chain_lanes lanes -- a chain's frames run one after another
     in a lane, so more chains cut the time a job needs for many frames;
@@ -98,16 +102,27 @@ class Server:
         return int(max(lo, min(hi, self._need(job))))
 
     def _plan(self, jobs, samples_per_frame):
-        """frames per job for one tick: each job's wish (_quantum) when there is
-        no budget; otherwise the smallest quantum for every job (or its first
-        min_frames), then the budget to the least remaining work first"""
+        """frames per job for one tick (0: the job waits for a later tick):
+        each job's wish (_quantum) when there is no budget; otherwise the
+        smallest quantum (or the first min_frames) in round-robin order while
+        the budget lasts, at least one job, then the rest of the budget to the
+        least remaining work first"""
         want = [self._quantum(j) for j in jobs]
         if self.tick_samples is None:
             return want
         lo = self.quantum[0]
-        q = [w if j.frames < self.min_frames else min(lo, w) for j, w in zip(jobs, want)]
+        first = [w if j.frames < self.min_frames else min(lo, w) for j, w in zip(jobs, want)]
+        q = [0] * len(jobs)
         budget = self.tick_samples
+        for i in sorted(range(len(jobs)), key=lambda i: (jobs[i].last_tick, jobs[i].arrival)):
+            cost = first[i] * samples_per_frame[i]
+            if cost > budget and any(q):
+                continue
+            q[i] = first[i]
+            budget -= cost
         for i in sorted(range(len(jobs)), key=lambda i: max(self._need(jobs[i]), 1) * samples_per_frame[i]):
+            if q[i] == 0 or budget <= 0:
+                continue
             extra = min(want[i] - q[i], budget // samples_per_frame[i])
             if extra > 0:
                 q[i] += extra
@@ -146,12 +161,15 @@ class Server:
                     time.sleep(max(0.0, min(waiting[-1].arrival - (self.clock() - t0), 0.05)))
                 continue
             counts = self._plan(active, [job.pixels * job.chains for job in active])
-            batch = [(job.a.prog, job.a.state, q) for job, q in zip(active, counts)]
+            batch = [(job.a.prog, job.a.state, q) for job, q in zip(active, counts) if q > 0]
             st = native.render_batch(batch)
-            self.ticks += 1
             self.gpu_ms += st.ms_total
             still = []
             for job, q in zip(active, counts):
+                if q == 0:
+                    still.append(job)
+                    continue
+                job.last_tick = self.ticks
                 job.frames += q
                 self.samples += q * job.pixels * job.chains
                 if job.frames >= self.min_frames:
@@ -166,13 +184,34 @@ class Server:
                 else:
                     still.append(job)
             active = still
+            self.ticks += 1
         for job in active:
             job.close()
         return done, active + queue + waiting
 
     @staticmethod
-    def report(done, wall_s, samples):
-        lat = np.array([j.finished - j.arrival for j in done]) if done else np.array([math.nan])
-        return {"jobs_done": len(done), "p50_s": round(float(np.percentile(lat, 50)), 4),
-                "p99_s": round(float(np.percentile(lat, 99)), 4), "mean_s": round(float(lat.mean()), 4),
-                "path_samples_per_s": round(samples / wall_s, 1) if wall_s > 0 else None}
+    def report(done, left, wall_s, samples, cutoff_s=None):
+        """latency percentiles over all jobs that arrived by the cutoff: a job
+        left unfinished counts as right-censored at cutoff - arrival (its true
+        latency is at least that).  A percentile that falls on a censored job
+        is only a lower bound: `<name>_censored` says so."""
+        cutoff = wall_s if cutoff_s is None else cutoff_s
+        lat = [(j.finished - j.arrival, False) for j in done]
+        lat += [(max(0.0, cutoff - j.arrival), True) for j in left if j.arrival <= cutoff]
+        lat.sort(key=lambda x: x[0])
+        out = {"jobs_done": len(done), "jobs_unfinished": sum(1 for _, c in lat if c),
+               "path_samples_per_s": round(samples / wall_s, 1) if wall_s > 0 else None}
+        if not lat:
+            out.update({"p50_s": None, "p99_s": None, "p50_censored": False, "p99_censored": False})
+            return out
+        vals = np.array([v for v, _ in lat])
+        for name, q in (("p50", 50), ("p99", 99)):
+            k = min(len(lat) - 1, int(math.ceil(q / 100.0 * len(lat))) - 1)
+            k = max(k, 0)
+            out[f"{name}_s"] = round(float(vals[k]), 4)
+            # exact unless a censored job (true latency >= its recorded value)
+            # could rank at or below it
+            out[f"{name}_censored"] = bool(any(c and v <= vals[k] for v, c in lat))
+        done_lat = np.array([j.finished - j.arrival for j in done]) if done else np.array([math.nan])
+        out["mean_done_s"] = round(float(done_lat.mean()), 4)
+        return out

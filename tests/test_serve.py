@@ -8,8 +8,9 @@ from snelly_amd import host, native, scenes, serve
 
 
 class FakeJob:
-    def __init__(self, frames, error, target):
+    def __init__(self, frames, error, target, last_tick=-1, arrival=0.0):
         self.frames, self.error, self.target = frames, error, target
+        self.last_tick, self.arrival = last_tick, arrival
 
 
 def test_quantum_follows_error_trend():
@@ -30,28 +31,68 @@ def test_plan_budget_goes_to_least_remaining_work():
     costly = FakeJob(100, 0.2, 0.1)          # same frames, 10 samples per frame: remaining 3000
     jobs, spf = [new, near, cheap, costly], [10, 10, 1, 10]
     q = s._plan(jobs, spf)
-    # base: 4 + 2 + 2 + 2 frames; cheap takes 254 more (254 samples), costly the rest (746 // 10 = 74)
-    assert q == [4, 2, 256, 76]
+    # first quanta 4 + 2 + 2 + 2 frames (82 samples of the 1000); cheap takes 254
+    # more (254 samples), costly the rest (664 // 10 = 66)
+    assert q == [4, 2, 256, 68]
     s.tick_samples = None
     assert s._plan(jobs, spf) == [4, 2, 256, 256]
 
 
+def test_plan_budget_bounds_the_tick_round_robin():
+    """a budget below the jobs' smallest quanta: the longest-unserved jobs go
+    first, at least one per tick, and the others wait for a later tick"""
+    s = serve.Server.__new__(serve.Server)
+    s.min_frames, s.quantum, s.tick_samples = 4, (2, 64), 45
+    a = FakeJob(8, 0.5, 0.1, last_tick=3, arrival=0.0)
+    b = FakeJob(8, 0.5, 0.1, last_tick=1, arrival=1.0)
+    c = FakeJob(8, 0.5, 0.1, last_tick=1, arrival=0.5)
+    assert s._plan([a, b, c], [10, 10, 10]) == [0, 2, 2]
+    s.tick_samples = 5   # smaller than any quantum: still one job
+    assert s._plan([a, b, c], [10, 10, 10]) == [0, 0, 2]
+
+
 def test_served_workload():
-    a = scenes.served_jobs(50, seed=3)
-    b = scenes.served_jobs(50, seed=3)
+    """config 5: Poisson arrivals at 20/s, sides log-uniform 64..2048, target
+    relative RMSE uniform in [0.5%, 5%], scenes of the batched families"""
+    a = scenes.served_jobs(400, seed=3)
+    b = scenes.served_jobs(400, seed=3)
     assert [x[0] for x in a] == [x[0] for x in b]
     times = np.array([x[0] for x in a])
     assert np.all(np.diff(times) > 0)
-    assert all(64 <= s.width <= 512 and 0.05 <= t <= 0.2 for _, s, t in a)
+    assert abs(len(a) / times[-1] - 20.0) < 3.0
+    sides = np.array([s.width for _, s, _ in a])
+    assert sides.min() >= 64 and sides.max() <= 2048 and sides.max() > 1500 and sides.min() < 90
+    assert all(s.width == s.height for _, s, _ in a)
+    assert all(0.005 <= t <= 0.05 for _, _, t in a)
+
+
+class J:
+    def __init__(self, a, f=None):
+        self.arrival, self.finished = a, f
 
 
 def test_report_percentiles():
-    class J:
-        def __init__(self, a, f):
-            self.arrival, self.finished = a, f
-    r = serve.Server.report([J(0.0, float(k)) for k in range(1, 101)], 10.0, 1e9)
-    assert r["jobs_done"] == 100 and r["p50_s"] == 50.5 and abs(r["p99_s"] - 99.01) < 1e-6
+    r = serve.Server.report([J(0.0, float(k)) for k in range(1, 101)], [], 10.0, 1e9)
+    assert r["jobs_done"] == 100 and r["jobs_unfinished"] == 0
+    assert r["p50_s"] == 50.0 and r["p99_s"] == 99.0
+    assert not r["p50_censored"] and not r["p99_censored"]
     assert r["path_samples_per_s"] == 1e8
+
+
+def test_report_counts_unfinished_jobs_as_censored():
+    """jobs still running at the cutoff enter the percentiles at cutoff -
+    arrival, and a percentile they can reach is flagged as a lower bound"""
+    done = [J(0.0, float(k)) for k in range(1, 91)]       # latencies 1..90 s
+    left = [J(5.0 * k) for k in range(10)]                # arrived at 0..45 s, unfinished
+    r = serve.Server.report(done, left, 100.0, 1e9, cutoff_s=100.0)
+    assert r["jobs_done"] == 90 and r["jobs_unfinished"] == 10
+    # censored latencies 100, 95, ..., 55: p99 is the 99th of 100 -> 95 s, a lower bound
+    assert r["p99_s"] == 95.0 and r["p99_censored"]
+    # p50: the 50th value (50 s) -- censored jobs start at 55 s, so it is exact
+    assert r["p50_s"] == 50.0 and not r["p50_censored"]
+    # jobs arriving after the cutoff are not counted
+    r2 = serve.Server.report(done, left + [J(150.0)], 100.0, 1e9, cutoff_s=100.0)
+    assert r2["jobs_unfinished"] == 10
 
 
 def test_chains_for_small_images():

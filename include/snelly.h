@@ -89,9 +89,14 @@ int snl_uniform_texture(snl_prog prog, int idx, snl_tex tex);
  * (js/gl.js:485-523); depth > 1 makes a 3-D texture.  texels may be NULL. */
 int snl_texture_create(snl_ctx ctx, int width, int height, int depth, int channels, int linear, int clamp,
                        const float* texels, snl_tex* out);
-/* GLU.Texture.copy (js/gl.js:503-507) */
+/* GLU.Texture.copy (js/gl.js:503-507): width*height*depth*channels floats */
 int snl_texture_upload(snl_tex tex, const float* texels);
+/* Deletes the texture (gl.deleteTexture); sampler uniforms still bound to it
+ * then read (0, 0, 0, 1), as a texture unit without a texture does in WebGL. */
 int snl_texture_destroy(snl_tex tex);
+/* The texture's shape, for bindings that check buffer sizes (a WebGL texture
+ * object carries its own; no C counterpart in the reference). */
+int snl_texture_info(snl_tex tex, int* width, int* height, int* depth, int* channels);
 
 /* PathtracerState pair (js/renderer.js:692-745): per-pixel radiance
  * (X, Y, Z, N) and RNG state, resident in HBM. */
@@ -104,6 +109,10 @@ int snl_state_clear(snl_state st);
 /* which: 0 = radiance RGBA32F, 1 = RNG RGBA32F; 4*width*height floats, row 0 = bottom */
 int snl_state_read(snl_state st, int which, float* out);
 int snl_state_write(snl_state st, int which, const float* in);
+/* The state's shape (out buffers above hold 4 * width * height * chains
+ * floats), for bindings that check buffer sizes; no counterpart in the
+ * reference, whose render targets carry their own size. */
+int snl_state_info(snl_state st, int* width, int* height, int* chains);
 
 /* Renderer.render's pathtracer draw (js/renderer.js:462-644) repeated for
  * `frames` frames: one path sample per pixel per frame per sample-per-frame. */
@@ -175,7 +184,8 @@ int snl_set_march_burst(snl_prog prog, int steps);
  * queue count (GPU_MAX_HW_QUEUES).  The reference renders
  * one scene per canvas per frame (js/renderer.js:462-644); per job the result
  * is identical to snl_render(progs[j], states[j], frames[j]).  All programs
- * belong to one context; every job has its own state.  Blocks until the batch
+ * belong to one context; every job has its own state and its own program (a
+ * program's uniforms are the job's; a repeated one is SNL_E_ARG).  Blocks until the batch
  * is done; `stats` (may be NULL) receives the batch's wall time and summed
  * counters. */
 int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, const int* frames,
@@ -199,9 +209,14 @@ int snl_state_error(snl_state st, double* out);
  * admission control */
 int snl_ctx_mem_info(snl_ctx ctx, unsigned long long* free_bytes, unsigned long long* total_bytes);
 /* device bytes of a state of `chains` width x height images rendered by the
- * persistent engine or snl_render_batch (the wavefront engine adds its path
- * buffers on first use) */
+ * megakernel, the persistent engine or snl_render_batch: radiance and RNG,
+ * counters, the tile-order buffers and the error-estimate partials (the
+ * wavefront engine adds its path buffers on first use) */
 unsigned long long snl_state_bytes(int width, int height, int chains);
+/* the same for a state rendered by `prog`: adds the 16 B/px radiance snapshot
+ * of programs that sample texture unit 0 (an unbound sampler2D, such as a
+ * tabulated metal's kTex, js/materials.js:664-665) */
+unsigned long long snl_program_state_bytes(snl_prog prog, int width, int height, int chains);
 
 #ifdef __cplusplus
 }

@@ -58,6 +58,7 @@ struct snl_tex_s {
   snl_ctx ctx;
   float* d = nullptr;
   int w = 0, h = 0, depth = 1, ch = 4, flags = 0;
+  std::set<snl_prog_s*> users;  // programs with a sampler uniform bound to this texture
 };
 
 struct snl_state_s {
@@ -72,6 +73,7 @@ struct snl_state_s {
   int* cnt = nullptr;  // C_COUNT ints
   int* tileCnt = nullptr;  // SNL_MAX_TILES * C_COUNT ints (persistent engine)
   float* radPrev = nullptr;  // radiance as of the frame start, for samplers left on texture unit 0
+  double* errPartial = nullptr;  // snl_state_error's block partials and result (allocated on first use)
   int* tileCost = nullptr;   // per 8x8 tile: march steps of the render's first frame, then the
                              // cost-bucket histogram and the tile order (ordered megakernel renders)
 };
@@ -107,6 +109,7 @@ struct snl_prog_s {
   // bound; an unbound one samples texture unit 0, the radiance being read
   std::vector<int> sampled2D;
   std::set<int> bound;
+  std::map<int, snl_tex_s*> texOf;  // sampler uniform -> the texture bound to it (null once destroyed)
 };
 
 namespace {
@@ -586,6 +589,8 @@ int snl_program_destroy(snl_prog p) {
   if (p->dP) hipFree(p->dP);
   if (p->dU) hipFree(p->dU);
   if (p->mod) hipModuleUnload(p->mod);
+  for (auto& kv : p->texOf)
+    if (kv.second) kv.second->users.erase(p);
   for (hipEvent_t e : {p->ev0, p->ev1, p->et0, p->et1}) if (e) hipEventDestroy(e);
   for (hipEvent_t e : p->traceEvents) hipEventDestroy(e);
   delete p;
@@ -661,16 +666,25 @@ int snl_uniform_texture(snl_prog p, int idx, snl_tex t) {
     off = p->user[j].offset;
     utype = p->user[j].type;
   }
+  if (utype != SNL_T_SAMPLER2D && utype != SNL_T_SAMPLER3D) return SNL_OK;  // not a sampler: GL error, no effect
   if (t) p->bound.insert(idx);
   else p->bound.erase(idx);
+  // keep the texture -> program links that snl_texture_destroy follows
+  snl_tex_s* old = p->texOf.count(idx) ? p->texOf[idx] : nullptr;
+  if (t) p->texOf[idx] = t;
+  else p->texOf.erase(idx);
+  if (old && old != t) {
+    bool still = false;
+    for (auto& kv : p->texOf) still |= kv.second == old;
+    if (!still) old->users.erase(p);
+  }
+  if (t) t->users.insert(p);
   if (utype == SNL_T_SAMPLER2D) {
     sampler2D s{t ? t->d : nullptr, t ? t->w : 0, t ? t->h : 0, t ? t->ch : 4, t ? t->flags : 0};
     memcpy(base + off, &s, sizeof s);
-  } else if (utype == SNL_T_SAMPLER3D) {
+  } else {
     sampler3D s{t ? t->d : nullptr, t ? t->w : 0, t ? t->h : 0, t ? t->depth : 0, t ? t->ch : 4, t ? t->flags : 0};
     memcpy(base + off, &s, sizeof s);
-  } else {
-    return SNL_OK;  // not a sampler: GL error, no effect
   }
   p->dirty = true;
   return SNL_OK;
@@ -693,6 +707,14 @@ int snl_texture_create(snl_ctx ctx, int w, int h, int depth, int ch, int linear,
   return SNL_OK;
 }
 
+int snl_texture_info(snl_tex t, int* width, int* height, int* depth, int* channels) {
+  if (!t) return fail(nullptr, SNL_E_ARG, "texture is NULL");
+  if (width) *width = t->w;
+  if (height) *height = t->h;
+  if (depth) *depth = t->depth;
+  if (channels) *channels = t->ch;
+  return SNL_OK;
+}
 int snl_texture_upload(snl_tex t, const float* texels) {
   if (!t || !texels) return fail(t ? t->ctx : nullptr, SNL_E_ARG, "NULL argument");
   size_t bytes = (size_t)t->w * t->h * t->depth * t->ch * sizeof(float);
@@ -704,6 +726,26 @@ int snl_texture_upload(snl_tex t, const float* texels) {
 int snl_texture_destroy(snl_tex t) {
   if (!t) return SNL_OK;
   hipStreamSynchronize(t->ctx->stream);
+  // a sampler still bound to the texture now reads its unit's missing texture,
+  // (0, 0, 0, 1) as in WebGL, instead of freed memory; it stays assigned a unit
+  for (snl_prog_s* p : t->users) {
+    for (auto& kv : p->texOf) {
+      if (kv.second != t) continue;
+      int idx = kv.first;
+      char* base = idx < 1000 ? reinterpret_cast<char*>(&p->hostP) : p->hostU.data();
+      int off = idx < 1000 ? builtins()[idx].offset : p->user[idx - 1000].offset;
+      int utype = idx < 1000 ? builtins()[idx].type : p->user[idx - 1000].type;
+      if (utype == SNL_T_SAMPLER2D) {
+        sampler2D s{nullptr, 0, 0, 4, 0};
+        memcpy(base + off, &s, sizeof s);
+      } else {
+        sampler3D s{nullptr, 0, 0, 0, 4, 0};
+        memcpy(base + off, &s, sizeof s);
+      }
+      kv.second = nullptr;
+      p->dirty = true;
+    }
+  }
   if (t->d) hipFree(t->d);
   delete t;
   return SNL_OK;
@@ -743,7 +785,7 @@ int snl_state_destroy(snl_state s) {
   hipSetDevice(s->ctx->device);
   hipStreamSynchronize(s->ctx->stream);
   for (void* p : {(void*)s->radiance, (void*)s->rng, (void*)s->pf, (void*)s->pi, (void*)s->q, (void*)s->cnt,
-                  (void*)s->tileCnt, (void*)s->radPrev, (void*)s->tileCost})
+                  (void*)s->tileCnt, (void*)s->radPrev, (void*)s->tileCost, (void*)s->errPartial})
     if (p) hipFree(p);
   delete s;
   return SNL_OK;
@@ -801,6 +843,14 @@ static int seedMode(snl_state s, unsigned seed, int mode) {
 }
 int snl_state_seed(snl_state s, unsigned seed) { return seedMode(s, seed, 3); }
 int snl_state_clear(snl_state s) { return seedMode(s, 0, 2); }
+
+int snl_state_info(snl_state s, int* width, int* height, int* chains) {
+  if (!s) return fail(nullptr, SNL_E_ARG, "state is NULL");
+  if (width) *width = s->w;
+  if (height) *height = s->h;
+  if (chains) *chains = s->chains;
+  return SNL_OK;
+}
 
 int snl_state_read(snl_state s, int which, float* outp) {
   if (!s || !outp || which < 0 || which > 1) return fail(s ? s->ctx : nullptr, SNL_E_ARG, "bad argument");
@@ -1133,14 +1183,15 @@ int snl_state_error(snl_state st, double* out) {
   snl_ctx ctx = st->ctx;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const int blocks = 256;
-  double* d = nullptr;
-  HIPCHK(ctx, hipMallocAsync((void**)&d, (2 * blocks + 2) * sizeof(double), ctx->stream));
+  // block partials and the result: allocated once per state, on its first estimate
+  if (!st->errPartial && hipMalloc((void**)&st->errPartial, (2 * blocks + 2) * sizeof(double)) != hipSuccess)
+    return fail(ctx, SNL_E_NOMEM, "error-estimate partials: allocation failed");
+  double* d = st->errPartial;
   hipLaunchKernelGGL(chain_error_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
                      reinterpret_cast<const float4*>(st->radiance), (long long)st->w * st->h, st->chains, d);
   hipLaunchKernelGGL(error_sum_kernel, dim3(1), dim3(64), 0, ctx->stream, d, blocks, d + 2 * blocks);
   HIPCHK(ctx, hipGetLastError());
   HIPCHK(ctx, hipMemcpyAsync(out, d + 2 * blocks, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(ctx, hipFreeAsync(d, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return SNL_OK;
 }
@@ -1156,8 +1207,23 @@ int snl_ctx_mem_info(snl_ctx ctx, unsigned long long* free_bytes, unsigned long 
 }
 
 unsigned long long snl_state_bytes(int width, int height, int chains) {
-  return (unsigned long long)width * height * (unsigned long long)(chains > 0 ? chains : 1) * 32ull +
-         (unsigned long long)(SNL_MAX_TILES + 1) * C_COUNT * 4ull;
+  if (width <= 0 || height <= 0) return 0ull;
+  const unsigned long long npix = (unsigned long long)width * height * (unsigned long long)(chains > 0 ? chains : 1);
+  const unsigned long long tiles = (unsigned long long)(width >> 3) * (unsigned long long)(height >> 3);
+  return npix * 32ull                                        // radiance + RNG
+         + (unsigned long long)(SNL_MAX_TILES + 1) * C_COUNT * 4ull  // queue counters
+         + (2ull * tiles + kCostBuckets) * 4ull                 // tile costs, histogram, order
+         + (2ull * 256 + 2) * 8ull;                             // error-estimate partials
+}
+
+unsigned long long snl_program_state_bytes(snl_prog p, int width, int height, int chains) {
+  unsigned long long b = snl_state_bytes(width, height, chains);
+  if (!p) return b;
+  bool unit0 = false;
+  for (int idx : p->sampled2D) unit0 |= !p->bound.count(idx);
+  if (unit0)  // the per-frame radiance snapshot for samplers left on texture unit 0
+    b += (unsigned long long)width * height * (unsigned long long)(chains > 0 ? chains : 1) * 16ull;
+  return b;
 }
 
 // batched jobs run the megakernel, or the persistent kernel where that is
@@ -1181,11 +1247,16 @@ int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, 
   if (njobs == 0) return SNL_OK;
   snl_ctx ctx = progs[0] ? progs[0]->ctx : nullptr;
   if (!ctx) return fail(nullptr, SNL_E_ARG, "NULL program in batch");
+  // a program's uniforms (resolution, unit-0 snapshot) belong to one job at a
+  // time, as a state's buffers do: each may appear once
   std::set<snl_state> seen;
+  std::set<snl_prog> seenProg;
   for (int j = 0; j < njobs; ++j) {
     if (!progs[j] || !states[j] || frames[j] < 0) return fail(ctx, SNL_E_ARG, "bad job %d", j);
     if (progs[j]->ctx != ctx || states[j]->ctx != ctx) return fail(ctx, SNL_E_ARG, "job %d: other context", j);
     if (!seen.insert(states[j]).second) return fail(ctx, SNL_E_ARG, "job %d: state used twice", j);
+    if (!seenProg.insert(progs[j]).second)
+      return fail(ctx, SNL_E_ARG, "job %d: program used twice (its uniforms are per job; build one per job)", j);
   }
   HIPCHK(ctx, hipSetDevice(ctx->device));
   // parameters up, per-job counters zeroed, all on the main stream

@@ -575,6 +575,14 @@ def has_hook(shader, name):
     return shader.find(name + "(") != -1
 
 
+def reads_texture_unit0(scene):
+    """whether the scene's program samples texture unit 0 -- the radiance
+    being rendered -- and so needs a per-frame snapshot of it: a tabulated
+    metal's K_METAL reads kTex, which the reference never binds
+    (js/materials.js:664-665), and only a scene with a metal calls it"""
+    return isinstance(scene.materials.metal, TabulatedMetal) and has_hook(scene.glsl, "SDF_METAL")
+
+
 def assemble(scene_glsl, r, mats):
     """Renderer.compileShaders (js/renderer.js:217-316): inject default hooks,
     derive the feature #defines, return (shader, ior_code, defines)."""

@@ -13,7 +13,9 @@ const fs = require('fs');
 const path = require('path');
 
 const HERE = __dirname;
-const addon = require(path.join(HERE, '..', 'snelly_napi.node'));
+// SNL_NAPI_ADDON: a module standing in for the addon, for tests that record
+// the calls this host makes
+const addon = require(process.env.SNL_NAPI_ADDON || path.join(HERE, '..', 'snelly_napi.node'));
 const TABLES = JSON.parse(fs.readFileSync(path.join(HERE, '..', 'data', 'spectral_tables.json'), 'utf8'));
 
 const T = { FLOAT: 1, VEC2: 2, VEC3: 3, VEC4: 4, INT: 5, MAT2: 6, MAT3: 7, MAT4: 8 };
@@ -243,6 +245,8 @@ class GLTexture {
   constructor() {
     this.w = 1; this.h = 1; this.d = 1; this.ch = 4; this.data = null;
     this.linear = true; this.clamp = false; this.handle = null; this.dirty = true;
+    this.shape = '';              // w x h x d x ch, linear, clamp of the device texture in `handle`
+    this.bindings = new Map();    // program -> sampler uniform indices bound to this texture
   }
 }
 
@@ -251,9 +255,15 @@ class WebGLStandIn {
     Object.assign(this, GLC);
     this.unit = 0;
     this.units = {};
+    this.slotTex = new Map();   // program -> (sampler uniform index -> the GLTexture bound there)
   }
   createTexture() { return new GLTexture(); }
-  deleteTexture(t) { if (t && t.handle) { addon.textureDestroy(t.handle); t.handle = null; } }
+  // the runtime resets samplers still bound to a deleted texture to an empty
+  // unit, which reads (0, 0, 0, 1) as in WebGL
+  deleteTexture(t) {
+    if (t && t.handle) { addon.textureDestroy(t.handle); t.handle = null; }
+    if (t) t.bindings.clear();
+  }
   activeTexture(u) { this.unit = u - GLC.TEXTURE0; }
   bindTexture(target, t) { this.units[this.unit] = t; }
   pixelStorei() {}
@@ -295,13 +305,25 @@ class WebGLStandIn {
     if (t && level === 0) this._store(t, w, h, depth, format, type, data);
   }
   getUniformLocation(prog, name) { return { prog, name, idx: addon.uniformIndex(prog, name) }; }
+  // The device copy of a texture object.  A texture object keeps its
+  // bindings when its image is re-specified (WebGL): the same shape updates the
+  // device texture in place; a new shape or filter gets a new device texture,
+  // bound again to every sampler that held the old one before that is freed.
   _upload(t) {
     const ctx = SNELLY ? SNELLY.ctx : null;
     if (!ctx) return null;
     if (t.dirty || !t.handle) {
-      if (t.handle) addon.textureDestroy(t.handle);
       const data = t.data || new Float32Array(t.w * t.h * t.d * t.ch);
-      t.handle = addon.textureCreate(ctx, t.w, t.h, t.d, t.ch, t.linear ? 1 : 0, t.clamp ? 1 : 0, data);
+      const shape = `${t.w}x${t.h}x${t.d}x${t.ch}:${t.linear ? 1 : 0}${t.clamp ? 1 : 0}`;
+      if (t.handle && shape === t.shape) {
+        addon.textureUpload(t.handle, data);
+      } else {
+        const old = t.handle;
+        t.handle = addon.textureCreate(ctx, t.w, t.h, t.d, t.ch, t.linear ? 1 : 0, t.clamp ? 1 : 0, data);
+        t.shape = shape;
+        for (const [prog, idxs] of t.bindings) for (const idx of idxs) addon.uniformTexture(prog, idx, t.handle);
+        if (old) addon.textureDestroy(old);
+      }
       t.dirty = false;
     }
     return t.handle;
@@ -311,7 +333,16 @@ class WebGLStandIn {
     const t = this.units[v];
     if (t instanceof GLTexture) {
       const h = this._upload(t);
-      if (h) addon.uniformTexture(loc.prog, loc.idx, h);
+      if (h) {
+        addon.uniformTexture(loc.prog, loc.idx, h);
+        if (!this.slotTex.has(loc.prog)) this.slotTex.set(loc.prog, new Map());
+        const slots = this.slotTex.get(loc.prog);
+        const prev = slots.get(loc.idx);
+        if (prev && prev !== t && prev.bindings.has(loc.prog)) prev.bindings.get(loc.prog).delete(loc.idx);
+        slots.set(loc.idx, t);
+        if (!t.bindings.has(loc.prog)) t.bindings.set(loc.prog, new Set());
+        t.bindings.get(loc.prog).add(loc.idx);
+      }
       return;
     }
     addon.uniformSet(loc.prog, loc.idx, T.INT, Int32Array.of(v), 1);
@@ -1215,5 +1246,7 @@ module.exports = {
   addon, T, BUILD, THREE, GLU, Shader, Texture, Spectrum, BlackbodySpectrum, FlatSpectrum, MonochromaticSpectrum,
   Material, Surface, Volume, Metal, TabulatedMetal, Dielectric, AbbeDielectric, SellmeierDielectric,
   Sellmeier2Dielectric, PolyanskiyDielectric, CauchyDielectric, Materials, Renderer, Snelly, GUI,
-  extractSceneScript, loadScene, sceneGlobals, wavelengthToXYZTable,
+  extractSceneScript, loadScene, sceneGlobals, wavelengthToXYZTable, WebGLStandIn, GLTexture,
+  // the context the stand-in uploads textures to (set by Snelly; tests may set it)
+  setContextForTests(ctx) { SNELLY = { ctx }; },
 };

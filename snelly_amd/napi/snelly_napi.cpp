@@ -75,6 +75,41 @@ void* typedData(napi_env env, napi_value v, size_t* count = nullptr) {
   if (count) *count = length;
   return data;
 }
+// the TypedArray's data when it has `type` elements and at least `need` of
+// them; otherwise a JS TypeError / RangeError is pending and nullptr returns
+void* checkedArray(napi_env env, napi_value v, napi_typedarray_type want, size_t need, const char* what) {
+  bool is = false;
+  napi_is_typedarray(env, v, &is);
+  if (!is) {
+    napi_throw_type_error(env, nullptr, (std::string(what) + ": expected a TypedArray").c_str());
+    return nullptr;
+  }
+  napi_typedarray_type type;
+  size_t length;
+  void* data;
+  napi_value ab;
+  size_t off;
+  napi_get_typedarray_info(env, v, &type, &length, &data, &ab, &off);
+  if (type != want) {
+    napi_throw_type_error(env, nullptr,
+                          (std::string(what) + (want == napi_int32_array ? ": expected an Int32Array"
+                                                                          : ": expected a Float32Array")).c_str());
+    return nullptr;
+  }
+  if (length < need) {
+    napi_throw_range_error(env, nullptr, (std::string(what) + ": " + std::to_string(length) + " elements, " +
+                                          std::to_string(need) + " needed").c_str());
+    return nullptr;
+  }
+  return data;
+}
+// float count of a state's radiance or RNG buffer (4 per pixel of every chain)
+bool stateFloats(snl_state s, size_t* n) {
+  int w = 0, h = 0, k = 0;
+  if (snl_state_info(s, &w, &h, &k) != SNL_OK) return false;
+  *n = (size_t)w * h * k * 4;
+  return true;
+}
 napi_value ext(napi_env env, void* p) {
   napi_value v;
   napi_create_external(env, p, nullptr, nullptr, &v);
@@ -159,9 +194,22 @@ napi_value UniformSet(napi_env env, napi_callback_info info) {
   Args a;
   if (!a.ok(env, info)) return nullptr;
   snl_prog p = static_cast<snl_prog>(getPtr(env, a.argv[0]));
-  void* data = typedData(env, a.argv[3]);
-  if (!data) { napi_throw_type_error(env, nullptr, "uniformSet: data must be a TypedArray"); return nullptr; }
-  int rc = snl_uniform_set(p, getInt(env, a.argv[1]), getInt(env, a.argv[2]), data, getInt(env, a.argv[4]));
+  const int type = getInt(env, a.argv[2]), count = getInt(env, a.argv[4]);
+  int per = 0;  // values of one element of `type` (include/snelly.h: SNL_T_*)
+  switch (type) {
+    case SNL_T_FLOAT: case SNL_T_INT: per = 1; break;
+    case SNL_T_VEC2: per = 2; break;
+    case SNL_T_VEC3: per = 3; break;
+    case SNL_T_VEC4: case SNL_T_MAT2: per = 4; break;
+    case SNL_T_MAT3: per = 9; break;
+    case SNL_T_MAT4: per = 16; break;
+    default: napi_throw_type_error(env, nullptr, "uniformSet: bad type"); return nullptr;
+  }
+  if (count <= 0) { napi_throw_range_error(env, nullptr, "uniformSet: count must be positive"); return nullptr; }
+  void* data = checkedArray(env, a.argv[3], type == SNL_T_INT ? napi_int32_array : napi_float32_array,
+                            (size_t)per * count, "uniformSet");
+  if (!data) return nullptr;
+  int rc = snl_uniform_set(p, getInt(env, a.argv[1]), type, data, count);
   if (rc) return throwSnl(env, rc, nullptr);
   return undef(env);
 }
@@ -179,17 +227,30 @@ napi_value TextureCreate(napi_env env, napi_callback_info info) {
   if (!a.ok(env, info)) return nullptr;
   snl_ctx c = static_cast<snl_ctx>(getPtr(env, a.argv[0]));
   snl_tex t = nullptr;
-  int rc = snl_texture_create(c, getInt(env, a.argv[1]), getInt(env, a.argv[2]), getInt(env, a.argv[3]),
-                              getInt(env, a.argv[4]), getInt(env, a.argv[5]), getInt(env, a.argv[6]),
-                              static_cast<const float*>(typedData(env, a.argv[7])), &t);
+  const int w = getInt(env, a.argv[1]), h = getInt(env, a.argv[2]), d = getInt(env, a.argv[3]), ch = getInt(env, a.argv[4]);
+  const float* texels = nullptr;
+  napi_valuetype vt;
+  napi_typeof(env, a.argv[7], &vt);
+  if (vt != napi_null && vt != napi_undefined) {
+    if (w <= 0 || h <= 0 || d <= 0 || ch <= 0) { napi_throw_range_error(env, nullptr, "textureCreate: bad shape"); return nullptr; }
+    texels = static_cast<const float*>(checkedArray(env, a.argv[7], napi_float32_array, (size_t)w * h * d * ch,
+                                                    "textureCreate"));
+    if (!texels) return nullptr;
+  }
+  int rc = snl_texture_create(c, w, h, d, ch, getInt(env, a.argv[5]), getInt(env, a.argv[6]), texels, &t);
   if (rc) return throwSnl(env, rc, c);
   return ext(env, t);
 }
 napi_value TextureUpload(napi_env env, napi_callback_info info) {
   Args a;
   if (!a.ok(env, info)) return nullptr;
-  int rc = snl_texture_upload(static_cast<snl_tex>(getPtr(env, a.argv[0])),
-                              static_cast<const float*>(typedData(env, a.argv[1])));
+  snl_tex t = static_cast<snl_tex>(getPtr(env, a.argv[0]));
+  int w = 0, h = 0, d = 0, ch = 0;
+  if (snl_texture_info(t, &w, &h, &d, &ch) != SNL_OK) return throwSnl(env, SNL_E_ARG, nullptr);
+  const float* texels = static_cast<const float*>(checkedArray(env, a.argv[1], napi_float32_array,
+                                                               (size_t)w * h * d * ch, "textureUpload"));
+  if (!texels) return nullptr;
+  int rc = snl_texture_upload(t, texels);
   if (rc) return throwSnl(env, rc, nullptr);
   return undef(env);
 }
@@ -230,21 +291,29 @@ napi_value StateClear(napi_env env, napi_callback_info info) {
   if (rc) return throwSnl(env, rc, nullptr);
   return undef(env);
 }
-// stateRead(state, which, npix) -> Float32Array(4*npix)
+// stateRead(state, which) -> Float32Array(4 * width * height * chains), sized
+// from the state itself
 napi_value StateRead(napi_env env, napi_callback_info info) {
   Args a;
   if (!a.ok(env, info)) return nullptr;
+  snl_state st = static_cast<snl_state>(getPtr(env, a.argv[0]));
+  size_t n = 0;
+  if (!stateFloats(st, &n)) return throwSnl(env, SNL_E_ARG, nullptr);
   float* out = nullptr;
-  napi_value arr = floatArray(env, (size_t)getNum(env, a.argv[2]) * 4, &out);
-  int rc = snl_state_read(static_cast<snl_state>(getPtr(env, a.argv[0])), getInt(env, a.argv[1]), out);
+  napi_value arr = floatArray(env, n, &out);
+  int rc = snl_state_read(st, getInt(env, a.argv[1]), out);
   if (rc) return throwSnl(env, rc, nullptr);
   return arr;
 }
 napi_value StateWrite(napi_env env, napi_callback_info info) {
   Args a;
   if (!a.ok(env, info)) return nullptr;
-  int rc = snl_state_write(static_cast<snl_state>(getPtr(env, a.argv[0])), getInt(env, a.argv[1]),
-                           static_cast<const float*>(typedData(env, a.argv[2])));
+  snl_state st = static_cast<snl_state>(getPtr(env, a.argv[0]));
+  size_t n = 0;
+  if (!stateFloats(st, &n)) return throwSnl(env, SNL_E_ARG, nullptr);
+  const float* in = static_cast<const float*>(checkedArray(env, a.argv[2], napi_float32_array, n, "stateWrite"));
+  if (!in) return nullptr;
+  int rc = snl_state_write(st, getInt(env, a.argv[1]), in);
   if (rc) return throwSnl(env, rc, nullptr);
   return undef(env);
 }
@@ -269,29 +338,35 @@ napi_value Pick(napi_env env, napi_callback_info info) {
   Args a;
   if (!a.ok(env, info)) return nullptr;
   size_t n = 0;
-  const float* xy = static_cast<const float*>(typedData(env, a.argv[1], &n));
+  typedData(env, a.argv[1], &n);
+  const float* xy = static_cast<const float*>(checkedArray(env, a.argv[1], napi_float32_array, n & ~(size_t)1, "pick"));
+  if (!xy) return nullptr;
+  n &= ~(size_t)1;
   float* out = nullptr;
   napi_value arr = floatArray(env, n, &out);
   int rc = snl_pick(static_cast<snl_prog>(getPtr(env, a.argv[0])), xy, (int)(n / 2), out);
   if (rc) return throwSnl(env, rc, nullptr);
   return arr;
 }
-// tonemap(prog, state, npix) -> Float32Array rgb (3 per pixel)
+// tonemap(prog, state) -> Float32Array rgb (3 per pixel of the state)
 napi_value Tonemap(napi_env env, napi_callback_info info) {
   Args a;
   if (!a.ok(env, info)) return nullptr;
+  size_t n4 = 0;
+  if (!stateFloats(static_cast<snl_state>(getPtr(env, a.argv[1])), &n4)) return throwSnl(env, SNL_E_ARG, nullptr);
   float* out = nullptr;
-  napi_value arr = floatArray(env, (size_t)getNum(env, a.argv[2]) * 3, &out);
+  napi_value arr = floatArray(env, n4 / 4 * 3, &out);
   int rc = snl_tonemap(static_cast<snl_prog>(getPtr(env, a.argv[0])), static_cast<snl_state>(getPtr(env, a.argv[1])),
                        out, nullptr);
   if (rc) return throwSnl(env, rc, nullptr);
   return arr;
 }
-// tonemapRGBA8(prog, state, npix) -> Uint8Array rgba (4 per pixel, GL UNORM8)
+// tonemapRGBA8(prog, state) -> Uint8Array rgba (4 per pixel of the state, GL UNORM8)
 napi_value TonemapRGBA8(napi_env env, napi_callback_info info) {
   Args a;
   if (!a.ok(env, info)) return nullptr;
-  size_t n = (size_t)getNum(env, a.argv[2]) * 4;
+  size_t n = 0;
+  if (!stateFloats(static_cast<snl_state>(getPtr(env, a.argv[1])), &n)) return throwSnl(env, SNL_E_ARG, nullptr);
   napi_value ab, arr;
   void* raw = nullptr;
   napi_create_arraybuffer(env, n, &raw, &ab);

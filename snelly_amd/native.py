@@ -26,7 +26,7 @@ EXPORTS = [
     "snl_state_write", "snl_render", "snl_render_normals", "snl_pick", "snl_tonemap", "snl_sync",
     "snl_get_stats", "snl_set_tuning", "snl_set_profiling", "snl_set_engine", "snl_render_batch",
     "snl_state_create_chains", "snl_state_error", "snl_ctx_mem_info", "snl_state_bytes", "snl_set_march_burst",
-    "snl_render_noalloc",
+    "snl_render_noalloc", "snl_program_state_bytes", "snl_state_info", "snl_texture_info",
 ]
 ENGINE_WAVEFRONT, ENGINE_PERSISTENT, ENGINE_MEGAKERNEL = 0, 1, 2
 
@@ -87,6 +87,9 @@ def lib():
         "snl_state_error": (ip, [vp, ctypes.POINTER(ctypes.c_double)]),
         "snl_ctx_mem_info": (ip, [vp, ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_ulonglong)]),
         "snl_state_bytes": (ctypes.c_ulonglong, [ip, ip, ip]),
+        "snl_program_state_bytes": (ctypes.c_ulonglong, [vp, ip, ip, ip]),
+        "snl_state_info": (ip, [vp] + [ctypes.POINTER(ip)] * 3),
+        "snl_texture_info": (ip, [vp] + [ctypes.POINTER(ip)] * 4),
         "snl_set_march_burst": (ip, [vp, ip]),
     }
     for name, (res, args) in sig.items():
@@ -121,7 +124,11 @@ def mem_info(ctx):
     return f.value, t.value
 
 
-def state_bytes(width, height, chains=1):
+def state_bytes(width, height, chains=1, prog=None):
+    """device bytes of a state; with `prog`, those of a state rendered by that
+    program (its texture-unit-0 snapshot included)"""
+    if prog is not None:
+        return int(lib().snl_program_state_bytes(prog.h, width, height, chains))
     return int(lib().snl_state_bytes(width, height, chains))
 
 

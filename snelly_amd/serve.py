@@ -149,6 +149,8 @@ class Server:
                 job = queue[0]
                 k = Job.chains_for(job.pixels, self.chain_lanes, self.max_chains)
                 need = native.state_bytes(job.scene.width, job.scene.height, k)
+                if host.reads_texture_unit0(job.scene):
+                    need += 16 * job.pixels * k  # the per-frame radiance snapshot
                 if need + self.reserve > free:
                     break
                 queue.pop(0)

@@ -91,3 +91,13 @@ def test_megakernel_rebuilt_without_occupancy_bound_where_it_spills(tmp_path, mo
     native.precompile(*warm.program_key(scenes.menger_metal(64, 64, 4), native.BUILD_FAST))
     objs = list(tmp_path.glob("clean/*.hsaco"))
     assert len(objs) == 1 and kernel_private_bytes(str(objs[0]), "snl_mega.kd") == 0
+
+
+def test_state_bytes_counts_every_buffer_of_a_state():
+    """radiance + RNG (32 B/px over all chains), the queue counters, the tile
+    order's costs / histogram / order and the error estimate's partials"""
+    from snelly_amd import native
+    W, H, K = 1024, 768, 3
+    tiles = (W // 8) * (H // 8)
+    assert native.state_bytes(W, H, K) == W * H * K * 32 + 5 * 16 * 4 + (2 * tiles + 128) * 4 + 514 * 8
+    assert native.state_bytes(0, 10, 1) == 0

@@ -229,3 +229,190 @@ console.log(JSON.stringify([sn.getCamera().focalDistance, p.distance, sn.lengthS
         assert (w, h) == (40, 30)
         want = np.array(rgba, np.uint8).reshape(30, 40, 4)[::-1, :, :3]
         assert np.array_equal(rgb, want)
+
+
+STUB_ADDON = """
+let next = 1; const log = [];
+module.exports = {
+  _log: log,
+  textureCreate: (...a) => { const h = { id: next++ }; log.push(['create', h.id, a.slice(1, 5).join('x')]); return h; },
+  textureUpload: (h) => log.push(['upload', h.id]),
+  textureDestroy: (h) => log.push(['destroy', h.id]),
+  uniformTexture: (p, idx, h) => log.push(['bind', p.name, idx, h ? h.id : null]),
+  uniformIndex: (p, name) => (name === 'a' ? 0 : (name === 'b' ? 1 : -1)),
+  uniformSet: () => {},
+  version: () => 'recording stand-in',
+};
+"""
+
+
+@needs_node
+def test_texture_object_keeps_bindings_when_respecified():
+    """the WebGL stand-in, over a recording stand-in for the addon: re-uploading
+    an image of the same shape updates the device texture in place; a new shape
+    makes a new device texture, bound again to every sampler that held the old
+    one before the old one is freed; deleting a texture frees it"""
+    with tempfile.TemporaryDirectory() as td:
+        stub = os.path.join(td, "stub_addon.js")
+        with open(stub, "w") as f:
+            f.write(STUB_ADDON)
+        code = f"""
+const host = require({json.dumps(HOST_JS)});
+const stub = require({json.dumps(stub)});
+host.setContextForTests({{}});
+const gl = new host.WebGLStandIn();
+const P = {{ name: 'P' }}, Q = {{ name: 'Q' }};
+const t = gl.createTexture(), u = gl.createTexture();
+gl.activeTexture(gl.TEXTURE0 + 7); gl.bindTexture(gl.TEXTURE_2D, t);
+gl.texImage2D(gl.TEXTURE_2D, 0, gl.RGBA32F, 2, 2, 0, gl.RGBA, gl.FLOAT, new Float32Array(16));
+gl.uniform1i(gl.getUniformLocation(P, 'a'), 7);
+gl.uniform1i(gl.getUniformLocation(Q, 'a'), 7);
+gl.texImage2D(gl.TEXTURE_2D, 0, gl.RGBA32F, 2, 2, 0, gl.RGBA, gl.FLOAT, new Float32Array(16));
+gl.uniform1i(gl.getUniformLocation(P, 'a'), 7);
+gl.texImage2D(gl.TEXTURE_2D, 0, gl.RGBA32F, 4, 2, 0, gl.RGBA, gl.FLOAT, new Float32Array(32));
+gl.uniform1i(gl.getUniformLocation(P, 'a'), 7);
+// Q's slot moves to texture u: a later re-specification of t leaves it alone
+gl.activeTexture(gl.TEXTURE0 + 8); gl.bindTexture(gl.TEXTURE_2D, u);
+gl.texImage2D(gl.TEXTURE_2D, 0, gl.RGBA32F, 1, 1, 0, gl.RGBA, gl.FLOAT, new Float32Array(4));
+gl.uniform1i(gl.getUniformLocation(Q, 'a'), 8);
+gl.activeTexture(gl.TEXTURE0 + 7);
+gl.texImage2D(gl.TEXTURE_2D, 0, gl.RGBA32F, 8, 2, 0, gl.RGBA, gl.FLOAT, new Float32Array(64));
+gl.uniform1i(gl.getUniformLocation(P, 'a'), 7);
+gl.deleteTexture(u);
+console.log(JSON.stringify(stub._log));
+"""
+        r = subprocess.run([NODE, "-e", code], capture_output=True, text=True,
+                           env=dict(os.environ, SNL_NAPI_ADDON=stub))
+        assert r.returncode == 0, r.stderr
+        log = json.loads(r.stdout.strip().splitlines()[-1])
+    assert log == [["create", 1, "2x2x1x4"], ["bind", "P", 0, 1], ["bind", "Q", 0, 1],
+                   ["upload", 1], ["bind", "P", 0, 1],
+                   ["create", 2, "4x2x1x4"], ["bind", "P", 0, 2], ["bind", "Q", 0, 2], ["destroy", 1],
+                   ["bind", "P", 0, 2],
+                   ["create", 3, "1x1x1x4"], ["bind", "Q", 0, 3],
+                   ["create", 4, "8x2x1x4"], ["bind", "P", 0, 4], ["destroy", 2], ["bind", "P", 0, 4],
+                   ["destroy", 3]]
+
+
+@needs_node
+def test_addon_checks_buffer_types_and_lengths():
+    """the addon checks each TypedArray against what the call reads before it
+    calls the library (these calls fail before any GPU work)"""
+    code = f"""
+const a = require({json.dumps(ADDON)});
+const out = [];
+function t(f) {{ try {{ f(); out.push('ok'); }} catch (e) {{ out.push(e.constructor.name + ': ' + e.message); }} }}
+t(() => a.uniformSet(null, 0, 3, new Float32Array(5), 2));       // vec3 x 2 needs 6 floats
+t(() => a.uniformSet(null, 0, 5, new Float32Array(1), 1));       // int needs an Int32Array
+t(() => a.uniformSet(null, 0, 1, [1.0], 1));                     // not a TypedArray
+t(() => a.textureCreate(null, 2, 2, 1, 4, 1, 1, new Float32Array(15)));
+t(() => a.stateWrite(null, 0, new Float32Array(4)));             // no state: no size to check against
+console.log(JSON.stringify(out));
+"""
+    r = subprocess.run([NODE, "-e", code], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out[0].startswith("RangeError") and "6 needed" in out[0]
+    assert out[1].startswith("TypeError") and "Int32Array" in out[1]
+    assert out[2].startswith("TypeError")
+    assert out[3].startswith("RangeError") and "16 needed" in out[3]
+    assert out[4].startswith("Error") and "state is NULL" in out[4]
+
+
+def gl_linear(tex, coords):
+    """GL LINEAR + CLAMP_TO_EDGE sampling of a texture array (..., C) whose
+    leading axes are (depth,) height, width -- texel centres at (i + 0.5) / n"""
+    dims = tex.shape[:-1][::-1]                 # width, height[, depth]
+    idx0, frac = [], []
+    for k, n in enumerate(dims):
+        x = coords[:, k] * n - 0.5
+        f = np.floor(x)
+        idx0.append(f.astype(int))
+        frac.append(x - f)
+    out = np.zeros((coords.shape[0], tex.shape[-1]))
+    import itertools
+    for corner in itertools.product((0, 1), repeat=len(dims)):
+        w = np.ones(coords.shape[0])
+        ii = []
+        for k, n in enumerate(dims):
+            w = w * (frac[k] if corner[k] else 1.0 - frac[k])
+            ii.append(np.clip(idx0[k] + corner[k], 0, n - 1))
+        out += w[:, None] * tex[tuple(ii[::-1])]
+    return out
+
+
+@needs_node
+@pytest.mark.gpu
+def test_js_user_textures_on_gpu():
+    """a scene that uploads a sampler3D (RGB32F) and a sampler2D (RGBA8) through
+    raw WebGL calls at the user texture units and samples them in its albedo
+    hook.  A convex Lambertian sphere under a white sky: a pixel's expected
+    colour is the albedo at its hit point, and every sample scales all three
+    channels alike, so each pixel's colour ratios match the texture value
+    restated here (GL LINEAR / CLAMP_TO_EDGE) with no Monte-Carlo noise"""
+    W = H = 48
+    frames = 64
+    with tempfile.TemporaryDirectory() as td:
+        rad_f = os.path.join(td, "rad.f32")
+        r = run_node([os.path.join(SCENES, "user_textures.html"), "--precise", "--width", str(W), "--height", str(H),
+                      "--frames", str(frames), "--seed", "3", "--out", rad_f, "--json"])
+        assert r.returncode == 0, r.stderr
+        g = np.fromfile(rad_f, np.float32).reshape(-1, 4).astype(np.float64)
+    img = g[:, :3] * (g[:, 3:] / (g[:, 3:] - 1.0))
+    # the textures of the scene file
+    t3 = np.array([[[[0.2 + 0.7 * (((i * 7 + j * 13 + k * 29 + c * 3) % 17) / 16.0) for c in range(3)]
+                     for i in range(4)] for j in range(4)] for k in range(4)])
+    t2 = np.array([[[(40 + ((i * 11 + j * 23 + c * 5) % 9) * 25) / 255.0 for c in range(4)]
+                    for i in range(8)] for j in range(4)])
+    # each pixel's (unjittered) camera ray and its hit on the unit sphere
+    fh = np.tan(0.5 * np.radians(45.0))
+    ys, xs = np.mgrid[0:H, 0:W]
+    nx = -1.0 + 2.0 * (xs.ravel() + 0.5) / W
+    ny = -1.0 + 2.0 * (ys.ravel() + 0.5) / H
+    d = np.stack([fh * nx, fh * ny, -np.ones_like(nx)], -1)   # camera at z = 4 looking down -z: camX = -x
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    o = np.array([0.0, 0.0, 4.0])
+    b = d @ o
+    disc = b * b - (o @ o - 1.0)
+    hit = disc > 1e-3
+    t = -b - np.sqrt(np.maximum(disc, 0.0))
+    M = np.array([[0.4124564, 0.3575761, 0.1804375], [0.2126729, 0.7151522, 0.0721750],
+                  [0.0193339, 0.1191920, 0.9503041]])
+
+    def expected(X):
+        albedo = np.zeros((W * H, 3))
+        pos = hit & (X[:, 0] > 0.0)
+        neg = hit & (X[:, 0] <= 0.0)
+        albedo[pos] = gl_linear(t3, 0.5 * X[pos] + 0.5)
+        uv = np.stack([np.arctan2(X[neg, 2], X[neg, 0]) / (2 * np.pi) + 0.5, 0.5 * X[neg, 1] + 0.5], -1)
+        albedo[neg] = gl_linear(t2, uv)[:, :3]
+        return albedo @ M.T
+
+    # Where the albedo is read: the sphere tracer stops at the first point whose
+    # SDF is below minLengthScale (1e-4), on the ray between the analytic hit
+    # and 1e-4 / cos(incidence) before it (cos(incidence) on the unit sphere is
+    # sqrt(disc)); the sampled bounce reads it there, and the direct-light
+    # estimate reads it at that point pushed 3 minLengthScale out along the
+    # normal.  A pixel's colour is a positive mix of those reads, so its
+    # chromaticity lies within theirs.
+    X = o + t[:, None] * d
+    Xe = o + (t - 1.5e-4 / np.sqrt(np.maximum(disc, 1e-6)))[:, None] * d
+    reads = [X, Xe, X + 3e-4 * X / np.linalg.norm(X, axis=1, keepdims=True),
+             Xe + 3e-4 * Xe / np.linalg.norm(Xe, axis=1, keepdims=True)]
+    want = expected(X)
+    # pixels whose branch or seam a last-bit difference in the hit point could flip
+    safe = hit & (np.abs(X[:, 0]) > 2e-3) & ~((X[:, 0] < 0) & (np.abs(X[:, 2]) < 2e-3)) & (disc > 0.05)
+    assert safe.sum() > 600
+
+    def chroma(v):
+        return v / v.sum(axis=1, keepdims=True)
+    c = chroma(img[safe])
+    cs = np.stack([chroma(expected(r)[safe]) for r in reads])
+    assert np.all(c >= cs.min(0) - 2e-6) and np.all(c <= cs.max(0) + 2e-6)
+    # every sample scales the three channels alike: the colour is the albedo's
+    s = img[safe].sum(axis=1) / want[safe].sum(axis=1)
+    # and that common scale is 1 in expectation (the sky is white)
+    assert abs(s.mean() - 1.0) < 4.0 * s.std() / np.sqrt(s.size) + 1e-3
+    # the rays that miss see the sky: rgbToXyz(1, 1, 1)
+    miss = ~hit & (disc < -1e-3)
+    assert np.allclose(img[miss], M.sum(axis=1), rtol=1e-5)

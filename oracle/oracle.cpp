@@ -1514,12 +1514,21 @@ struct Tracer {
 inline V3 toXYZ(float L, V3 xyz) { return xyz * L; }  // dispersive: xyz * L
 inline V3 toXYZ(V3 L, V3) { return rgbToXyz(L); }
 
+// GLSL's tan, correctly rounded (through double), as the HIP path computes
+// it: the camera's frustum height scales every primary ray
+inline float tanCR(float x) {
+#ifdef ORACLE_COUNT
+  orccount::trans();
+#endif
+  return (float)std::tan((double)x);
+}
+
 float sample_jitter(float xi) { return xi < 0.5f ? std::sqrt(2.0f * xi) - 1.0f : 1.0f - std::sqrt(2.0f - 2.0f * xi); }
 
 void constructPrimaryRay(float px, float py, float* rnd, V3& start, V3& dir, bool safe) {
   float ndcx = -1.0f + 2.0f * (px / P.resolution[0]);
   float ndcy = -1.0f + 2.0f * (py / P.resolution[1]);
-  float fh = std::tan(0.5f * (P.camFovy * 0.017453292519943295f));
+  float fh = tanCR(0.5f * (P.camFovy * 0.017453292519943295f));
   float fw = P.camAspect * fh;
   V3 s = -fw * ndcx * A3(P.camX) + fh * ndcy * A3(P.camY);
   dir = safe ? safe_normalize(A3(P.camDir) + s) : normalize(A3(P.camDir) + s);

@@ -371,6 +371,9 @@ int launch(snl_ctx ctx, hipFunction_t f, int grid, int block, SnlKArgs& a) {
 
 int upload(snl_prog p) {
   if (!p->dirty) return SNL_OK;
+  // 0.5 * radians(camFovy) in float, as the shaders form the argument
+  const float halfFovy = 0.5f * (p->hostP.camFovy * 0.017453292519943295f);
+  p->hostP.camFrustumHeight = (float)std::tan((double)halfFovy);
   HIPCHK(p->ctx, hipMemcpyAsync(p->dP, &p->hostP, sizeof(SnlParams), hipMemcpyHostToDevice, p->ctx->stream));
   if (!p->hostU.empty())
     HIPCHK(p->ctx, hipMemcpyAsync(p->dU, p->hostU.data(), p->hostU.size(), hipMemcpyHostToDevice, p->ctx->stream));

@@ -695,7 +695,7 @@ __device__ inline void pick_body(const SnlKArgs& a, const float* xy, float* out,
   if (k >= n) return;
   vec2 pixel(xy[2 * k], xy[2 * k + 1]);
   vec2 ndc = -1.0f + 2.0f * (pixel / P.resolution);
-  float fh = tan(0.5f * radians(P.camFovy));
+  float fh = P.camFrustumHeight;  // tan(0.5 * radians(camFovy)), formed once per upload
   float fw = P.camAspect * fh;
   vec3 s = -fw * ndc.x * P.camX + fh * ndc.y * P.camY;
   vec3 primaryDir = normalize(P.camDir + s);

@@ -116,5 +116,12 @@ struct SnlParams {
   SNL_BUILTIN_SAMPLERS(SNL_DECL)
   SNL_BUILTIN_VALUES(SNL_DECL)
 #undef SNL_DECL
+  // Derived by the runtime at upload, not a uniform: the camera's frustum
+  // height tan(0.5 radians(camFovy)), correctly rounded (through double).  The
+  // reference's shaders compute it per fragment (pathtracer-fragment-shader.glsl:1607,
+  // normals-fragment-shader.glsl:276, pick-fragment-shader.glsl:126,
+  // ao-fragment-shader.glsl:378, simplepathtracer-fragment-shader.glsl:874);
+  // it scales every primary ray, so one ulp of it moves every first hit.
+  float camFrustumHeight;
 };
 }  // namespace glsl

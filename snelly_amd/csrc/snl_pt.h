@@ -873,7 +873,7 @@ struct PT {
   // ---------------- camera (1602-1622)
   SNL_HD void constructPrimaryRay(const vec2& pixel, vec4& rnd, vec3& primaryStart, vec3& primaryDir, bool safe) {
     vec2 ndc = -1.0f + 2.0f * (pixel / P.resolution);
-    float fh = tan(0.5f * radians(P.camFovy));
+    float fh = P.camFrustumHeight;  // tan(0.5 * radians(camFovy)), formed once per upload
     float fw = P.camAspect * fh;
     vec3 s = -fw * ndc.x * P.camX + fh * ndc.y * P.camY;
     primaryDir = safe ? safe_normalize(P.camDir + s) : normalize(P.camDir + s);

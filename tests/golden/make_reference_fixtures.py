@@ -303,6 +303,48 @@ def run_tiny(td, tag, body, x, W, H, math=False):
     return (np.fromfile(f["o0"], np.float32).reshape(-1, 4), np.fromfile(f["o1"], np.float32).reshape(-1, 4))
 
 
+def check_mix_normalize(td):
+    """mix and normalize of the math block are bit-exact against their
+    definitions evaluated in float32 (x * (1 - a) + y * a; x / sqrt(x.x x.x +
+    x.y x.y + x.z x.z) per component), as the oracle computes them"""
+    W = H = 32
+    rs = np.random.RandomState(3)
+    x = rs.uniform(-3, 3, (W * H, 4)).astype(np.float32)
+    body = ("  o0 = vec4(mix(r.x, r.y, r.z), mix(r.xyz, r.yzw, r.w).xy, mix(r.xy, r.zw, r.yx).y);\n"
+            "  o1 = vec4(normalize(r.xyz), normalize(r.zw).x);")
+    o0, o1 = run_tiny(td, "mixnorm", body, x, W, H, math=True)
+    f = np.float32
+    one = f(1)
+
+    def mix(a, b, t):
+        return (a * (one - t) + b * t).astype(np.float32)
+    X, Y, Z, Wc = (x[:, k] for k in range(4))
+    want0 = np.stack([mix(X, Y, Z), mix(X, Y, Wc), mix(Y, Z, Wc), mix(Y, Wc, X)], -1)
+    l3 = np.sqrt(X * X + Y * Y + Z * Z).astype(np.float32)
+    l2 = np.sqrt(Z * Z + Wc * Wc).astype(np.float32)
+    want1 = np.stack([X / l3, Y / l3, Z / l3, Z / l2], -1).astype(np.float32)
+    if not (np.array_equal(o0, want0) and np.array_equal(o1, want1)):
+        raise RuntimeError("mix/normalize of the math block differ from their float32 definitions: "
+                           f"{np.mean(o0 == want0, axis=0)} {np.mean(o1 == want1, axis=0)}")
+
+
+def check_camera_tan(td, names):
+    """the camera's frustum height tan(0.5 * radians(camFovy)), as the math
+    block computes it under the GLES implementation, is the correctly rounded
+    float for every case's field of view -- the value the oracle and the HIP
+    path take (through double); one ulp there tilts every primary ray"""
+    fovs = sorted({float(np.float32(host.frame_uniforms(s.renderer, s.camera, s.scales, s.materials, W, H)["camFovy"]))
+                   for s, W, H, _, _ in (reference_cases.build(n) for n in names)})
+    x = np.zeros((len(fovs), 4), np.float32)
+    x[:, 0] = fovs
+    o0, _ = run_tiny(td, "camtan", "  o0 = vec4(tan(0.5 * radians(r.x)), 0.0, 0.0, 0.0);\n  o1 = vec4(0.0);",
+                     x, len(fovs), 1, math=True)
+    f = np.float32
+    want = np.tan((f(0.5) * (x[:, 0] * f(0.017453292519943295))).astype(np.float64)).astype(np.float32)
+    if not np.array_equal(o0[:, 0], want):
+        raise RuntimeError(f"camera tan not correctly rounded for fovy {fovs}: {o0[:, 0]} vs {want}")
+
+
 def check_min_max(td):
     """min, max and clamp of the math block follow the GLSL definitions with a
     NaN operand: min(3, NaN) = 3, max(1e-6, NaN) = 1e-6, min(NaN, 3) = NaN,
@@ -433,6 +475,8 @@ def main():
         print("precise_math ulp:", json.dumps({k: round(v, 2) for k, v in check_precise_math(td).items()}))
         print("loop with continue: wrong results at", check_loop_control(td), "of 256 pixels; flag form exact")
         check_min_max(td)
+        check_mix_normalize(td)
+        check_camera_tan(td, names)
         for n in names:
             make(n, td)
 

@@ -187,6 +187,12 @@ void CONSTRUCT_PRIMARY_RAY(in vec2 pixel, inout vec4 rnd, inout vec3 primaryStar
     s.oracle_sdfs = []
 
 
+def _three_bounces(s):
+    """the Mandelbox glass at maxBounces 3: few enough internal bounces that
+    every path stays the reference's, so it is compared per pixel"""
+    s.renderer.maxBounces = 3
+
+
 def _ao(s):
     """renderMode 'ao' (shaders/ao-fragment-shader.glsl): sky and sun occlusion,
     the interactive skip"""
@@ -230,6 +236,8 @@ CASES = {
     "menger_metal": dict(builder="menger_metal", kwargs=dict(width=48, height=48, spp=4), frames=4, seed=1234),
     "mandelbox_dielectric": dict(builder="mandelbox_dielectric", kwargs=dict(width=48, height=36, spp=4), frames=4,
                                  seed=99),
+    "mandelbox_dielectric_b3": dict(builder="mandelbox_dielectric", kwargs=dict(width=64, height=48, spp=8),
+                                    frames=8, seed=98, tweak=_three_bounces),
     "sun_specular": dict(builder="sphere_plane", kwargs=dict(width=48, height=48, spp=8), frames=8, seed=5,
                          tweak=_sun_and_specular),
     "tabulated_metal": dict(builder="sphere_plane", kwargs=dict(width=40, height=32, spp=4), frames=4, seed=11,

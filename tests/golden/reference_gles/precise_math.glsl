@@ -4,11 +4,12 @@
 // puts this block ahead of the reference's shader code and maps the builtins
 // onto it with #defines: GLSL leaves builtin precision to the implementation,
 // and the one used for fixtures (SwiftShader 4.1) has ~1e-3 relative error in
-// sin/cos/tan inside large programs.
+// sin/cos/tan inside large programs.  mix and normalize follow their
+// definitions, and min, max and clamp theirs, for the same reason.
 //
 // Algorithms and coefficients: the Cephes single-precision library by
-// S. L. Moshier (sinf.c, expf.c, logf.c, atanf.c, asinf.c): Cody-Waite range
-// reduction and minimax polynomials, about 1-2 ulp on the reduced ranges.
+// S. L. Moshier (sinf.c, tanf.c, expf.c, logf.c, atanf.c, asinf.c): Cody-Waite
+// range reduction and minimax polynomials, about 1-2 ulp on the reduced ranges.
 
 float snl_sinp(float x) {
   float z = x * x;
@@ -44,7 +45,17 @@ float snl_cos(float x) {
   float v = (j == 1 || j == 2) ? snl_sinp(r) : snl_cosp(r);
   return s * v;
 }
-float snl_tan(float x) { return snl_sin(x) / snl_cos(x); }
+// Cephes tanf: tan(r) = r + r^3 P(r^2) on |r| <= pi/4, and -1/tan on odd
+// octant pairs (about 1 ulp; sin/cos would round twice more)
+float snl_tan(float x) {
+  int j;
+  float r = snl_reduce(x, j);
+  float z = r * r;
+  float y = (((((9.38540185543e-3 * z + 3.11992232697e-3) * z + 2.44301354525e-2) * z + 5.34112807005e-2) * z
+               + 1.33387994085e-1) * z + 3.33331568548e-1) * z * r + r;
+  if ((j & 2) == 2) y = -1.0 / y;
+  return x < 0.0 ? -y : y;
+}
 
 float snl_pow2i(int n) { return intBitsToFloat((clamp(n, -126, 127) + 127) << 23); }
 float snl_exp(float x) {
@@ -153,6 +164,37 @@ SNL_MINMAX(vec3, bvec3, float)
 SNL_MINMAX(vec4, bvec4, float)
 #undef SNL_MINMAX
 
+// mix as GLSL ES 3.00 defines it (section 8.3), x * (1 - a) + y * a, and
+// normalize as x / length(x) divided per component.  SwiftShader 4.1 computes
+// mix as x + a * (y - x), which differs in the last bit for about half of all
+// operands, and normalize by a multiply with an inverse square root (1-2 ulp):
+// those last bits move camera-ray hit points, and the normals pass's central
+// differences (eps = 2e-4) magnify them to ~1e-4.  The boolean-select forms
+// of mix are exact and keep the builtin.
+float snl_mix(float x, float y, float a) { return x * (1.0 - a) + y * a; }
+vec2 snl_mix(vec2 x, vec2 y, float a) { return x * (1.0 - a) + y * a; }
+vec3 snl_mix(vec3 x, vec3 y, float a) { return x * (1.0 - a) + y * a; }
+vec4 snl_mix(vec4 x, vec4 y, float a) { return x * (1.0 - a) + y * a; }
+vec2 snl_mix(vec2 x, vec2 y, vec2 a) { return x * (1.0 - a) + y * a; }
+vec3 snl_mix(vec3 x, vec3 y, vec3 a) { return x * (1.0 - a) + y * a; }
+vec4 snl_mix(vec4 x, vec4 y, vec4 a) { return x * (1.0 - a) + y * a; }
+float snl_mix(float x, float y, bool a) { return mix(x, y, a); }
+vec2 snl_mix(vec2 x, vec2 y, bvec2 a) { return mix(x, y, a); }
+vec3 snl_mix(vec3 x, vec3 y, bvec3 a) { return mix(x, y, a); }
+vec4 snl_mix(vec4 x, vec4 y, bvec4 a) { return mix(x, y, a); }
+float snl_normalize(float x) { return x / abs(x); }
+vec2 snl_normalize(vec2 x) { float l = sqrt(x.x * x.x + x.y * x.y); return vec2(x.x / l, x.y / l); }
+vec3 snl_normalize(vec3 x) {
+  float l = sqrt(x.x * x.x + x.y * x.y + x.z * x.z);
+  return vec3(x.x / l, x.y / l, x.z / l);
+}
+vec4 snl_normalize(vec4 x) {
+  float l = sqrt(x.x * x.x + x.y * x.y + x.z * x.z + x.w * x.w);
+  return vec4(x.x / l, x.y / l, x.z / l, x.w / l);
+}
+
+#define mix snl_mix
+#define normalize snl_normalize
 #define min snl_min
 #define max snl_max
 #define clamp snl_clamp

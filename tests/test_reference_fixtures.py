@@ -10,23 +10,29 @@ The oracle (CPU) and the HIP path (GPU) are held to the same bars:
   * the sample count channel is identical;
   * the per-pixel RNG states after all frames are bit-identical for >= 99% of
     pixels -- every random decision of those paths was the same;
-  * per pixel, radiance within 1e-2 relative for >= 99% of pixels, within 1e-3
-    for >= 97% and within 1e-4 for >= 85%.  Central-difference normals
-    (eps = 2e-4) turn last-ulp SDF differences into ~1e-4 relative noise in the
-    shading, and near-specular lobes (roughness 0.02) at grazing incidence
-    amplify direction differences of an ulp or two into ~0.5% (the BK7 sphere's
+  * per pixel, radiance within 1e-3 relative for >= 99.5% of pixels and within
+    1e-4 for >= 98.5%.  What remains are last-ulp differences of the
+    transcendentals (the fixture's are Cephes', 1-2 ulp) that near-specular
+    lobes (roughness 0.02) at grazing incidence amplify (the BK7 sphere's
     silhouette);
-  * first hits: pick distance within 1e-4 relative and the same material for
-    >= 99.5% of pixels; the normals pass within 1e-3 for >= 99.5%.
-The Mandelbox dielectric (16 bounces through a fractal) is chaotic: paths
-separate after a few internal bounces, so its radiance is compared as
-statistics (mean per channel within 4 standard errors) and its first hits per
-pixel.
-The environment-map case holds every bar but the 1e-4 one: an SRGB8 texture's
-decode and filter precision are the implementation's (the GLES implementation
-that wrote the fixture decodes texel centres to within 1.4e-3 relative and
-filters to within a median 2e-4 of exact float bilinear filtering of the
-exactly decoded texels, which is what the HIP path and the oracle compute).
+  * first hits: the pick distance bit-identical and the same material for
+    >= 99.5% of pixels (within 1e-4 relative for all of them); the normals
+    pass within 1e-4 for >= 99.5% of pixels.  This needs the camera rays to be
+    bit-identical: the fixture's tan, normalize and mix follow their
+    definitions (the frustum height tan(fovy/2) correctly rounded, normalize
+    as x / length(x), mix as x (1 - a) + y a), because central-difference
+    normals (eps = 2e-4) turn one ulp of a hit point into ~1e-4.
+The Mandelbox dielectric at 16 bounces is chaotic: paths separate after a few
+internal bounces, so its radiance is compared as statistics (mean per channel
+within 4 standard errors) beside floors on the per-pixel agreement; at
+maxBounces 3 (mandelbox_dielectric_b3) it meets the per-pixel bars.
+The environment-map case holds every bar but the 1e-4 ones on pixels that see
+the map: an SRGB8 texture's decode and filter precision are the
+implementation's (the GLES implementation that wrote the fixture decodes texel
+centres to within 1.4e-3 relative and filters to within a median 2e-4 of exact
+float bilinear filtering of the exactly decoded texels, which is what the HIP
+path and the oracle compute); its radiance is held to 1e-3 and its normals to
+1e-4 on the pixels whose first ray hits the sphere.
 """
 import json
 import math
@@ -60,31 +66,37 @@ def rel_close(a, b, tol):
 def check_pathtrace(name, rad, rng, ref):
     """the bars of the module docstring for one render against its fixture"""
     assert np.array_equal(rad[:, 3], ref["radiance"][:, 3])
+    same_rng = np.all(rng == ref["rng"], axis=1).mean()
+    c3, c4 = (rel_close(rad, ref["radiance"], t).mean() for t in (1e-3, 1e-4))
     if name in CHAOTIC:
         d = rad[:, :3].astype(np.float64) - ref["radiance"][:, :3]
         se = d.std(axis=0) / math.sqrt(d.shape[0])
         assert np.all(np.abs(d.mean(axis=0)) <= 4.0 * se + 1e-6), (d.mean(axis=0), se)
+        assert same_rng >= 0.85 and c4 >= 0.95, (same_rng, c4)
         return
-    same_rng = np.all(rng == ref["rng"], axis=1).mean()
-    c2, c3, c4 = (rel_close(rad, ref["radiance"], t).mean() for t in (1e-2, 1e-3, 1e-4))
     assert same_rng >= 0.99, same_rng
-    assert c2 >= 0.99 and c3 >= 0.97, (c2, c3, c4)
+    assert c3 >= (0.99 if name in TEXTURE_PRECISION else 0.995), (c3, c4)
     if name not in TEXTURE_PRECISION:
-        assert c4 >= 0.85, (c2, c3, c4)
+        assert c4 >= 0.985, (c3, c4)
 
 
 def check_pick(pk, ref):
     rp = ref["pick"]
     same = pk[:, 1] == rp[:, 1]
     assert same.mean() >= 0.995
+    assert np.mean(same & (pk[:, 0] == rp[:, 0])) >= 0.995
     rel = np.abs(pk[same, 0] - rp[same, 0]) / np.abs(rp[same, 0])
-    assert np.mean(rel <= 1e-4) >= 0.995
+    assert np.all(rel <= 1e-4)
 
 
 def check_normals(name, nrad, nrng, ref):
     assert np.array_equal(nrng, ref["normals_rng"])
-    tol, frac = (1e-2, 0.95) if name in CHAOTIC else (1e-3, 0.995)
-    assert rel_close(nrad, ref["normals"], tol).mean() >= frac
+    if name in TEXTURE_PRECISION:
+        hit = ref["pick"][:, 1] >= 0
+        assert rel_close(nrad, ref["normals"], 1e-3).mean() >= 0.995
+        assert rel_close(nrad[hit], ref["normals"][hit], 1e-4).mean() >= 0.98
+    else:
+        assert rel_close(nrad, ref["normals"], 1e-4).mean() >= 0.995
 
 
 def pixel_grid(w, h):

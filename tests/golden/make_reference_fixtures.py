@@ -17,6 +17,7 @@ data (initial state is implied by the seed; outputs are the
 accumulated radiance and the final RNG state).
 
     python tests/golden/make_reference_fixtures.py [case ...]
+    python tests/golden/make_reference_fixtures.py --time   # the reference's CPU rate
 """
 import json
 import math
@@ -468,8 +469,34 @@ def make(name, td):
     print(json.dumps(meta))
 
 
+def time_plain_case(td, W=256, H=256, frames=16):
+    """the reference's pathtracer on this container's CPUs through the GLES
+    implementation, on the CPU-runnable case (unit sphere over a plane, 256 x
+    256, 16 spp, 4 bounces): one frame and 1 + `frames` frames are timed, and
+    their difference, which leaves out program compilation and process start,
+    gives the per-frame rate; the children's CPU time over the wall time gives
+    the cores the implementation kept busy"""
+    import resource
+    from snelly_amd import scenes
+    out = {}
+    for n in (1, 1 + frames):
+        c0 = resource.getrusage(resource.RUSAGE_CHILDREN)
+        _, _, gl, secs = run_reference(scenes.sphere_plane(W, H, frames), W, H, n, 1234, td, "timing")
+        c1 = resource.getrusage(resource.RUSAGE_CHILDREN)
+        out[n] = (secs, (c1.ru_utime - c0.ru_utime) + (c1.ru_stime - c0.ru_stime))
+    wall = out[1 + frames][0] - out[1][0]
+    cpu = out[1 + frames][1] - out[1][1]
+    return dict(workload=f"sphere over plane {W}x{H}, {frames} spp, 4 bounces", gl=gl, seconds_per_frame=wall / frames,
+                path_samples_per_s=W * H * frames / wall, cores_busy=round(cpu / wall, 2),
+                nproc=os.cpu_count())
+
+
 def main():
     subprocess.run(["make", "-s", "-C", os.path.join(HERE, "reference_gles")], check=True)
+    if sys.argv[1:] == ["--time"]:
+        with tempfile.TemporaryDirectory() as td:
+            print(json.dumps(time_plain_case(td)))
+        return
     names = sys.argv[1:] or list(reference_cases.CASES)
     with tempfile.TemporaryDirectory() as td:
         print("precise_math ulp:", json.dumps({k: round(v, 2) for k, v in check_precise_math(td).items()}))

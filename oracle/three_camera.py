@@ -18,8 +18,17 @@ that chain runs through, in doubles as JavaScript computes them:
   Vector3.applyQuaternion          three.js:2240-2260
   Vector3.transformDirection       three.js:2292-2306    camUp = up.transformDirection(matrixWorld)
   Vector3.normalize / divideScalar three.js:2460-2464, 2318-2322
+
+Matrix4.elements is a Float32Array in r79 (three.js:4806-4815): every matrix
+write rounds to float32 (`_store`), while vectors and quaternions stay doubles.
+camera_uniforms(..., float32_matrices=False) gives the all-doubles variant, which
+tests use to show that the rounding is visible in the uniforms.
 """
 import math
+
+import numpy as np
+
+_store = [lambda v: float(np.float32(v))]
 
 
 def _length(v):
@@ -45,10 +54,11 @@ def _look_at(eye, target, up):
         z[2] += 0.0001
         x = _normalize(_cross(up, z))
     y = _cross(z, x)
+    st = _store[0]
     te = [1.0, 0, 0, 0, 0, 1.0, 0, 0, 0, 0, 1.0, 0, 0, 0, 0, 1.0]
-    te[0], te[4], te[8] = x[0], y[0], z[0]
-    te[1], te[5], te[9] = x[1], y[1], z[1]
-    te[2], te[6], te[10] = x[2], y[2], z[2]
+    te[0], te[4], te[8] = st(x[0]), st(y[0]), st(z[0])
+    te[1], te[5], te[9] = st(x[1]), st(y[1]), st(z[1])
+    te[2], te[6], te[10] = st(x[2]), st(y[2]), st(z[2])
     return te
 
 
@@ -76,13 +86,14 @@ def _compose(position, q):
     xx, xy, xz = x * x2, x * y2, x * z2
     yy, yz, zz = y * y2, y * z2, z * z2
     wx, wy, wz = w * x2, w * y2, w * z2
+    st = _store[0]
     te = [0.0] * 16
-    te[0] = 1 - (yy + zz); te[4] = xy - wz; te[8] = xz + wy
-    te[1] = xy + wz; te[5] = 1 - (xx + zz); te[9] = yz - wx
-    te[2] = xz - wy; te[6] = yz + wx; te[10] = 1 - (xx + yy)
+    te[0] = st(1 - (yy + zz)); te[4] = st(xy - wz); te[8] = st(xz + wy)
+    te[1] = st(xy + wz); te[5] = st(1 - (xx + zz)); te[9] = st(yz - wx)
+    te[2] = st(xz - wy); te[6] = st(yz + wx); te[10] = st(1 - (xx + yy))
     te[15] = 1.0
     # scale (1, 1, 1) multiplies by one: no change
-    te[12], te[13], te[14] = position
+    te[12], te[13], te[14] = [st(c) for c in position]
     return te
 
 
@@ -97,11 +108,12 @@ def _decompose_quaternion(te):
     sz = _length([te[8], te[9], te[10]])
     if _determinant3(te) < 0:
         sx = -sx
+    st = _store[0]
     m = list(te)
     for k, inv in ((0, 1 / sx), (4, 1 / sy), (8, 1 / sz)):
-        m[k] *= inv
-        m[k + 1] *= inv
-        m[k + 2] *= inv
+        m[k] = st(m[k] * inv)
+        m[k + 1] = st(m[k + 1] * inv)
+        m[k + 2] = st(m[k + 2] * inv)
     return _quaternion_from_rotation(m)
 
 
@@ -123,9 +135,10 @@ def _transform_direction(v, te):
                        te[2] * x + te[6] * y + te[10] * z])
 
 
-def camera_uniforms(position, target, up):
+def camera_uniforms(position, target, up, float32_matrices=True):
     """(camPos, camDir, camX, camY) as js/renderer.js:520-527 computes them for
     a camera at `position` looking at `target` with `up`"""
+    _store[0] = (lambda v: float(np.float32(v))) if float32_matrices else float
     position, target, up = [float(c) for c in position], [float(c) for c in target], [float(c) for c in up]
     q = _quaternion_from_rotation(_look_at(position, target, up))
     world = _compose(position, q)

@@ -476,35 +476,160 @@ class Camera:
         self.aperture = -10.0
         self.focalDistance = 1.0
 
-    def basis(self):
-        """camDir / camX / camY as js/renderer.js:520-527 derives them (in
-        doubles, as JavaScript does).  The camera's rotation R = [x y z] is the
-        lookAt frame of three.js r79 (js/thirdparty/three/three.js:5105-5150):
-        z = normalize(pos - target), x = normalize(up x z), y = z x x.
-        camDir = getWorldDirection() = R (0, 0, -1) = -z; camY is `up` carried
-        through R and normalised (Vector3.transformDirection, 2292-2306), which
-        is y only for up = (0, 1, 0); camX = camY x camDir."""
-        def unit(v):
-            # Vector3.normalize (2460-2464): times 1 / length, NaN for a zero
-            # vector -- so three.js's zero-length fallbacks below never fire
-            n = math.sqrt(float(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]))
-            return v * (1.0 / n if n > 0.0 else math.inf)
-        p = np.asarray(self.position, np.float64)
-        t = np.asarray(self.target, np.float64)
-        up = np.asarray(self.up, np.float64)
-        with np.errstate(invalid="ignore"):
-            z = unit(p - t)
-            if float(z @ z) == 0.0:
-                z[2] = 1.0
-            x = unit(np.cross(up, z))
-            if float(x @ x) == 0.0:
-                z[2] += 0.0001
-                x = unit(np.cross(up, z))
-        y = np.cross(z, x)
-        camDir = -z
-        camY = unit(up[0] * x + up[1] * y + up[2] * z)
-        camX = np.cross(camY, camDir)
-        return p, camDir, camX, camY
+    def controls_position(self):
+        """The position the reference renders its first frame from.  Its
+        OrbitControls.update (js/thirdparty/three/controls/OrbitControls.js:
+        269-321) re-derives the camera position from spherical coordinates
+        about the target (polar angle kept within [EPS, pi - EPS], EPS = 1e-6),
+        and a page runs it twice before that frame: in Snelly._resize (or the
+        constructor's Snelly.resize) and at the end of Snelly.initScene
+        (js/snelly.js:80, 274-279, 291, 605).  The controls' up-space rotation
+        is the identity: they are made while camera.up is still (0, 1, 0)."""
+        eps = 0.000001
+        p = [float(x) for x in self.position]
+        t = [float(x) + 0.0 for x in self.target]  # target.add(pan) with pan = 0
+        for _ in range(2):
+            ox, oy, oz = p[0] - t[0], p[1] - t[1], p[2] - t[2]
+            theta = math.atan2(ox, oz)
+            phi = math.atan2(math.sqrt(ox * ox + oz * oz), oy)
+            phi = max(eps, min(math.pi - eps, max(0.0, min(math.pi, phi))))
+            radius = math.sqrt(ox * ox + oy * oy + oz * oz)
+            p = [t[0] + radius * math.sin(phi) * math.sin(theta), t[1] + radius * math.cos(phi),
+                 t[2] + radius * math.sin(phi) * math.cos(theta)]
+        return p
+
+    def basis(self, position=None):
+        """(camPos, camDir, camX, camY) as js/renderer.js:520-527 derives them
+        from the THREE.PerspectiveCamera that OrbitControls points at its
+        target, through three.js r79's chain (js/thirdparty/three/three.js):
+        Camera.lookAt sets the quaternion from Matrix4.lookAt's rotation;
+        getWorldDirection rebuilds the world matrix from it (compose), takes the
+        quaternion back out (decompose) and turns (0, 0, -1) by it; camY is `up`
+        carried through the world matrix and normalised (transformDirection),
+        camX = camY x camDir.  Vector arithmetic is in doubles, as JavaScript
+        computes it, but every Matrix4 element is stored in a Float32Array, so
+        each matrix write rounds to float32."""
+        p = [float(c) for c in (self.position if position is None else position)]
+        t = [float(c) for c in self.target]
+        up = [float(c) for c in self.up]
+        q = _three_quaternion_from_rotation(_three_look_at(p, t, up))
+        world = _three_compose(q)
+        cam_dir = _three_apply_quaternion([0.0, 0.0, -1.0], _three_decompose_quaternion(world))
+        cam_y = _three_transform_direction(up, world)
+        cam_x = _three_cross(cam_y, cam_dir)
+        return (np.array(p, np.float64), np.array(cam_dir, np.float64), np.array(cam_x, np.float64),
+                np.array(cam_y, np.float64))
+
+
+# three.js r79's camera chain (js/thirdparty/three/three.js).  Matrix4.elements
+# is a Float32Array: `_f32` is a store into it.
+def _f32(v):
+    return float(np.float32(v))
+
+
+def _three_normalize(v):
+    """Vector3.normalize (2460-2464): times 1 / length (NaN for a zero vector)"""
+    n = math.sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])
+    with np.errstate(divide="ignore", invalid="ignore"):
+        s = 1.0 / n if n != 0.0 else math.inf
+    return [c * s if not (math.isinf(s) and c == 0.0) else math.nan for c in v]
+
+
+def _three_cross(a, b):
+    """Vector3.crossVectors (2393-2402)"""
+    return [a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]]
+
+
+def _three_look_at(eye, target, up):
+    """Matrix4.lookAt (5105-5150): the rotation part of a fresh matrix"""
+    z = _three_normalize([eye[k] - target[k] for k in range(3)])
+    if z[0] * z[0] + z[1] * z[1] + z[2] * z[2] == 0:
+        z[2] = 1.0
+    x = _three_normalize(_three_cross(up, z))
+    if x[0] * x[0] + x[1] * x[1] + x[2] * x[2] == 0:
+        z[2] += 0.0001
+        x = _three_normalize(_three_cross(up, z))
+    y = _three_cross(z, x)
+    te = [1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 1.0]
+    te[0], te[4], te[8] = _f32(x[0]), _f32(y[0]), _f32(z[0])
+    te[1], te[5], te[9] = _f32(x[1]), _f32(y[1]), _f32(z[1])
+    te[2], te[6], te[10] = _f32(x[2]), _f32(y[2]), _f32(z[2])
+    return te
+
+
+def _three_quaternion_from_rotation(te):
+    """Quaternion.setFromRotationMatrix (1048-1105)"""
+    m11, m12, m13 = te[0], te[4], te[8]
+    m21, m22, m23 = te[1], te[5], te[9]
+    m31, m32, m33 = te[2], te[6], te[10]
+    trace = m11 + m22 + m33
+    if trace > 0:
+        s = 0.5 / math.sqrt(trace + 1.0)
+        return [(m32 - m23) * s, (m13 - m31) * s, (m21 - m12) * s, 0.25 / s]
+    if m11 > m22 and m11 > m33:
+        s = 2.0 * math.sqrt(1.0 + m11 - m22 - m33)
+        return [0.25 * s, (m12 + m21) / s, (m13 + m31) / s, (m32 - m23) / s]
+    if m22 > m33:
+        s = 2.0 * math.sqrt(1.0 + m22 - m11 - m33)
+        return [(m12 + m21) / s, 0.25 * s, (m23 + m32) / s, (m13 - m31) / s]
+    s = 2.0 * math.sqrt(1.0 + m33 - m11 - m22)
+    return [(m13 + m31) / s, (m23 + m32) / s, 0.25 * s, (m21 - m12) / s]
+
+
+def _three_compose(q):
+    """Object3D.updateMatrix (8999-9005) = Matrix4.compose (5585-5593):
+    makeRotationFromQuaternion (5068-5100), scale by (1, 1, 1), setPosition.
+    The translation column is not read by the chain."""
+    x, y, z, w = q
+    x2, y2, z2 = x + x, y + y, z + z
+    xx, xy, xz = x * x2, x * y2, x * z2
+    yy, yz, zz = y * y2, y * z2, z * z2
+    wx, wy, wz = w * x2, w * y2, w * z2
+    te = [0.0] * 16
+    te[0], te[4], te[8] = _f32(1 - (yy + zz)), _f32(xy - wz), _f32(xz + wy)
+    te[1], te[5], te[9] = _f32(xy + wz), _f32(1 - (xx + zz)), _f32(yz - wx)
+    te[2], te[6], te[10] = _f32(xz - wy), _f32(yz + wx), _f32(1 - (xx + yy))
+    te[15] = 1.0
+    return te
+
+
+def _three_decompose_quaternion(te):
+    """the quaternion of Matrix4.decompose (5595-5650); the sign test is the
+    n44 term of Matrix4.determinant (5290-5336), the others being zero"""
+    sx = math.sqrt(te[0] * te[0] + te[1] * te[1] + te[2] * te[2])
+    sy = math.sqrt(te[4] * te[4] + te[5] * te[5] + te[6] * te[6])
+    sz = math.sqrt(te[8] * te[8] + te[9] * te[9] + te[10] * te[10])
+    n11, n12, n13 = te[0], te[4], te[8]
+    n21, n22, n23 = te[1], te[5], te[9]
+    n31, n32, n33 = te[2], te[6], te[10]
+    det = (-n13 * n22 * n31 - n11 * n23 * n32 + n11 * n22 * n33 + n13 * n21 * n32 - n12 * n21 * n33
+           + n12 * n23 * n31)
+    if det < 0:
+        sx = -sx
+    m = list(te)
+    for k, inv in ((0, 1 / sx), (4, 1 / sy), (8, 1 / sz)):
+        m[k], m[k + 1], m[k + 2] = _f32(m[k] * inv), _f32(m[k + 1] * inv), _f32(m[k + 2] * inv)
+    return _three_quaternion_from_rotation(m)
+
+
+def _three_apply_quaternion(v, q):
+    """Vector3.applyQuaternion (2240-2260)"""
+    x, y, z = v
+    qx, qy, qz, qw = q
+    ix = qw * x + qy * z - qz * y
+    iy = qw * y + qz * x - qx * z
+    iz = qw * z + qx * y - qy * x
+    iw = -qx * x - qy * y - qz * z
+    return [ix * qw + iw * -qx + iy * -qz - iz * -qy,
+            iy * qw + iw * -qy + iz * -qx - ix * -qz,
+            iz * qw + iw * -qz + ix * -qy - iy * -qx]
+
+
+def _three_transform_direction(v, te):
+    """Vector3.transformDirection (2292-2306)"""
+    x, y, z = v
+    return _three_normalize([te[0] * x + te[4] * y + te[8] * z, te[1] * x + te[5] * y + te[9] * z,
+                             te[2] * x + te[6] * y + te[10] * z])
 
 
 # ---------------------------------------------------------------- renderer
@@ -657,7 +782,7 @@ def assemble(scene_glsl, r, mats):
 def frame_uniforms(r, cam, s, mats, width, height):
     """The uniform values Renderer.render uploads each frame
     (js/renderer.js:517-584 and Materials.syncShader, js/materials.js:1335-1341)."""
-    pos, camDir, camX, camY = cam.basis()
+    pos, camDir, camX, camY = cam.basis(cam.controls_position())
     ls = s.lengthScale
     u = {
         "camPos": pos, "camDir": camDir, "camX": camX, "camY": camY,

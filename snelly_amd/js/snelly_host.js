@@ -36,6 +36,7 @@ class Vector3 {
   crossVectors(a, b) {
     return this.set(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
   }
+  cross(v) { return this.crossVectors(this.clone(), v); }
   toArray() { return [this.x, this.y, this.z]; }
 }
 
@@ -44,48 +45,139 @@ class PerspectiveCamera {
     this.fov = fov; this.aspect = aspect; this.near = near; this.far = far;
     this.position = new Vector3(0, 0, 0);
     this.up = new Vector3(0, 1, 0);
-    this.target = new Vector3(0, 0, 0);
+    // the orientation, [x, y, z, w]: set by lookAt only (as Object3D.quaternion),
+    // so moving the camera or its up vector afterwards keeps the old rotation
+    this.quaternion = [0, 0, 0, 1];
     this.aperture = -10.0;
     this.focalDistance = 1.0;
     this.auto_focus = false;
   }
-  lookAt(v) { this.target.copy(v); }
+  // Camera.lookAt (js/thirdparty/three/three.js:17025-17036)
+  lookAt(v) { this.quaternion = threeQuaternionFromRotation(threeLookAt(this.position, v, this.up)); }
   updateProjectionMatrix() {}
-  // The camera's rotation R = [x y z] is three.js r79's lookAt frame
-  // (js/thirdparty/three/three.js:5105-5150): z = normalize(pos - target),
-  // x = normalize(up x z), y = z x x.  The renderer (js/renderer.js:520-527) takes
-  // camDir = R (0, 0, -1) = -z, camY = `up` carried through R and normalised
-  // (Vector3.transformDirection), which is y only for up = (0, 1, 0), and
-  // camX = camY x camDir.
+  // (camPos, camDir, camX, camY) as js/renderer.js:520-527 derives them,
+  // through three.js r79's chain (js/thirdparty/three/three.js):
+  // getWorldDirection rebuilds the world matrix from the quaternion that the
+  // last lookAt set (compose), takes the quaternion back out (decompose) and
+  // turns (0, 0, -1) by it; camY is the current `up` carried through the world
+  // matrix and normalised (transformDirection); camX = camY x camDir.
+  // Matrix4.elements is a Float32Array, so every matrix write rounds to float32.
   basis() {
-    // Vector3.normalize of three.js r79: times 1 / length, NaN for a zero vector
-    // (so its zero-length fallbacks in lookAt never fire)
-    const unit = (v) => v.multiplyScalar(1 / v.length());
-    const z = unit(this.position.clone().sub(this.target));
-    if (z.dot(z) === 0) z.z = 1;
-    let x = unit(new Vector3().crossVectors(this.up, z));
-    if (x.dot(x) === 0) {
-      z.z += 0.0001;
-      x = unit(new Vector3().crossVectors(this.up, z));
-    }
-    const y = new Vector3().crossVectors(z, x);
-    const dir = z.clone().multiplyScalar(-1);
-    const u = this.up;
-    const camY = unit(new Vector3(u.x * x.x + u.y * y.x + u.z * z.x, u.x * x.y + u.y * y.y + u.z * z.y,
-                                  u.x * x.z + u.y * y.z + u.z * z.z));
+    const world = threeCompose(this.quaternion);
+    const dir = threeApplyQuaternion(new Vector3(0, 0, -1), threeDecomposeQuaternion(world));
+    const camY = threeTransformDirection(this.up.clone(), world);
     const camX = new Vector3().crossVectors(camY, dir);
     return { pos: this.position.clone(), dir, camX, camY };
   }
   getWorldDirection() { return this.basis().dir; }
 }
 
+// three.js r79's camera chain, Matrix4 elements as Float32Array
+// Vector3.normalize (2460-2464): times 1 / length, NaN for a zero vector
+function threeNormalize(v) { return v.multiplyScalar(1 / v.length()); }
+// Matrix4.lookAt (5105-5150) into a fresh matrix
+function threeLookAt(eye, target, up) {
+  const te = Float32Array.of(1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1);
+  const z = threeNormalize(eye.clone().sub(target));
+  if (z.dot(z) === 0) z.z = 1;
+  let x = threeNormalize(new Vector3().crossVectors(up, z));
+  if (x.dot(x) === 0) { z.z += 0.0001; x = threeNormalize(new Vector3().crossVectors(up, z)); }
+  const y = new Vector3().crossVectors(z, x);
+  te[0] = x.x; te[4] = y.x; te[8] = z.x;
+  te[1] = x.y; te[5] = y.y; te[9] = z.y;
+  te[2] = x.z; te[6] = y.z; te[10] = z.z;
+  return te;
+}
+// Quaternion.setFromRotationMatrix (1048-1105): [x, y, z, w]
+function threeQuaternionFromRotation(te) {
+  const m11 = te[0], m12 = te[4], m13 = te[8], m21 = te[1], m22 = te[5], m23 = te[9];
+  const m31 = te[2], m32 = te[6], m33 = te[10];
+  const trace = m11 + m22 + m33;
+  let s;
+  if (trace > 0) {
+    s = 0.5 / Math.sqrt(trace + 1.0);
+    return [(m32 - m23) * s, (m13 - m31) * s, (m21 - m12) * s, 0.25 / s];
+  } else if (m11 > m22 && m11 > m33) {
+    s = 2.0 * Math.sqrt(1.0 + m11 - m22 - m33);
+    return [0.25 * s, (m12 + m21) / s, (m13 + m31) / s, (m32 - m23) / s];
+  } else if (m22 > m33) {
+    s = 2.0 * Math.sqrt(1.0 + m22 - m11 - m33);
+    return [(m12 + m21) / s, 0.25 * s, (m23 + m32) / s, (m13 - m31) / s];
+  }
+  s = 2.0 * Math.sqrt(1.0 + m33 - m11 - m22);
+  return [(m13 + m31) / s, (m23 + m32) / s, 0.25 * s, (m21 - m12) / s];
+}
+// Object3D.updateMatrix = Matrix4.compose (5585-5593): makeRotationFromQuaternion
+// (5068-5100), scale (1, 1, 1); the translation column is not read below
+function threeCompose(q) {
+  const [x, y, z, w] = q;
+  const x2 = x + x, y2 = y + y, z2 = z + z;
+  const xx = x * x2, xy = x * y2, xz = x * z2, yy = y * y2, yz = y * z2, zz = z * z2;
+  const wx = w * x2, wy = w * y2, wz = w * z2;
+  const te = new Float32Array(16);
+  te[0] = 1 - (yy + zz); te[4] = xy - wz; te[8] = xz + wy;
+  te[1] = xy + wz; te[5] = 1 - (xx + zz); te[9] = yz - wx;
+  te[2] = xz - wy; te[6] = yz + wx; te[10] = 1 - (xx + yy);
+  te[15] = 1;
+  return te;
+}
+// the quaternion of Matrix4.decompose (5595-5650); the determinant's sign
+// (5290-5336) is its n44 term, the others being zero
+function threeDecomposeQuaternion(te) {
+  let sx = Math.sqrt(te[0] * te[0] + te[1] * te[1] + te[2] * te[2]);
+  const sy = Math.sqrt(te[4] * te[4] + te[5] * te[5] + te[6] * te[6]);
+  const sz = Math.sqrt(te[8] * te[8] + te[9] * te[9] + te[10] * te[10]);
+  const n11 = te[0], n12 = te[4], n13 = te[8], n21 = te[1], n22 = te[5], n23 = te[9];
+  const n31 = te[2], n32 = te[6], n33 = te[10];
+  const det = -n13 * n22 * n31 - n11 * n23 * n32 + n11 * n22 * n33 + n13 * n21 * n32 - n12 * n21 * n33
+    + n12 * n23 * n31;
+  if (det < 0) sx = -sx;
+  const m = Float32Array.from(te);
+  const inv = [1 / sx, 1 / sy, 1 / sz];
+  for (let c = 0; c < 3; ++c) for (let r = 0; r < 3; ++r) m[4 * c + r] *= inv[c];
+  return threeQuaternionFromRotation(m);
+}
+// Vector3.applyQuaternion (2240-2260)
+function threeApplyQuaternion(v, q) {
+  const x = v.x, y = v.y, z = v.z, [qx, qy, qz, qw] = q;
+  const ix = qw * x + qy * z - qz * y, iy = qw * y + qz * x - qx * z;
+  const iz = qw * z + qx * y - qy * x, iw = -qx * x - qy * y - qz * z;
+  return v.set(ix * qw + iw * -qx + iy * -qz - iz * -qy, iy * qw + iw * -qy + iz * -qx - ix * -qz,
+               iz * qw + iw * -qz + ix * -qy - iy * -qx);
+}
+// Vector3.transformDirection (2292-2306)
+function threeTransformDirection(v, te) {
+  const x = v.x, y = v.y, z = v.z;
+  return threeNormalize(v.set(te[0] * x + te[4] * y + te[8] * z, te[1] * x + te[5] * y + te[9] * z,
+                              te[2] * x + te[6] * y + te[10] * z));
+}
+
 class OrbitControls {
   constructor(camera) {
     this.camera = camera;
-    this.target = camera.target;
+    this.target = new Vector3(0, 0, 0);
     this.zoomSpeed = 1.0; this.flySpeed = 0.01; this.keyPanSpeed = 1.0; this.enabled = true;
   }
-  update() { this.camera.target = this.target; }
+  // OrbitControls.update (js/thirdparty/three/controls/OrbitControls.js:269-321)
+  // without user input: the position is re-derived from spherical coordinates
+  // about the target (the polar angle kept within [EPS, pi - EPS]), then the
+  // camera looks at the target.  The controls' up-space rotation is the identity
+  // (they are made while camera.up is still (0, 1, 0)).
+  update() {
+    const EPS = 0.000001;
+    const p = this.camera.position, t = this.target;
+    const ox = p.x - t.x, oy = p.y - t.y, oz = p.z - t.z;
+    const theta = Math.max(-Infinity, Math.min(Infinity, Math.atan2(ox, oz)));
+    let phi = Math.atan2(Math.sqrt(ox * ox + oz * oz), oy);
+    phi = Math.max(0, Math.min(Math.PI, phi));
+    phi = Math.max(EPS, Math.min(Math.PI - EPS, phi));
+    const radius = Math.max(0, Math.min(Infinity, Math.sqrt(ox * ox + oy * oy + oz * oz) * 1));
+    t.set(t.x + 0, t.y + 0, t.z + 0);
+    const sx = radius * Math.sin(phi) * Math.sin(theta), sy = radius * Math.cos(phi);
+    const sz = radius * Math.sin(phi) * Math.cos(theta);
+    p.set(t.x + sx, t.y + sy, t.z + sz);
+    this.camera.lookAt(t);
+  }
   addEventListener() {}
 }
 
@@ -1049,8 +1141,13 @@ class Snelly {
     if (opts.override) opts.override(this);
     if (typeof s.initGui !== 'undefined') s.initGui(this.gui);
     this.materials.getLoadedDielectric().initGui();
-    this.camControls.update();
     const r = this.pathtracer;
+    // the reference's page updates the controls twice before its first frame:
+    // Snelly._resize (for a scene that fixes the size, js/snelly.js:274-279,
+    // 605) or the constructor's Snelly.resize (js/snelly.js:80, 627), and the
+    // end of initScene (js/snelly.js:291)
+    this.camControls.update();
+    this.camControls.update();
     const w = r.width || this.width, h = r.height || this.height;
     this.width = w; this.height = h;
     this.camera.aspect = w / h;

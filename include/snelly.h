@@ -51,6 +51,7 @@ typedef struct snl_ctx_s* snl_ctx;
 typedef struct snl_prog_s* snl_prog;
 typedef struct snl_tex_s* snl_tex;
 typedef struct snl_state_s* snl_state;
+typedef struct snl_batch_s* snl_batch;
 
 /* Context on one GPU: its stream and last-error string.
  * Replaces the WebGL2 context set-up of GLU (js/gl.js:16-29). */
@@ -190,6 +191,34 @@ int snl_set_march_burst(snl_prog prog, int steps);
  * counters. */
 int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, const int* frames,
                      snl_stats* stats);
+
+/* The batch without blocking (continuous batching; SURVEY's
+ * snl_batch_submit / snl_job_poll -- no counterpart in the reference, which
+ * draws one canvas).  snl_batch_submit uploads each program's uniforms,
+ * queues the jobs' launches as snl_render_batch does and returns at once with
+ * a handle.  Jobs of batches submitted while earlier ones run are queued on
+ * the least loaded streams, counting the unfinished jobs of those batches, so
+ * their work overlaps on the device.  Until job j completes, progs[j] and
+ * states[j] are busy: any call that touches them (render, read, write, seed,
+ * error, tonemap, destroy, a later submit) first waits for that job, so results
+ * never depend on the timing; uniform setters only change the host copy, which
+ * the next render or submit uploads.
+ *   snl_job_poll(b, j): 1 when job j has completed, 0 while it runs (j = -1:
+ *     the whole batch), < 0 on an error;
+ *   snl_batch_wait(b, stats): blocks until every job has completed; stats (may
+ *     be NULL) as for snl_render_batch, ms_total from the submit to the last
+ *     job's end;
+ *   snl_batch_job_stats(b, j, out): waits for job j; its time from the submit
+ *     to its end and its counters;
+ *   snl_batch_release(b): waits for the batch and frees the handle (a context
+ *     releases its batches when it is destroyed).
+ * Per job the result is identical to snl_render(progs[j], states[j], frames[j]). */
+int snl_batch_submit(int njobs, const snl_prog* progs, const snl_state* states, const int* frames,
+                     snl_batch* out);
+int snl_job_poll(snl_batch batch, int job);
+int snl_batch_wait(snl_batch batch, snl_stats* stats);
+int snl_batch_job_stats(snl_batch batch, int job, snl_stats* out);
+int snl_batch_release(snl_batch batch);
 
 /* Served rendering support (no counterpart in the reference, which renders one
  * interactive canvas).  A state of K chains holds K independent accumulations

@@ -52,6 +52,7 @@ struct snl_ctx_s {
   std::string err;
   int numCU = 256;
   int* scratch = nullptr;  // this device's copy of the allocation-free render's `scratch` array
+  std::set<snl_batch_s*> batches;  // submitted batches not yet released
 };
 
 struct snl_tex_s {
@@ -76,6 +77,8 @@ struct snl_state_s {
   double* errPartial = nullptr;  // snl_state_error's block partials and result (allocated on first use)
   int* tileCost = nullptr;   // per 8x8 tile: march steps of the render's first frame, then the
                              // cost-bucket histogram and the tile order (ordered megakernel renders)
+  snl_batch_s* busy = nullptr;  // the submitted batch whose job `busyJob` renders into this state
+  int busyJob = -1;
 };
 
 struct snl_prog_s {
@@ -110,6 +113,24 @@ struct snl_prog_s {
   std::vector<int> sampled2D;
   std::set<int> bound;
   std::map<int, snl_tex_s*> texOf;  // sampler uniform -> the texture bound to it (null once destroyed)
+  snl_batch_s* busy = nullptr;  // the submitted batch whose job `busyJob` runs this program
+  int busyJob = -1;
+};
+
+// A submitted batch (snl_batch_submit): its jobs' launches are queued on the
+// context's batch streams; each job's end is an event on its stream.
+struct snl_batch_s {
+  snl_ctx ctx = nullptr;
+  int njobs = 0, pending = 0;
+  std::vector<snl_prog> progs;
+  std::vector<snl_state> states;
+  std::vector<int> frames, stream;
+  std::vector<char> unit0, ordered, done;
+  std::vector<double> work;            // the job's estimated cost (orders jobs onto streams)
+  std::vector<long long> steps, normals;  // the job's counters, read when it is seen complete
+  std::vector<float> ms;               // submit to the job's end (hipEvents)
+  std::vector<hipEvent_t> ev;          // per job: after its last launch
+  hipEvent_t ev0 = nullptr;            // before the first launch
 };
 
 namespace {
@@ -369,7 +390,49 @@ int launch(snl_ctx ctx, hipFunction_t f, int grid, int block, SnlKArgs& a) {
   return SNL_OK;
 }
 
+// ---- submitted batches: a program or state that a job of a submitted batch
+// renders is busy until that job completes; every call that touches its device
+// memory first waits for that job (settle)
+
+// job j of b is complete: read its counters, free its program and state
+int jobDone(snl_batch_s* b, int j) {
+  if (b->done[j]) return SNL_OK;
+  snl_ctx ctx = b->ctx;
+  b->done[j] = 1;
+  --b->pending;
+  float ms = 0.0f;
+  if (hipEventElapsedTime(&ms, b->ev0, b->ev[j]) == hipSuccess) b->ms[j] = ms;
+  HIPCHK(ctx, hipMemcpyAsync(&b->steps[j], b->states[j]->tileCnt + C_TRACE_STEPS, 8, hipMemcpyDeviceToHost,
+                             ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(&b->normals[j], b->states[j]->tileCnt + C_NORMALS, 8, hipMemcpyDeviceToHost,
+                             ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (b->progs[j]->busy == b && b->progs[j]->busyJob == j) b->progs[j]->busy = nullptr;
+  if (b->states[j]->busy == b && b->states[j]->busyJob == j) b->states[j]->busy = nullptr;
+  return SNL_OK;
+}
+int jobWait(snl_batch_s* b, int j) {
+  if (b->done[j]) return SNL_OK;
+  HIPCHK(b->ctx, hipEventSynchronize(b->ev[j]));
+  return jobDone(b, j);
+}
+// 1 complete, 0 still running, < 0 error
+int jobQuery(snl_batch_s* b, int j) {
+  if (b->done[j]) return 1;
+  hipError_t e = hipEventQuery(b->ev[j]);
+  if (e == hipErrorNotReady) return 0;
+  if (e != hipSuccess) return fail(b->ctx, SNL_E_HIP, "job %d of a batch: %s", j, hipGetErrorString(e));
+  int rc = jobDone(b, j);
+  return rc ? rc : 1;
+}
+template <class T>
+int settle(T* o) {
+  return (o && o->busy) ? jobWait(o->busy, o->busyJob) : SNL_OK;
+}
+
 int upload(snl_prog p) {
+  int rc = settle(p);
+  if (rc) return rc;
   if (!p->dirty) return SNL_OK;
   // 0.5 * radians(camFovy) in float, as the shaders form the argument
   const float halfFovy = 0.5f * (p->hostP.camFovy * 0.017453292519943295f);
@@ -479,6 +542,7 @@ int snl_ctx_create(int device, snl_ctx* out) {
 int snl_ctx_destroy(snl_ctx ctx) {
   if (!ctx) return SNL_OK;
   hipSetDevice(ctx->device);
+  while (!ctx->batches.empty()) (void)snl_batch_release(*ctx->batches.begin());
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   for (int k = 0; k < SNL_MAX_TILES; ++k) {
     if (ctx->tileStream[k]) hipStreamDestroy(ctx->tileStream[k]);
@@ -588,6 +652,7 @@ int snl_program_build(snl_ctx ctx, const char* shader, const char* ior, const ch
 int snl_program_destroy(snl_prog p) {
   if (!p) return SNL_OK;
   hipSetDevice(p->ctx->device);
+  (void)settle(p);
   hipStreamSynchronize(p->ctx->stream);
   if (p->dP) hipFree(p->dP);
   if (p->dU) hipFree(p->dU);
@@ -786,6 +851,7 @@ int snl_state_create_chains(snl_ctx ctx, int w, int h, int chains, snl_state* ou
 int snl_state_destroy(snl_state s) {
   if (!s) return SNL_OK;
   hipSetDevice(s->ctx->device);
+  (void)settle(s);
   hipStreamSynchronize(s->ctx->stream);
   for (void* p : {(void*)s->radiance, (void*)s->rng, (void*)s->pf, (void*)s->pi, (void*)s->q, (void*)s->cnt,
                   (void*)s->tileCnt, (void*)s->radPrev, (void*)s->tileCost, (void*)s->errPartial})
@@ -838,6 +904,7 @@ __global__ void error_sum_kernel(const double* partial, int n, double* out) {
 
 static int seedMode(snl_state s, unsigned seed, int mode) {
   if (!s) return fail(nullptr, SNL_E_ARG, "state is NULL");
+  if (int rc = settle(s)) return rc;
   hipLaunchKernelGGL(seed_kernel, dim3(blocksFor(s->npix)), dim3(256), 0, s->ctx->stream, s->radiance, s->rng,
                      s->npix, seed, mode);
   HIPCHK(s->ctx, hipGetLastError());
@@ -857,6 +924,7 @@ int snl_state_info(snl_state s, int* width, int* height, int* chains) {
 
 int snl_state_read(snl_state s, int which, float* outp) {
   if (!s || !outp || which < 0 || which > 1) return fail(s ? s->ctx : nullptr, SNL_E_ARG, "bad argument");
+  if (int rc = settle(s)) return rc;
   HIPCHK(s->ctx, hipMemcpyAsync(outp, which ? s->rng : s->radiance, (size_t)s->npix * 16, hipMemcpyDeviceToHost,
                                 s->ctx->stream));
   HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
@@ -865,6 +933,7 @@ int snl_state_read(snl_state s, int which, float* outp) {
 
 int snl_state_write(snl_state s, int which, const float* in) {
   if (!s || !in || which < 0 || which > 1) return fail(s ? s->ctx : nullptr, SNL_E_ARG, "bad argument");
+  if (int rc = settle(s)) return rc;
   HIPCHK(s->ctx, hipMemcpyAsync(which ? s->rng : s->radiance, in, (size_t)s->npix * 16, hipMemcpyHostToDevice,
                                 s->ctx->stream));
   HIPCHK(s->ctx, hipStreamSynchronize(s->ctx->stream));
@@ -1182,6 +1251,7 @@ static int renderPersistent(snl_prog p, snl_state st, int frames, hipFunction_t 
 
 int snl_state_error(snl_state st, double* out) {
   if (!st || !out) return fail(st ? st->ctx : nullptr, SNL_E_ARG, "NULL argument");
+  if (int rc = settle(st)) return rc;
   if (st->chains < 2) return fail(st->ctx, SNL_E_ARG, "the error estimate needs a state of two or more chains");
   snl_ctx ctx = st->ctx;
   HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -1243,11 +1313,13 @@ static hipFunction_t queueKernel(snl_prog p, int& blocks) {
   return p->k_mega;
 }
 
-int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, const int* frames,
-                     snl_stats* stats) {
-  if (njobs < 0 || (njobs > 0 && (!progs || !states || !frames))) return fail(nullptr, SNL_E_ARG, "bad batch");
-  if (stats) *stats = snl_stats{};
-  if (njobs == 0) return SNL_OK;
+static int enqueueBatch(snl_batch_s* b, const std::vector<int>& order, std::vector<double>& load);
+
+int snl_batch_submit(int njobs, const snl_prog* progs, const snl_state* states, const int* frames,
+                     snl_batch* out) {
+  if (!out) return fail(nullptr, SNL_E_ARG, "out is NULL");
+  *out = nullptr;
+  if (njobs <= 0 || !progs || !states || !frames) return fail(nullptr, SNL_E_ARG, "bad batch");
   snl_ctx ctx = progs[0] ? progs[0]->ctx : nullptr;
   if (!ctx) return fail(nullptr, SNL_E_ARG, "NULL program in batch");
   // a program's uniforms (resolution, unit-0 snapshot) belong to one job at a
@@ -1262,13 +1334,29 @@ int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, 
       return fail(ctx, SNL_E_ARG, "job %d: program used twice (its uniforms are per job; build one per job)", j);
   }
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  // parameters up, per-job counters zeroed, all on the main stream
-  std::vector<char> unit0(njobs, 0);
   int rc;
+  // a program or state still rendering in an earlier batch finishes first
+  for (int j = 0; j < njobs; ++j)
+    if ((rc = settle(progs[j])) || (rc = settle(states[j]))) return rc;
+  // parameters up, per-job counters zeroed, all on the main stream
+  std::unique_ptr<snl_batch_s> b(new snl_batch_s);
+  b->ctx = ctx;
+  b->njobs = njobs;
+  b->progs.assign(progs, progs + njobs);
+  b->states.assign(states, states + njobs);
+  b->frames.assign(frames, frames + njobs);
+  b->unit0.assign(njobs, 0);
+  b->ordered.assign(njobs, 0);
+  b->done.assign(njobs, 0);
+  b->stream.assign(njobs, 0);
+  b->work.assign(njobs, 0.0);
+  b->steps.assign(njobs, 0);
+  b->normals.assign(njobs, 0);
+  b->ms.assign(njobs, 0.0f);
   for (int j = 0; j < njobs; ++j) {
     bool u = false;
     if ((rc = prepareUnit0(progs[j], states[j], u))) return rc;
-    unit0[j] = u;
+    b->unit0[j] = u;
     if ((rc = upload(progs[j]))) return rc;
     SnlKArgs z;
     fillArgs(progs[j], states[j], z);
@@ -1288,34 +1376,69 @@ int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, 
   }
   // queue-kernel jobs of an 8x8-tiled image take their tiles in cost order;
   // their tile buffers are allocated before anything is launched
-  std::vector<char> ordered(njobs, 0);
   for (int j = 0; j < njobs; ++j) {
     int resident = 0;
-    ordered[j] = !unit0[j] && tileOrdered(progs[j], states[j], queueKernel(progs[j], resident), frames[j]);
-    if (ordered[j] && (rc = ensureTileBuffers(ctx, states[j]))) return rc;
+    b->ordered[j] = !b->unit0[j] && tileOrdered(progs[j], states[j], queueKernel(progs[j], resident), frames[j]);
+    if (b->ordered[j] && (rc = ensureTileBuffers(ctx, states[j]))) return rc;
   }
+  b->ev.assign(njobs, nullptr);
+  HIPCHK(ctx, hipEventCreate(&b->ev0));
+  for (int j = 0; j < njobs; ++j) HIPCHK(ctx, hipEventCreate(&b->ev[j]));
   // Longest job first onto the least loaded stream.  A job lasts about the
   // larger of its throughput time (all its path samples at a device-wide rate)
   // and its latency (its frames run back to back in one lane: frames times a
   // deep path's time, ~bounces); both scales are coarse and only order jobs.
+  // A stream's load counts the unfinished jobs of earlier batches queued on it.
   std::vector<int> order(njobs);
-  for (int j = 0; j < njobs; ++j) order[j] = j;
-  auto work = [&](int j) {
+  for (int j = 0; j < njobs; ++j) {
+    order[j] = j;
     double samples = (double)states[j]->npix * frames[j] * progs[j]->spf;
     double latency = (double)frames[j] * progs[j]->spf * (progs[j]->maxBounces + 1) * 4.0e5;
-    return std::max(samples, latency);
-  };
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
+    b->work[j] = std::max(samples, latency);
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->work[x] > b->work[y]; });
   const int NS = (int)ctx->batchStream.size();
   std::vector<double> load(NS, 0.0);
-  HIPCHK(ctx, hipEventRecord(progs[0]->ev0, ctx->stream));
+  for (snl_batch_s* o : ctx->batches)
+    for (int j = 0; j < o->njobs; ++j) {
+      if ((rc = jobQuery(o, j)) < 0) return rc;
+      if (rc == 0) load[o->stream[j]] += o->work[j];
+    }
+  // on a failure part way, what was queued finishes before the batch is dropped
+  auto drop = [&](int code) {
+    for (int k = 0; k < NS; ++k) (void)hipStreamSynchronize(ctx->batchStream[k]);
+    for (hipEvent_t e : b->ev)
+      if (e) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(b->ev0);
+    return code;
+  };
+  if ((rc = enqueueBatch(b.get(), order, load))) return drop(rc);
+  for (int j = 0; j < njobs; ++j) {
+    progs[j]->busy = states[j]->busy = b.get();
+    progs[j]->busyJob = states[j]->busyJob = j;
+  }
+  b->pending = njobs;
+  ctx->batches.insert(b.get());
+  *out = b.release();
+  return SNL_OK;
+}
+
+// the launches of a submitted batch, jobs in `order`, each onto the least
+// loaded batch stream, with the job's end event after them
+static int enqueueBatch(snl_batch_s* b, const std::vector<int>& order, std::vector<double>& load) {
+  snl_ctx ctx = b->ctx;
+  const int NS = (int)ctx->batchStream.size();
+  int rc;
+  HIPCHK(ctx, hipEventRecord(b->ev0, ctx->stream));
   HIPCHK(ctx, hipEventRecord(ctx->joinEvent, ctx->stream));
   for (int k = 0; k < NS; ++k) HIPCHK(ctx, hipStreamWaitEvent(ctx->batchStream[k], ctx->joinEvent, 0));
+  const int* frames = b->frames.data();
   for (int j : order) {
     int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-    load[k] += work(j);
-    snl_prog p = progs[j];
-    snl_state st = states[j];
+    load[k] += b->work[j];
+    b->stream[j] = k;
+    snl_prog p = b->progs[j];
+    snl_state st = b->states[j];
     hipStream_t ts = ctx->batchStream[k];
     SnlKArgs a;
     fillArgs(p, st, a);
@@ -1331,46 +1454,102 @@ int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, 
     long long want = (st->npix + 255) / 256;
     int blocks = (int)std::max(1LL, std::min<long long>(resident, want));
     void* args[] = {&a};
-    if (ordered[j]) {  // the costliest tiles first after frame 1, as in snl_render
+    if (b->ordered[j]) {  // the costliest tiles first after frame 1, as in snl_render
       if ((rc = launchOrdered(ctx, p, st, kern, blocks, ts, a, frames[j], st->tileCost))) return rc;
-      continue;
+    } else {
+      int launches = b->unit0[j] ? frames[j] : (frames[j] > 0 ? 1 : 0);
+      a.frames = b->unit0[j] ? 1 : frames[j];
+      for (int f = 0; f < launches; ++f) {
+        if (b->unit0[j] && (rc = copyUnit0(ctx, st, ts))) return rc;
+        HIPCHK(ctx, hipModuleLaunchKernel(p->k_persist_reset, 1, 1, 1, 64, 1, 1, 0, ts, args, nullptr));
+        HIPCHK(ctx, hipModuleLaunchKernel(kern, blocks, 1, 1, 256, 1, 1, 0, ts, args, nullptr));
+      }
     }
-    int launches = unit0[j] ? frames[j] : (frames[j] > 0 ? 1 : 0);
-    a.frames = unit0[j] ? 1 : frames[j];
-    for (int f = 0; f < launches; ++f) {
-      if (unit0[j] && (rc = copyUnit0(ctx, st, ts))) return rc;
-      HIPCHK(ctx, hipModuleLaunchKernel(p->k_persist_reset, 1, 1, 1, 64, 1, 1, 0, ts, args, nullptr));
-      HIPCHK(ctx, hipModuleLaunchKernel(kern, blocks, 1, 1, 256, 1, 1, 0, ts, args, nullptr));
-    }
-  }
-  for (int k = 0; k < NS; ++k) {
-    HIPCHK(ctx, hipEventRecord(ctx->batchEvent[k], ctx->batchStream[k]));
-    HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, ctx->batchEvent[k], 0));
-  }
-  HIPCHK(ctx, hipEventRecord(progs[0]->ev1, ctx->stream));
-  HIPCHK(ctx, hipEventSynchronize(progs[0]->ev1));
-  float ms = 0.0f;
-  HIPCHK(ctx, hipEventElapsedTime(&ms, progs[0]->ev0, progs[0]->ev1));
-  if (stats) {
-    stats->ms_total = ms;
-    stats->ms_trace = ms;
-    for (int j = 0; j < njobs; ++j) {
-      long long sk = 0, nk = 0;
-      HIPCHK(ctx, hipMemcpyAsync(&sk, states[j]->tileCnt + C_TRACE_STEPS, 8, hipMemcpyDeviceToHost, ctx->stream));
-      HIPCHK(ctx, hipMemcpyAsync(&nk, states[j]->tileCnt + C_NORMALS, 8, hipMemcpyDeviceToHost, ctx->stream));
-      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-      stats->trace_steps += sk;
-      stats->normal_evals += nk;
-      stats->paths += (long long)frames[j] * progs[j]->spf * states[j]->npix;
-      stats->trace_launches += unit0[j] ? frames[j] : (ordered[j] ? 2 : 1);
-    }
+    HIPCHK(ctx, hipEventRecord(b->ev[j], ts));
   }
   return SNL_OK;
+}
+
+int snl_job_poll(snl_batch b, int job) {
+  if (!b) return fail(nullptr, SNL_E_ARG, "batch is NULL");
+  if (job < -1 || job >= b->njobs) return fail(b->ctx, SNL_E_ARG, "job %d of a %d-job batch", job, b->njobs);
+  HIPCHK(b->ctx, hipSetDevice(b->ctx->device));
+  if (job >= 0) return jobQuery(b, job);
+  for (int j = 0; j < b->njobs; ++j) {
+    int rc = jobQuery(b, j);
+    if (rc < 0) return rc;
+  }
+  return b->pending == 0 ? 1 : 0;
+}
+
+int snl_batch_wait(snl_batch b, snl_stats* stats) {
+  if (!b) return fail(nullptr, SNL_E_ARG, "batch is NULL");
+  snl_ctx ctx = b->ctx;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  for (int j = 0; j < b->njobs; ++j) {
+    int rc = jobWait(b, j);
+    if (rc) return rc;
+  }
+  if (stats) {
+    *stats = snl_stats{};
+    for (int j = 0; j < b->njobs; ++j) {
+      stats->ms_total = std::max(stats->ms_total, (double)b->ms[j]);
+      stats->trace_steps += b->steps[j];
+      stats->normal_evals += b->normals[j];
+      stats->paths += (long long)b->frames[j] * b->progs[j]->spf * b->states[j]->npix;
+      stats->trace_launches += b->unit0[j] ? b->frames[j] : (b->ordered[j] ? 2 : 1);
+    }
+    stats->ms_trace = stats->ms_total;
+  }
+  return SNL_OK;
+}
+
+int snl_batch_job_stats(snl_batch b, int job, snl_stats* out) {
+  if (!b || !out) return fail(b ? b->ctx : nullptr, SNL_E_ARG, "NULL argument");
+  if (job < 0 || job >= b->njobs) return fail(b->ctx, SNL_E_ARG, "job %d of a %d-job batch", job, b->njobs);
+  HIPCHK(b->ctx, hipSetDevice(b->ctx->device));
+  int rc = jobWait(b, job);
+  if (rc) return rc;
+  *out = snl_stats{};
+  out->ms_total = out->ms_trace = b->ms[job];
+  out->trace_steps = b->steps[job];
+  out->normal_evals = b->normals[job];
+  out->paths = (long long)b->frames[job] * b->progs[job]->spf * b->states[job]->npix;
+  out->trace_launches = b->unit0[job] ? b->frames[job] : (b->ordered[job] ? 2 : 1);
+  return SNL_OK;
+}
+
+int snl_batch_release(snl_batch b) {
+  if (!b) return SNL_OK;
+  snl_ctx ctx = b->ctx;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  int rc = SNL_OK;
+  for (int j = 0; j < b->njobs; ++j)
+    if (!rc) rc = jobWait(b, j);
+  for (hipEvent_t e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  ctx->batches.erase(b);
+  delete b;
+  return rc;
+}
+
+int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, const int* frames,
+                     snl_stats* stats) {
+  if (stats) *stats = snl_stats{};
+  if (njobs == 0) return SNL_OK;
+  snl_batch b = nullptr;
+  int rc = snl_batch_submit(njobs, progs, states, frames, &b);
+  if (rc) return rc;
+  rc = snl_batch_wait(b, stats);
+  int rc2 = snl_batch_release(b);
+  return rc ? rc : rc2;
 }
 
 int snl_render(snl_prog p, snl_state st, int frames) {
   if (!p || !st || frames < 0) return fail(p ? p->ctx : nullptr, SNL_E_ARG, "bad argument");
   if (st->ctx != p->ctx) return fail(p->ctx, SNL_E_ARG, "program and state belong to different contexts");
+  if (int rc = settle(st)) return rc;
   // only the megakernel renders the atmosphere; the persistent kernel does not
   // run the sub-surface walk, such programs use the wavefront unless the
   // megakernel is asked for
@@ -1386,6 +1565,7 @@ int snl_render_noalloc(snl_prog p, snl_state st, int frames) {
   if (!p || !st || frames < 0) return fail(p ? p->ctx : nullptr, SNL_E_ARG, "bad argument");
   snl_ctx ctx = p->ctx;
   if (st->ctx != ctx) return fail(ctx, SNL_E_ARG, "program and state belong to different contexts");
+  if (int rc = settle(st)) return rc;
   const long long tiles = (long long)(st->w >> 3) * (st->h >> 3);
   if (tiles > SNL_NOALLOC_MAX_TILES)
     return fail(ctx, SNL_E_ARG,
@@ -1424,8 +1604,8 @@ int snl_set_engine(snl_prog p, int engine, int tiles, int persist_blocks, int sh
 
 int snl_render_normals(snl_prog p, snl_state st, int frames) {
   if (!p || !st || frames < 0) return fail(p ? p->ctx : nullptr, SNL_E_ARG, "bad argument");
-  int rc = upload(p);
-  if (rc) return rc;
+  int rc = settle(st);
+  if (rc || (rc = upload(p))) return rc;
   SnlKArgs a;
   fillArgs(p, st, a);
   for (int f = 0; f < frames; ++f)
@@ -1461,8 +1641,8 @@ int snl_pick(snl_prog p, const float* xy, int n, float* outp) {
 
 int snl_tonemap(snl_prog p, snl_state st, float* rgb, unsigned char* rgba8) {
   if (!p || !st) return fail(p ? p->ctx : nullptr, SNL_E_ARG, "bad argument");
-  int rc = upload(p);
-  if (rc) return rc;
+  int rc = settle(st);
+  if (rc || (rc = upload(p))) return rc;
   snl_ctx ctx = p->ctx;
   long long n = st->npix;
   float* drgb = nullptr;

@@ -27,6 +27,7 @@ EXPORTS = [
     "snl_get_stats", "snl_set_tuning", "snl_set_profiling", "snl_set_engine", "snl_render_batch",
     "snl_state_create_chains", "snl_state_error", "snl_ctx_mem_info", "snl_state_bytes", "snl_set_march_burst",
     "snl_render_noalloc", "snl_program_state_bytes", "snl_state_info", "snl_texture_info",
+    "snl_batch_submit", "snl_job_poll", "snl_batch_wait", "snl_batch_job_stats", "snl_batch_release",
 ]
 ENGINE_WAVEFRONT, ENGINE_PERSISTENT, ENGINE_MEGAKERNEL = 0, 1, 2
 
@@ -91,6 +92,12 @@ def lib():
         "snl_state_info": (ip, [vp] + [ctypes.POINTER(ip)] * 3),
         "snl_texture_info": (ip, [vp] + [ctypes.POINTER(ip)] * 4),
         "snl_set_march_burst": (ip, [vp, ip]),
+        "snl_batch_submit": (ip, [ip, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(ip),
+                                  ctypes.POINTER(vp)]),
+        "snl_job_poll": (ip, [vp, ip]),
+        "snl_batch_wait": (ip, [vp, ctypes.POINTER(Stats)]),
+        "snl_batch_job_stats": (ip, [vp, ip, ctypes.POINTER(Stats)]),
+        "snl_batch_release": (ip, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -143,6 +150,58 @@ def render_batch(jobs):
         return st
     jobs[0][0].ctx.check(lib().snl_render_batch(n, progs, states, frames, ctypes.byref(st)))
     return st
+
+
+class Batch:
+    """snl_batch_submit over [(Program, State, frames), ...]: the jobs' launches
+    are queued and the call returns; poll(j) / wait() / job_stats(j) follow
+    them, release() (or close()) frees the handle.  The jobs' programs and
+    states must stay open until the batch is released."""
+
+    def __init__(self, jobs):
+        n = len(jobs)
+        if n == 0:
+            raise SnellyError("an empty batch")
+        self.ctx = jobs[0][0].ctx
+        self.n = n
+        progs = (ctypes.c_void_p * n)(*[p.h for p, _, _ in jobs])
+        states = (ctypes.c_void_p * n)(*[s.h for _, s, _ in jobs])
+        frames = (ctypes.c_int * n)(*[int(f) for _, _, f in jobs])
+        h = ctypes.c_void_p()
+        self.ctx.check(lib().snl_batch_submit(n, progs, states, frames, ctypes.byref(h)))
+        self.h = h
+
+    def poll(self, job=-1):
+        """True once job `job` (-1: every job) has completed"""
+        rc = lib().snl_job_poll(self.h, job)
+        if rc < 0:
+            self.ctx.check(rc)
+        return rc == 1
+
+    def wait(self):
+        st = Stats()
+        self.ctx.check(lib().snl_batch_wait(self.h, ctypes.byref(st)))
+        return st
+
+    def job_stats(self, job):
+        st = Stats()
+        self.ctx.check(lib().snl_batch_job_stats(self.h, job, ctypes.byref(st)))
+        return st
+
+    def release(self):
+        if self.h:
+            rc = lib().snl_batch_release(self.h)
+            self.h = None
+            self.ctx.check(rc)
+
+    close = release
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().snl_batch_release(self.h)
+        except Exception:
+            pass
 
 
 def precompile(shader, ior, defines, max_march, max_bounces, max_atm, spf, flags=0):

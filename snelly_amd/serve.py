@@ -6,23 +6,27 @@ rendered into K independently seeded chains of its image held in one state
 (snl_state_create_chains), K = 2..32 chosen so that even small images give
 one launch tens of thousands of lanes; the chains' spread, reduced on the GPU
 (snl_state_error), estimates the relative RMS error of their mean image.
-Each tick the server
+The server loop never blocks on the GPU.  Each pass it
   1. admits waiting jobs in arrival order while the device has room for their
      states (snl_ctx_mem_info, snl_state_bytes) and the active set is not full;
-  2. renders one quantum of frames for the active jobs in one
-     snl_render_batch call.  A job wants the frames its error trend asks for,
+  2. submits one quantum of frames for the active jobs that have no work on the
+     GPU, as one snl_batch_submit, while the path samples in flight are below
+     `depth` ticks' budget.  A job wants the frames its error trend asks for,
      frames_needed ~ frames_done * (error / target)^2, within (smallest,
-     largest).  The tick's path-sample budget bounds the whole tick, so its
-     wall time and the wait of a job arriving meanwhile stay bounded: jobs
+     largest).  The tick's path-sample budget bounds each submission: jobs
      take their smallest quantum in round-robin order (the longest unserved
-     first) while the budget lasts -- at least one job per tick -- and what is
-     left goes to the jobs with the least remaining work first (so a cheap job
-     that needs many frames takes them in a few ticks, and a job whose error
-     hardly falls does not hold back the others);
-  3. retires the jobs whose estimate is at or below their target, recording
-     the time from arrival to completion.
-Pixels never read each other's state, so a job's images do not depend on
-which other jobs shared its batches (tests/test_serve.py checks this).
+     first) while the budget lasts -- at least one job -- and what is left goes
+     to the jobs with the least remaining work first (so a cheap job that needs
+     many frames takes them in a few submissions, and a job whose error hardly
+     falls does not hold back the others);
+  3. polls the submitted jobs (snl_job_poll); a finished quantum updates the
+     job's error estimate, and a job at or below its target retires with the
+     time from arrival to completion.
+So a job that arrives while others render is admitted and submitted at once,
+and its launches join the others on the device (the runtime queues them on
+the least loaded streams).  Pixels never read each other's state, so a job's
+images do not depend on which other jobs shared the device
+(tests/test_serve.py checks this).
 """
 import math
 import time
@@ -77,15 +81,16 @@ class Server:
 
     def __init__(self, ctx, flags=native.BUILD_FAST, min_frames=4, quantum=(2, 64), reserve=8 << 30,
                  max_active=256, clock=time.perf_counter, keep_images=False, tick_samples=None,
-                 chain_lanes=1 << 16, max_chains=32):
+                 chain_lanes=1 << 16, max_chains=32, depth=2):
         self.ctx, self.flags, self.keep_images = ctx, flags, keep_images
         self.min_frames, self.quantum, self.reserve, self.max_active = min_frames, quantum, reserve, max_active
         self.tick_samples = tick_samples
         self.chain_lanes, self.max_chains = chain_lanes, max_chains
+        self.depth = depth
         self.clock = clock
-        self.ticks = 0
+        self.ticks = 0      # submissions
         self.samples = 0
-        self.gpu_ms = 0.0
+        self.busy_s = 0.0   # wall time with work on the GPU
 
     def _need(self, job):
         """frames the job still needs by its error trend"""
@@ -137,6 +142,8 @@ class Server:
         waiting = [Job(i, s, tg, t) for i, (t, s, tg) in enumerate(arrivals)]
         waiting.reverse()  # pop() takes the earliest
         queue, active, done = [], [], []
+        inflight = []  # [Batch, [(job, frames)], [harvested flags]]
+        busy_since = None
         while waiting or queue or active:
             now = self.clock() - t0
             if time_limit is not None and now > time_limit:
@@ -144,7 +151,8 @@ class Server:
             while waiting and waiting[-1].arrival <= now:
                 queue.append(waiting.pop())
             # admission by free device memory
-            free, _ = native.mem_info(self.ctx)
+            if queue and len(active) < self.max_active:
+                free, _ = native.mem_info(self.ctx)
             while queue and len(active) < self.max_active:
                 job = queue[0]
                 k = Job.chains_for(job.pixels, self.chain_lanes, self.max_chains)
@@ -155,39 +163,71 @@ class Server:
                     break
                 queue.pop(0)
                 job.open(self.ctx, self.flags, 1000003 * (job.id + 1), self.chain_lanes, self.max_chains)
+                job.busy = False
                 job.admitted = self.clock() - t0
                 free -= need
                 active.append(job)
-            if not active:
-                if waiting:
-                    time.sleep(max(0.0, min(waiting[-1].arrival - (self.clock() - t0), 0.05)))
-                continue
-            counts = self._plan(active, [job.pixels * job.chains for job in active])
-            batch = [(job.a.prog, job.a.state, q) for job, q in zip(active, counts) if q > 0]
-            st = native.render_batch(batch)
-            self.gpu_ms += st.ms_total
-            still = []
-            for job, q in zip(active, counts):
-                if q == 0:
-                    still.append(job)
-                    continue
-                job.last_tick = self.ticks
-                job.frames += q
-                self.samples += q * job.pixels * job.chains
-                if job.frames >= self.min_frames:
-                    v2, m2 = native.state_error(job.a.state)
-                    job.error = math.sqrt(v2 / m2) if m2 > 0 else math.inf
-                if job.error <= job.target:
-                    job.finished = self.clock() - t0
-                    if self.keep_images:
-                        job.image = job.a.radiance().copy()
-                    job.close()
-                    done.append(job)
-                else:
-                    still.append(job)
-            active = still
-            self.ticks += 1
+            # submit the idle jobs while less than `depth` ticks of work is in flight
+            launched = False
+            idle = [job for job in active if not job.busy]
+            pending = sum(q * job.pixels * job.chains for _, jq, got in inflight
+                          for (job, q), g in zip(jq, got) if not g)
+            room = self.tick_samples is None and not inflight or \
+                self.tick_samples is not None and pending < self.depth * self.tick_samples
+            if idle and room:
+                counts = self._plan(idle, [job.pixels * job.chains for job in idle])
+                jq = [(job, q) for job, q in zip(idle, counts) if q > 0]
+                if jq:
+                    b = native.Batch([(job.a.prog, job.a.state, q) for job, q in jq])
+                    for job, _ in jq:
+                        job.busy = True
+                        job.last_tick = self.ticks
+                    inflight.append([b, jq, [False] * len(jq)])
+                    self.ticks += 1
+                    launched = True
+                    if busy_since is None:
+                        busy_since = self.clock()
+            # harvest the quanta that have finished
+            harvested = False
+            for entry in inflight:
+                b, jq, got = entry
+                for i, (job, q) in enumerate(jq):
+                    if got[i] or not b.poll(i):
+                        continue
+                    got[i] = harvested = True
+                    job.busy = False
+                    job.frames += q
+                    self.samples += q * job.pixels * job.chains
+                    if job.frames >= self.min_frames:
+                        v2, m2 = native.state_error(job.a.state)
+                        job.error = math.sqrt(v2 / m2) if m2 > 0 else math.inf
+                    if job.error <= job.target:
+                        job.finished = self.clock() - t0
+                        if self.keep_images:
+                            job.image = job.a.radiance().copy()
+                        active.remove(job)
+                        job.close_after = True
+                        done.append(job)
+                if all(got):
+                    b.release()
+                    for job, _ in jq:
+                        if getattr(job, "close_after", False):
+                            job.close()
+            inflight = [e for e in inflight if not all(e[2])]
+            if not inflight and busy_since is not None:
+                self.busy_s += self.clock() - busy_since
+                busy_since = None
+            if not launched and not harvested:
+                # nothing to do until a quantum finishes or a job arrives
+                nxt = waiting[-1].arrival - (self.clock() - t0) if waiting else 0.05
+                time.sleep(max(0.0, min(nxt, 0.002 if inflight else 0.05)))
+        for b, jq, got in inflight:
+            b.release()
+        if busy_since is not None:
+            self.busy_s += self.clock() - busy_since
         for job in active:
+            job.close()
+        for job in done:
             job.close()
         return done, active + queue + waiting
 

@@ -136,3 +136,68 @@ def test_batch_with_megakernel_jobs(ctx):
         assert np.array_equal(rad, alone.radiance()), s.name
         assert np.array_equal(rng, alone.state.read(1)), s.name
         alone.close()
+
+
+def test_batch_submit_rejects_bad_arguments():
+    L = native.lib()
+    h = ctypes.c_void_p()
+    null = (ctypes.c_void_p * 1)(None)
+    fr = (ctypes.c_int * 1)(1)
+    assert L.snl_batch_submit(0, null, null, fr, ctypes.byref(h)) != 0
+    assert L.snl_batch_submit(1, null, null, fr, ctypes.byref(h)) != 0
+    assert L.snl_batch_submit(1, null, null, fr, None) != 0
+    assert not h.value
+    assert L.snl_job_poll(None, 0) != 0 and L.snl_batch_wait(None, None) != 0
+    assert L.snl_batch_release(None) == 0
+
+
+@pytest.mark.gpu
+def test_submitted_batches_overlap_and_equal_alone(ctx):
+    """two batches submitted back to back without waiting: every job equals the
+    job rendered alone; a state read while its job may still run returns the
+    finished result (the read waits for the job); polling reaches 1 for every
+    job; a repeated state in one submit is rejected"""
+    jobs = small_jobs()
+    sessions = [host.Session(s, s.width, s.height, ctx, seed=300 + j, flags=native.BUILD_PRECISE)
+                for j, s in enumerate(jobs)]
+    first = [(se.prog, se.state, s.spp) for se, s in zip(sessions[:4], jobs[:4])]
+    second = [(se.prog, se.state, s.spp) for se, s in zip(sessions[4:], jobs[4:])]
+    b1 = native.Batch(first)
+    b2 = native.Batch(second)
+    early = sessions[0].radiance().copy()  # waits for job 0 of b1
+    st2 = b2.wait()
+    assert st2.paths == sum(s.width * s.height * s.spp for s in jobs[4:])
+    while not b1.poll():
+        pass
+    assert all(b1.poll(j) for j in range(4)) and all(b2.poll(j) for j in range(len(second)))
+    js = b1.job_stats(1)
+    assert js.paths == jobs[1].width * jobs[1].height * jobs[1].spp and js.ms_total > 0
+    b1.release()
+    b2.release()
+    with pytest.raises(native.SnellyError):
+        native.Batch([first[0], (sessions[1].prog, sessions[0].state, 1)])
+    for j, (se, s) in enumerate(zip(sessions, jobs)):
+        rad, rng = se.radiance(), se.state.read(1)
+        alone_rad, alone_rng = run_alone(ctx, s, 300 + j)
+        assert np.array_equal(rad, alone_rad), s.name
+        assert np.array_equal(rng, alone_rng), s.name
+        if j == 0:
+            assert np.array_equal(early, alone_rad)
+    for se in sessions:
+        se.close()
+
+
+@pytest.mark.gpu
+def test_resubmitting_a_busy_job_waits_for_it(ctx):
+    """a job submitted again while its first quantum may still run renders
+    after it: 2 + 3 frames in two batches equal 5 frames alone"""
+    s = scenes.batch_job("menger3", "DIELECTRIC", 40, 5, 5, np.random.RandomState(4))
+    se = host.Session(s, 40, 40, ctx, seed=17, flags=native.BUILD_PRECISE)
+    b1 = native.Batch([(se.prog, se.state, 2)])
+    b2 = native.Batch([(se.prog, se.state, 3)])
+    b2.wait()
+    b1.release()
+    b2.release()
+    rad = se.radiance().copy()
+    se.close()
+    assert np.array_equal(rad, run_alone(ctx, s, 17)[0])

@@ -154,3 +154,28 @@ def test_served_jobs_finish_and_match_alone(ctx, tick_samples):
         alone.render(job.frames)
         assert np.array_equal(alone.radiance(), job.image), s.name
         alone.close()
+
+
+@pytest.mark.gpu
+def test_served_2048_job_matches_alone(ctx):
+    """a 2048 x 2048 job arriving among small ones, all submitted without
+    blocking: every job's image equals the job rendered alone with the same
+    seeds and frames"""
+    rs = np.random.RandomState(5)
+    big = scenes.batch_job("sphere", "SURFACE", 2048, 0, 4, rs)
+    arr = [(0.0, scenes.batch_job("menger2", "METAL", 48, 0, 4, rs), 0.3),
+           (0.0, big, 0.3),
+           (0.01, scenes.batch_job("torus", "DIELECTRIC", 40, 0, 5, rs), 0.3),
+           (0.02, scenes.batch_job("menger3", "SURFACE", 64, 0, 4, rs), 0.3)]
+    srv = serve.Server(ctx, flags=native.BUILD_PRECISE, reserve=1 << 30, keep_images=True, quantum=(2, 16),
+                       tick_samples=1 << 22)
+    done, left = srv.run(arr, time_limit=120.0)
+    assert not left and len(done) == 4
+    assert any(j.scene.width == 2048 for j in done)
+    for job in done:
+        s = job.scene
+        alone = host.Session(s, s.width, s.height, ctx, seed=1000003 * (job.id + 1), flags=native.BUILD_PRECISE,
+                             chains=job.chains)
+        alone.render(job.frames)
+        assert np.array_equal(alone.radiance(), job.image), s.name
+        alone.close()

@@ -260,20 +260,35 @@ def batched(ctx, steps, warmup):
     ctx.sync()
     plain_frame_s = time.perf_counter() - t
     batch = [(se.prog, se.state, s.spp) for se, s in zip(sessions, jobs)]
-    for w in range(warmup):
-        native.render_batch(batch)
-    times, gpu_ms = [], []
-    for k in range(steps):
-        seed_all(900 + 64 * k)
-        ctx.sync()
-        t = time.perf_counter()
-        st = native.render_batch(batch)
-        ctx.sync()
-        times.append(time.perf_counter() - t)
-        gpu_ms.append(st.ms_total)
+    # the batch module of the jobs' programs (compiled ahead into the code-object
+    # cache): with fusion on, the megakernel jobs run in one persistent launch
+    native.batch_module(ctx, [se.prog for se in sessions])
+
+    def timed(fusion):
+        native.set_batch_fusion(ctx, fusion)
+        for w in range(warmup):
+            native.render_batch(batch)
+        times, gpu_ms, launches = [], [], 0
+        for k in range(steps):
+            seed_all(900 + 64 * k)
+            ctx.sync()
+            t = time.perf_counter()
+            st = native.render_batch(batch)
+            ctx.sync()
+            times.append(time.perf_counter() - t)
+            gpu_ms.append(st.ms_total)
+            launches = st.trace_launches
+        return sum(times) / len(times), sum(gpu_ms) / len(gpu_ms), launches
+
+    streams_s, streams_ms, streams_launches = timed(False)
+    fused_s, fused_ms, fused_launches = timed(True)
+    # the faster of the two is the batched figure (and the context's setting)
+    fusion = fused_s < streams_s
+    native.set_batch_fusion(ctx, fusion)
+    batch_s, batch_ms = (fused_s, fused_ms) if fusion else (streams_s, streams_ms)
+    times, gpu_ms = [batch_s], [batch_ms]
     for se in sessions:
         se.close()
-    batch_s = sum(times) / len(times)
     fams = {}
     for s in jobs:
         fams[s.name] = fams.get(s.name, 0) + 1
@@ -283,6 +298,11 @@ def batched(ctx, steps, warmup):
             "plain_per_frame_path_samples_per_s": round(samples / plain_frame_s, 1),
             "plain_per_frame_s": round(plain_frame_s, 3),
             "batched_path_samples_per_s": round(samples / batch_s, 1), "batched_s": round(batch_s, 3),
+            "batched_mode": "one persistent launch (batch module)" if fusion else "per-job launches over streams",
+            "streams_s": round(streams_s, 3), "streams_launches": streams_launches,
+            "fused_s": round(fused_s, 3), "fused_launches": fused_launches,
+            "fused": "megakernel jobs in one snl_mega_fused launch, heaviest first; persistent-engine jobs on "
+                     "streams beside it",
             "batched_gpu_ms": round(sum(gpu_ms) / len(gpu_ms), 2),
             "speedup_vs_plain": round(plain_own_s / batch_s, 2),
             "speedup_vs_plain_per_frame": round(plain_frame_s / batch_s, 2),
@@ -360,6 +380,7 @@ def main():
     ap.add_argument("--served-jobs", type=int, default=2000)
     ap.add_argument("--served-seconds", type=float, default=100.0)
     ap.add_argument("--copy-yardstick", type=int, nargs="?", const=1024, default=None, metavar="MIB")
+    ap.add_argument("--batch-only", action="store_true", help="only the batched leg (config 4), as one JSON line")
     args = ap.parse_args()
     if args.copy_yardstick is not None:
         copy_yardstick(args.copy_yardstick)
@@ -372,6 +393,9 @@ def main():
     from snelly_amd import autotune, native, scenes
 
     ctx = native.Context(0)
+    if args.batch_only:
+        print(json.dumps({"batched": batched(ctx, max(1, min(args.steps, 3)), min(args.warmup, 1))}), flush=True)
+        return
     # ---- headline: config 3 (megakernel: the medium's tracking runs inside a vertex)
     scene = scenes.mandelbox_volume(1920, 1080, 256)
     W, H, spp = scene.width, scene.height, scene.spp

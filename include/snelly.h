@@ -220,6 +220,28 @@ int snl_batch_wait(snl_batch batch, snl_stats* stats);
 int snl_batch_job_stats(snl_batch batch, int job, snl_stats* out);
 int snl_batch_release(snl_batch batch);
 
+/* Batch modules: one code object holding the megakernel of several programs,
+ * so that a batch of jobs of different scenes, resolutions and frame counts
+ * runs as one persistent launch (north-star batching; the reference draws one
+ * scene per canvas).  snl_batch_module builds (or fetches from the code-object
+ * cache) the module of the given programs' kinds -- a kind is a program's
+ * translated scene with its #defines and __MAX_*__ limits; programs that
+ * differ only in uniforms share one.  The programs must be path-tracer
+ * programs of one build group (dispersion, interactive mode, PRECISE/FAST).
+ * While batch fusion is on (snl_ctx_set_batch_fusion, default off), a
+ * submitted batch's jobs that one loaded module covers -- megakernel programs
+ * that do not sample texture unit 0 -- run in that module's persistent launch,
+ * heaviest first: its waves take pixels from one job until its queue is
+ * empty, then from the next; the other jobs run as before.  Per job the
+ * result is identical to snl_render.  snl_batch_module_precompile compiles the
+ * module of programs given as snl_program_build's sources into the cache,
+ * without a device. */
+int snl_batch_module(snl_ctx ctx, int nprogs, const snl_prog* progs);
+int snl_batch_module_precompile(int nprogs, const char* const* shaders, const char* const* iors,
+                                const char* const* defines, const int* max_march, const int* max_atm,
+                                unsigned flags);
+int snl_ctx_set_batch_fusion(snl_ctx ctx, int on);
+
 /* Served rendering support (no counterpart in the reference, which renders one
  * interactive canvas).  A state of K chains holds K independent accumulations
  * of one width x height image: pixel p of chain k is state pixel k*w*h + p,

@@ -38,6 +38,19 @@ using namespace glsl;
 
 // ---------------------------------------------------------------- objects
 #define SNL_MAX_TILES 4
+struct FusedModule;
+// One program as a part of a batch module: its translated scene, its #define
+// lines, its __MAX_*__ limits and the build modes a module must share.
+struct FusedPart {
+  std::string sceneCode, defs;
+  unsigned long long key = 0;  // hash of the scene, its defines and limits: the program's kind
+  int maxMarch = 256, maxAtm = 1;
+  bool fastDiv = false, precise = false, dispersion = false, interactive = false;
+  bool fusable = false;  // a path-tracer program (not 'ao' / 'ptsimple')
+  bool sameGroup(const FusedPart& o) const {
+    return fastDiv == o.fastDiv && precise == o.precise && dispersion == o.dispersion && interactive == o.interactive;
+  }
+};
 struct snl_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -53,6 +66,8 @@ struct snl_ctx_s {
   int numCU = 256;
   int* scratch = nullptr;  // this device's copy of the allocation-free render's `scratch` array
   std::set<snl_batch_s*> batches;  // submitted batches not yet released
+  std::map<unsigned long long, FusedModule*> fused;  // batch modules by source hash
+  bool batchFusion = false;  // snl_ctx_set_batch_fusion
 };
 
 struct snl_tex_s {
@@ -115,6 +130,18 @@ struct snl_prog_s {
   std::map<int, snl_tex_s*> texOf;  // sampler uniform -> the texture bound to it (null once destroyed)
   snl_batch_s* busy = nullptr;  // the submitted batch whose job `busyJob` runs this program
   int busyJob = -1;
+  FusedPart part;  // what a batch module holding this program needs of it
+};
+
+// A batch module: the megakernel of several programs in one code object, one
+// instantiation per distinct program (its "kind"), run by one persistent
+// launch over all jobs (snl_mega_fused); built on first use, kept per context.
+struct FusedModule {
+  hipModule_t mod = nullptr;
+  hipFunction_t k_run = nullptr, k_reset = nullptr;
+  int blocks = 0;
+  std::map<unsigned long long, int> kindOf;  // program kind (FusedPart::key) -> instantiation
+  FusedPart group;                           // the build modes its programs share
 };
 
 // A submitted batch (snl_batch_submit): its jobs' launches are queued on the
@@ -131,6 +158,12 @@ struct snl_batch_s {
   std::vector<float> ms;               // submit to the job's end (hipEvents)
   std::vector<hipEvent_t> ev;          // per job: after its last launch
   hipEvent_t ev0 = nullptr;            // before the first launch
+  // the jobs one batch module runs in one persistent launch (may be empty),
+  // and that launch's arguments (host copy and device copy)
+  FusedModule* fm = nullptr;
+  std::vector<int> fusedJobs;
+  std::vector<char> hFused;
+  void* dFused = nullptr;
 };
 
 namespace {
@@ -200,6 +233,9 @@ struct Compiled {
   std::vector<char> code;
   snl::TranslateResult tr;
   std::set<std::string> defines;
+  std::string defs;  // the program's #define lines
+  bool fastDiv = false;
+  int maxAtm = 1;
 };
 
 // Compile `source` with `opts` through hipRTC, or fetch the code object from
@@ -350,6 +386,9 @@ int compileProgram(snl_ctx ctx, const char* shader, const char* ior, const char*
     // scaling around sqrt, reciprocals and divisions
     opts.push_back("-fgpu-flush-denormals-to-zero");
   }
+  out.defs = defs;
+  out.fastDiv = fastDiv;
+  out.maxAtm = maxAtm;
   int rc = compileCached(ctx, source, opts, out.code, log, loglen);
   if (rc) return rc;
   // The megakernel asks for four waves per SIMD (at most 128 VGPRs); where that
@@ -359,6 +398,135 @@ int compileProgram(snl_ctx ctx, const char* shader, const char* ior, const char*
     rc = compileCached(ctx, source, opts, out.code, log, loglen);
   }
   return rc;
+}
+
+FusedPart partOf(const Compiled& c, int maxMarch, unsigned flags) {
+  FusedPart f;
+  f.sceneCode = c.tr.code;
+  f.defs = c.defs;
+  f.maxMarch = maxMarch;
+  f.maxAtm = c.maxAtm;
+  f.fastDiv = c.fastDiv;
+  f.precise = (flags & SNL_BUILD_PRECISE) != 0;
+  f.dispersion = c.defines.count("DISPERSION_ENABLED") > 0;
+  f.interactive = c.defines.count("INTERACTIVE_MODE") > 0;
+  f.fusable = (flags & (SNL_BUILD_AO | SNL_BUILD_PTSIMPLE)) == 0;
+  f.key = fnv1a(f.defs + "\n" + std::to_string(maxMarch) + " " + std::to_string(f.maxAtm) + "\n" + f.sceneCode);
+  return f;
+}
+
+// The batch module of the programs `kinds` (one build group, sorted by key):
+// each program's translated scene in a namespace of its own, between its own
+// #defines and __MAX_*__ limits, then the kernel headers once, then a
+// persistent kernel that walks the jobs in the order given and runs each with
+// its program's megakernel body -- the code and arithmetic of that program's
+// snl_mega, so a job renders bit for bit as it does alone.
+std::string fusedSource(const std::vector<const FusedPart*>& kinds) {
+  const FusedPart& g = *kinds[0];
+  std::ostringstream src;
+  src << "#define SNL_FUSED 1\n#include \"snl_prelude.h\"\n#include \"snl_wavefront.h\"\n";
+  if (g.dispersion) src << "#define DISPERSION_ENABLED\n";
+  if (g.interactive) src << "#define INTERACTIVE_MODE\n";
+  if (g.fastDiv) src << "#pragma clang fp reciprocal(on)\n";
+  for (size_t k = 0; k < kinds.size(); ++k) {
+    const FusedPart& p = *kinds[k];
+    std::set<std::string> names;
+    std::istringstream in(p.defs);
+    std::string line;
+    src << "#define SNL_MAX_MARCH_STEPS " << p.maxMarch << "\n#define SNL_MAX_ATMOSPHERE_SCATTERS " << p.maxAtm
+        << "\n";
+    while (std::getline(in, line)) {
+      std::istringstream w(line);
+      std::string d, name;
+      w >> d >> name;
+      if (d != "#define" || name.empty() || name == "DISPERSION_ENABLED" || name == "INTERACTIVE_MODE") continue;
+      src << line << "\n";
+      names.insert(name);
+    }
+    src << "namespace glsl {\nnamespace snlk" << k << " {\n" << p.sceneCode << "\n}  // namespace snlk" << k
+        << "\n}  // namespace glsl\n";
+    for (const auto& n : names) src << "#undef " << n << "\n";
+    src << "#undef SNL_MAX_MARCH_STEPS\n#undef SNL_MAX_ATMOSPHERE_SCATTERS\n";
+  }
+  if (g.fastDiv) src << "#pragma clang fp reciprocal(off)\n";
+  // for the single-program kernels' templates, which this module does not instantiate
+  src << "#define SNL_MAX_MARCH_STEPS 256\n#define SNL_MAX_ATMOSPHERE_SCATTERS 1\n";
+  src << "#line 1 \"snl_pt.h\"\n#include \"snl_pt.h\"\n#include \"snl_kernels.h\"\n#include \"snl_persist.h\"\n"
+         "#include \"snl_mega.h\"\n";
+  src << "namespace glsl {\n"
+         "struct SnlFusedArgs {\n"
+         "  const SnlKArgs* jobs;  // per job: its program's parameters and its state\n"
+         "  const int* kind;       // per job: its program's instantiation\n"
+         "  const int* order;      // the jobs in the order the waves take them\n"
+         "  int njobs;\n"
+         "};\n"
+         "extern \"C\" __global__ void snl_fused_reset(SnlFusedArgs f) {\n"
+         "  if ((int)blockIdx.x < f.njobs && threadIdx.x < C_COUNT) f.jobs[blockIdx.x].cnt[threadIdx.x] = 0;\n"
+         "}\n"
+         "extern \"C\" __global__ void __launch_bounds__(256, SNL_MEGA_WAVES) snl_mega_fused(SnlFusedArgs f) {\n"
+         "  for (int q = 0; q < f.njobs; ++q) {\n"
+         "    const int j = f.order[q];\n"
+         "    switch (f.kind[j]) {\n";
+  for (size_t k = 0; k < kinds.size(); ++k)
+    src << "      case " << k << ": mega_body<snlk" << k << "::SnlScene>(f.jobs[j]); break;\n";
+  src << "      default: break;\n    }\n  }\n}\n}  // namespace glsl\n";
+  return src.str();
+}
+
+// compile (or fetch from the cache) the batch module of `kinds`; no device needed
+int compileFused(snl_ctx ctx, std::vector<const FusedPart*> kinds, std::vector<char>& code, std::string& source) {
+  if (kinds.empty()) return fail(ctx, SNL_E_ARG, "a batch module needs programs");
+  std::sort(kinds.begin(), kinds.end(), [](const FusedPart* a, const FusedPart* b) { return a->key < b->key; });
+  kinds.erase(std::unique(kinds.begin(), kinds.end(), [](const FusedPart* a, const FusedPart* b) {
+                return a->key == b->key; }), kinds.end());
+  for (const FusedPart* k : kinds) {
+    if (!k->fusable) return fail(ctx, SNL_E_ARG, "a batch module holds path-tracer programs only");
+    if (!k->sameGroup(*kinds[0]))
+      return fail(ctx, SNL_E_ARG, "a batch module's programs share dispersion, interactive mode and build mode");
+  }
+  source = fusedSource(kinds);
+  const FusedPart& g = *kinds[0];
+  std::vector<std::string> opts = {"--offload-arch=gfx950", "-O3", "-std=c++17",
+                                   "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  if (g.precise) opts.push_back("-ffp-contract=off");
+  if (g.fastDiv) {
+    opts.push_back("-fno-hip-fp32-correctly-rounded-divide-sqrt");
+    opts.push_back("-fgpu-flush-denormals-to-zero");
+  }
+  int rc = compileCached(ctx, source, opts, code, nullptr, 0);
+  if (rc) return rc;
+  if (kernelPrivateBytes(code, "snl_mega_fused.kd") > 0) {  // four waves spill: build without the bound
+    opts.push_back("-DSNL_MEGA_WAVES=1");
+    rc = compileCached(ctx, source, opts, code, nullptr, 0);
+  }
+  return rc;
+}
+
+int buildFused(snl_ctx ctx, const std::vector<const FusedPart*>& kinds, FusedModule*& out) {
+  std::vector<char> code;
+  std::string source;
+  int rc = compileFused(ctx, kinds, code, source);
+  if (rc) return rc;
+  const unsigned long long key = fnv1a(source);
+  auto it = ctx->fused.find(key);
+  if (it != ctx->fused.end()) { out = it->second; return SNL_OK; }
+  std::unique_ptr<FusedModule> m(new FusedModule);
+  HIPCHK(ctx, hipModuleLoadData(&m->mod, code.data()));
+  HIPCHK(ctx, hipModuleGetFunction(&m->k_run, m->mod, "snl_mega_fused"));
+  HIPCHK(ctx, hipModuleGetFunction(&m->k_reset, m->mod, "snl_fused_reset"));
+  int occ = 0;
+  if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, m->k_run, 256, 0) != hipSuccess || occ < 1) occ = 2;
+  m->blocks = ctx->numCU * occ;
+  std::vector<const FusedPart*> sorted(kinds);
+  std::sort(sorted.begin(), sorted.end(), [](const FusedPart* a, const FusedPart* b) { return a->key < b->key; });
+  sorted.erase(std::unique(sorted.begin(), sorted.end(), [](const FusedPart* a, const FusedPart* b) {
+                 return a->key == b->key; }), sorted.end());
+  for (size_t k = 0; k < sorted.size(); ++k) m->kindOf[sorted[k]->key] = (int)k;
+  m->group = *sorted[0];
+  m->group.sceneCode.clear();
+  out = m.get();
+  ctx->fused[key] = m.release();
+  return SNL_OK;
 }
 
 void fillArgs(snl_prog p, snl_state st, SnlKArgs& a) {
@@ -543,6 +711,11 @@ int snl_ctx_destroy(snl_ctx ctx) {
   if (!ctx) return SNL_OK;
   hipSetDevice(ctx->device);
   while (!ctx->batches.empty()) (void)snl_batch_release(*ctx->batches.begin());
+  for (auto& kv : ctx->fused) {
+    if (kv.second->mod) (void)hipModuleUnload(kv.second->mod);
+    delete kv.second;
+  }
+  ctx->fused.clear();
   if (ctx->stream) hipStreamDestroy(ctx->stream);
   for (int k = 0; k < SNL_MAX_TILES; ++k) {
     if (ctx->tileStream[k]) hipStreamDestroy(ctx->tileStream[k]);
@@ -596,6 +769,7 @@ int snl_program_build(snl_ctx ctx, const char* shader, const char* ior, const ch
   p->maxBounces = maxBounces;
   p->maxMarch = maxMarch;
   p->spf = spf;
+  p->part = partOf(c, maxMarch, flags);
   memset(&p->hostP, 0, sizeof(SnlParams));
   p->user = c.tr.uniforms;
   p->hostU.assign(c.tr.userBlockSize, 0);
@@ -1381,6 +1555,27 @@ int snl_batch_submit(int njobs, const snl_prog* progs, const snl_state* states, 
     b->ordered[j] = !b->unit0[j] && tileOrdered(progs[j], states[j], queueKernel(progs[j], resident), frames[j]);
     if (b->ordered[j] && (rc = ensureTileBuffers(ctx, states[j]))) return rc;
   }
+  // the jobs that one loaded batch module can run together: megakernel
+  // programs of its build group whose kinds it holds (the module covering the
+  // most jobs; fewer than two gain nothing)
+  if (ctx->batchFusion && !ctx->fused.empty()) {
+    std::vector<int> cand;
+    for (int j = 0; j < njobs; ++j) {
+      int resident = 0;
+      if (progs[j]->part.fusable && !b->unit0[j] && frames[j] > 0 &&
+          queueKernel(progs[j], resident) == progs[j]->k_mega)
+        cand.push_back(j);
+    }
+    for (auto& kv : ctx->fused) {
+      FusedModule* m = kv.second;
+      std::vector<int> got;
+      for (int j : cand)
+        if (progs[j]->part.sameGroup(m->group) && m->kindOf.count(progs[j]->part.key)) got.push_back(j);
+      if (got.size() > b->fusedJobs.size()) { b->fm = m; b->fusedJobs = got; }
+    }
+    if (b->fusedJobs.size() < 2) { b->fm = nullptr; b->fusedJobs.clear(); }
+    for (int j : b->fusedJobs) b->ordered[j] = 0;  // one launch in queue order, jobs ordered by cost
+  }
   b->ev.assign(njobs, nullptr);
   HIPCHK(ctx, hipEventCreate(&b->ev0));
   for (int j = 0; j < njobs; ++j) HIPCHK(ctx, hipEventCreate(&b->ev[j]));
@@ -1410,6 +1605,7 @@ int snl_batch_submit(int njobs, const snl_prog* progs, const snl_state* states, 
     for (hipEvent_t e : b->ev)
       if (e) (void)hipEventDestroy(e);
     (void)hipEventDestroy(b->ev0);
+    if (b->dFused) (void)hipFree(b->dFused);
     return code;
   };
   if ((rc = enqueueBatch(b.get(), order, load))) return drop(rc);
@@ -1433,7 +1629,65 @@ static int enqueueBatch(snl_batch_s* b, const std::vector<int>& order, std::vect
   HIPCHK(ctx, hipEventRecord(ctx->joinEvent, ctx->stream));
   for (int k = 0; k < NS; ++k) HIPCHK(ctx, hipStreamWaitEvent(ctx->batchStream[k], ctx->joinEvent, 0));
   const int* frames = b->frames.data();
+  std::vector<char> isFused(b->njobs, 0);
+  if (b->fm) {
+    // one persistent launch over the module's jobs, heaviest first: its waves
+    // take pixels from one job until that job's queue runs dry, then move on
+    const int n = (int)b->fusedJobs.size();
+    std::vector<int> ord(n);
+    double w = 0.0;
+    long long blocksWanted = 0;
+    for (int i = 0; i < n; ++i) {
+      ord[i] = i;
+      isFused[b->fusedJobs[i]] = 1;
+      w += b->work[b->fusedJobs[i]];
+      blocksWanted += (b->states[b->fusedJobs[i]]->npix + 255) / 256;
+    }
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+      return b->work[b->fusedJobs[x]] > b->work[b->fusedJobs[y]]; });
+    const size_t argBytes = sizeof(SnlKArgs) * n, intBytes = sizeof(int) * n;
+    b->hFused.assign(argBytes + 2 * intBytes, 0);
+    SnlKArgs* ha = reinterpret_cast<SnlKArgs*>(b->hFused.data());
+    int* hk = reinterpret_cast<int*>(b->hFused.data() + argBytes);
+    int* ho = hk + n;
+    for (int i = 0; i < n; ++i) {
+      const int j = b->fusedJobs[i];
+      snl_prog p = b->progs[j];
+      snl_state st = b->states[j];
+      fillArgs(p, st, ha[i]);
+      ha[i].chunk = p->persistChunk;
+      ha[i].shadeThresh = p->shadeThresh;
+      ha[i].cnt = st->tileCnt;
+      ha[i].itemBase = 0;
+      ha[i].itemEnd = (int)st->npix;
+      ha[i].frames = frames[j];
+      ha[i].tileCost = nullptr;
+      ha[i].tileOrder = nullptr;
+      hk[i] = b->fm->kindOf.at(p->part.key);
+      ho[i] = ord[i];
+    }
+    int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[k] += w;
+    hipStream_t ts = ctx->batchStream[k];
+    HIPCHK(ctx, hipMalloc(&b->dFused, b->hFused.size()));
+    HIPCHK(ctx, hipMemcpyAsync(b->dFused, b->hFused.data(), b->hFused.size(), hipMemcpyHostToDevice, ts));
+    struct {
+      const void* jobs;
+      const void* kind;
+      const void* order;
+      int njobs;
+    } fa = {b->dFused, (const char*)b->dFused + argBytes, (const char*)b->dFused + argBytes + intBytes, n};
+    void* fargs[] = {&fa};
+    HIPCHK(ctx, hipModuleLaunchKernel(b->fm->k_reset, n, 1, 1, 64, 1, 1, 0, ts, fargs, nullptr));
+    const int blocks = (int)std::max(1LL, std::min<long long>(b->fm->blocks, blocksWanted));
+    HIPCHK(ctx, hipModuleLaunchKernel(b->fm->k_run, blocks, 1, 1, 256, 1, 1, 0, ts, fargs, nullptr));
+    for (int i = 0; i < n; ++i) {
+      b->stream[b->fusedJobs[i]] = k;
+      HIPCHK(ctx, hipEventRecord(b->ev[b->fusedJobs[i]], ts));
+    }
+  }
   for (int j : order) {
+    if (isFused[j]) continue;
     int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
     load[k] += b->work[j];
     b->stream[j] = k;
@@ -1499,6 +1753,7 @@ int snl_batch_wait(snl_batch b, snl_stats* stats) {
       stats->paths += (long long)b->frames[j] * b->progs[j]->spf * b->states[j]->npix;
       stats->trace_launches += b->unit0[j] ? b->frames[j] : (b->ordered[j] ? 2 : 1);
     }
+    if (b->fm) stats->trace_launches -= (int)b->fusedJobs.size() - 1;  // one launch for all of them
     stats->ms_trace = stats->ms_total;
   }
   return SNL_OK;
@@ -1529,6 +1784,7 @@ int snl_batch_release(snl_batch b) {
   for (hipEvent_t e : b->ev)
     if (e) (void)hipEventDestroy(e);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->dFused) (void)hipFree(b->dFused);
   ctx->batches.erase(b);
   delete b;
   return rc;
@@ -1544,6 +1800,43 @@ int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, 
   rc = snl_batch_wait(b, stats);
   int rc2 = snl_batch_release(b);
   return rc ? rc : rc2;
+}
+
+int snl_batch_module(snl_ctx ctx, int nprogs, const snl_prog* progs) {
+  if (!ctx || nprogs <= 0 || !progs) return fail(ctx, SNL_E_ARG, "bad argument");
+  std::vector<const FusedPart*> kinds;
+  for (int i = 0; i < nprogs; ++i) {
+    if (!progs[i] || progs[i]->ctx != ctx) return fail(ctx, SNL_E_ARG, "program %d: NULL or another context", i);
+    kinds.push_back(&progs[i]->part);
+  }
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  FusedModule* m = nullptr;
+  return buildFused(ctx, kinds, m);
+}
+
+int snl_batch_module_precompile(int nprogs, const char* const* shaders, const char* const* iors,
+                                const char* const* defines, const int* maxMarch, const int* maxAtm, unsigned flags) {
+  if (nprogs <= 0 || !shaders || !iors || !defines || !maxMarch || !maxAtm)
+    return fail(nullptr, SNL_E_ARG, "bad argument");
+  std::vector<FusedPart> parts(nprogs);
+  for (int i = 0; i < nprogs; ++i) {
+    Compiled c;
+    int rc = compileProgram(nullptr, shaders[i], iors[i], defines[i], maxMarch[i], 1, maxAtm[i], 1, flags, c,
+                            nullptr, 0);
+    if (rc) return rc;
+    parts[i] = partOf(c, maxMarch[i], flags);
+  }
+  std::vector<const FusedPart*> kinds;
+  for (auto& f : parts) kinds.push_back(&f);
+  std::vector<char> code;
+  std::string source;
+  return compileFused(nullptr, kinds, code, source);
+}
+
+int snl_ctx_set_batch_fusion(snl_ctx ctx, int on) {
+  if (!ctx) return fail(nullptr, SNL_E_ARG, "ctx is NULL");
+  ctx->batchFusion = on != 0;
+  return SNL_OK;
 }
 
 int snl_render(snl_prog p, snl_state st, int frames) {

@@ -9,7 +9,7 @@ namespace glsl {
 // scene's feature flags discards the calls into hooks a scene does not define.
 #define SNL_SCENE_SETUP                                               \
   const SnlParams& P = *a.P;                                          \
-  SC S(a.P, (const SnlUser*)a.U);                                     \
+  SC S(a.P, (const typename SC::User*)a.U);                           \
   S.INIT();                                                           \
   PT<SC> pt(S, P);                                                    \
   (void)pt;
@@ -132,7 +132,9 @@ __device__ inline void camera_body(const SnlKArgs& a) {
   int slot = snl_append(&a.cnt[C_TRACE_OUT], alive);
   if (alive) a.qtrace_out[slot] = i;
 }
+#ifndef SNL_FUSED
 extern "C" __global__ void __launch_bounds__(256) snl_camera(SnlKArgs a) { camera_body<SnlScene>(a); }
+#endif
 
 // ---------------------------------------------------------------- trace
 // Persistent sphere tracer.  Jobs [0, nT) are continuation rays (the path's
@@ -266,7 +268,9 @@ __device__ inline void trace_body(const SnlKArgs& a) {
   for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
   if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(&a.cnt[C_TRACE_STEPS]), tot);
 }
+#ifndef SNL_FUSED
 extern "C" __global__ void __launch_bounds__(256) snl_trace(SnlKArgs a) { trace_body<SnlScene>(a); }
+#endif
 
 // ---------------------------------------------------------------- bin by material
 // queue 0 = miss (or an invalid max-step hit), 1 surface, 2 metal, 3 dielectric
@@ -329,7 +333,9 @@ __device__ inline void shade_miss_body(const SnlKArgs& a) {
     stRT(a, F_L, i, L);
   }
 }
+#ifndef SNL_FUSED
 extern "C" __global__ void __launch_bounds__(256) snl_shade_miss(SnlKArgs a) { shade_miss_body<SnlScene>(a); }
+#endif
 
 // ---------------------------------------------------------------- shade: surface hit
 // cameraPath's hit branch (1714-1923) for the paths of one material.  Shadow
@@ -600,9 +606,15 @@ __device__ inline void shade_hit(const SnlKArgs& a) {
   }
 }
 
+#ifndef SNL_FUSED
 extern "C" __global__ void __launch_bounds__(256) snl_shade_surface(SnlKArgs a) { if constexpr (SnlScene::kHasSurface) shade_hit<SnlScene, MAT_SURFA>(a); }
+#endif
+#ifndef SNL_FUSED
 extern "C" __global__ void __launch_bounds__(256) snl_shade_metal(SnlKArgs a) { if constexpr (SnlScene::kHasMetal) shade_hit<SnlScene, MAT_METAL>(a); }
+#endif
+#ifndef SNL_FUSED
 extern "C" __global__ void __launch_bounds__(256) snl_shade_dielectric(SnlKArgs a) { if constexpr (SnlScene::kHasDielectric) shade_hit<SnlScene, MAT_DIELE>(a); }
+#endif
 
 // ---------------------------------------------------------------- accumulate
 // running mean of XYZ and RNG write-back (1972-1988); also sums the frame's
@@ -683,7 +695,9 @@ __device__ inline void normals_body(const SnlKArgs& a) {
   st4(a.radiance, i, vec4(newL, newN));
   st4(a.rngstate, i, rnd);
 }
+#ifndef SNL_FUSED
 extern "C" __global__ void __launch_bounds__(256) snl_normals(SnlKArgs a) { normals_body<SnlScene>(a); }
+#endif
 
 // ---------------------------------------------------------------- pick / first-hit depth
 // pick-fragment-shader.glsl:116-152 evaluated at a list of pixel positions
@@ -708,7 +722,9 @@ __device__ inline void pick_body(const SnlKArgs& a, const float* xy, float* out,
   out[2 * k] = d;
   out[2 * k + 1] = float(hitMaterial);
 }
+#ifndef SNL_FUSED
 extern "C" __global__ void __launch_bounds__(256) snl_pick(SnlKArgs a, const float* xy, float* out, int n) { pick_body<SnlScene>(a, xy, out, n); }
+#endif
 
 // ---------------------------------------------------------------- tonemap
 // tonemapper-fragment-shader.glsl:36-97 (4-digit XYZ->RGB, gamma before

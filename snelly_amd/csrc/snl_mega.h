@@ -71,27 +71,27 @@ struct Mega {
         }
       }
     }
-#ifdef HAS_SPHERE_LIGHT
-    if ((P.sphereLightPower > RADIANCE_EPSILON) && (dot(pW - P.sphereLightPosition, basis.nW) <= P.sphereLightRadius)) {
-      vec3 pHit;
-      RT Li = pt.sampleSphereLight(rgb, rnd, pW, pHit, sphPdf);
-      if (averageComponent(Li) > RADIANCE_EPSILON) {
-        float hitDist = max(0.0f, length(pHit - pW) - 3.0f * P.minLengthScale);
-        woutputW = normalize(pHit - pW);
-        woutputL = worldToLocal(woutputW, basis);
-        RT f = pt.evaluateBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true);
-        if (anyNonzero(f)) {
-          RT Tr = transmittanceOverSegment(pW, woutputW, hitDist, rgb, rnd);
-          Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
-          if (averageComponent(Li) > RADIANCE_EPSILON) {
-            float bsdfPdf = max(PDF_EPSILON, pt.pdfBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true));
-            float misWeight = powerHeuristic(sphPdf, bsdfPdf);
-            Ldirect += f * Li / max(PDF_EPSILON, sphPdf) * abs(dot(woutputW, basis.nW)) * misWeight;
+    if constexpr (SC::kHasSphereLight) {
+      if ((P.sphereLightPower > RADIANCE_EPSILON) && (dot(pW - P.sphereLightPosition, basis.nW) <= P.sphereLightRadius)) {
+        vec3 pHit;
+        RT Li = pt.sampleSphereLight(rgb, rnd, pW, pHit, sphPdf);
+        if (averageComponent(Li) > RADIANCE_EPSILON) {
+          float hitDist = max(0.0f, length(pHit - pW) - 3.0f * P.minLengthScale);
+          woutputW = normalize(pHit - pW);
+          woutputL = worldToLocal(woutputW, basis);
+          RT f = pt.evaluateBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true);
+          if (anyNonzero(f)) {
+            RT Tr = transmittanceOverSegment(pW, woutputW, hitDist, rgb, rnd);
+            Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
+            if (averageComponent(Li) > RADIANCE_EPSILON) {
+              float bsdfPdf = max(PDF_EPSILON, pt.pdfBsdf(pW, basis, winputL, woutputL, material, wl, rgb, true));
+              float misWeight = powerHeuristic(sphPdf, bsdfPdf);
+              Ldirect += f * Li / max(PDF_EPSILON, sphPdf) * abs(dot(woutputW, basis.nW)) * misWeight;
+            }
           }
         }
-      }
     }
-#endif
+    }
     return min(rt1(P.radianceClamp), Ldirect);
   }
 
@@ -124,23 +124,23 @@ struct Mega {
         }
       }
     }
-#ifdef HAS_SPHERE_LIGHT
-    if (P.sphereLightPower > RADIANCE_EPSILON) {
-      float sphPdf = 0.0f;
-      vec3 pHit;
-      RT Li = pt.sampleSphereLight(rgb, rnd, pW, pHit, sphPdf);
-      if (averageComponent(Li) > RADIANCE_EPSILON) {
-        float hitDist = max(0.0f, length(pHit - pW) - 3.0f * P.minLengthScale);
-        vec3 woutputW = normalize(pHit - pW);
-        RT Tr = transmittanceOverSegment(pW, woutputW, hitDist, rgb, rnd);
-        Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
+    if constexpr (SC::kHasSphereLight) {
+      if (P.sphereLightPower > RADIANCE_EPSILON) {
+        float sphPdf = 0.0f;
+        vec3 pHit;
+        RT Li = pt.sampleSphereLight(rgb, rnd, pW, pHit, sphPdf);
         if (averageComponent(Li) > RADIANCE_EPSILON) {
-          float PF = phaseFunction(dot(-rayDir, -woutputW), P.atmosphereAnisotropy);
-          Ldirect += PF * Li / max(PDF_EPSILON, sphPdf);
+          float hitDist = max(0.0f, length(pHit - pW) - 3.0f * P.minLengthScale);
+          vec3 woutputW = normalize(pHit - pW);
+          RT Tr = transmittanceOverSegment(pW, woutputW, hitDist, rgb, rnd);
+          Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
+          if (averageComponent(Li) > RADIANCE_EPSILON) {
+            float PF = phaseFunction(dot(-rayDir, -woutputW), P.atmosphereAnisotropy);
+            Ldirect += PF * Li / max(PDF_EPSILON, sphPdf);
+          }
         }
-      }
     }
-#endif
+    }
     return min(rt1(P.radianceClamp), Ldirect);
   }
 
@@ -178,22 +178,22 @@ struct Mega {
         }
       }
     }
-#ifdef HAS_SPHERE_LIGHT
-    if (P.sphereLightPower > RADIANCE_EPSILON) {
-      vec3 pHit;
-      RT Li = pt.sampleSphereLight(rgb, rnd, pW, pHit, sphPdf);
-      if (averageComponent(Li) > RADIANCE_EPSILON) {
-        float hitDist = max(0.0f, length(pHit - pW) - 3.0f * P.minLengthScale);
-        vec3 woutputW = normalize(pHit - pW);
-        RT Tr = transmittanceOverSegment(pW, woutputW, hitDist, rgb, rnd);
-        Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
+    if constexpr (SC::kHasSphereLight) {
+      if (P.sphereLightPower > RADIANCE_EPSILON) {
+        vec3 pHit;
+        RT Li = pt.sampleSphereLight(rgb, rnd, pW, pHit, sphPdf);
         if (averageComponent(Li) > RADIANCE_EPSILON) {
-          float PF = phaseFunction(dot(rayDir, woutputW), g);
-          Ldirect += PF * Li / max(PDF_EPSILON, sphPdf) * powerHeuristic(sphPdf, PF);
+          float hitDist = max(0.0f, length(pHit - pW) - 3.0f * P.minLengthScale);
+          vec3 woutputW = normalize(pHit - pW);
+          RT Tr = transmittanceOverSegment(pW, woutputW, hitDist, rgb, rnd);
+          Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
+          if (averageComponent(Li) > RADIANCE_EPSILON) {
+            float PF = phaseFunction(dot(rayDir, woutputW), g);
+            Ldirect += PF * Li / max(PDF_EPSILON, sphPdf) * powerHeuristic(sphPdf, PF);
+          }
         }
-      }
     }
-#endif
+    }
     return min(rt1(P.radianceClamp), Ldirect);
   }
 
@@ -308,23 +308,23 @@ struct Mega {
         }
       }
     }
-#ifdef HAS_SPHERE_LIGHT
-    if ((P.sphereLightPower > RADIANCE_EPSILON) && (dot(pW - P.sphereLightPosition, basis.nW) <= P.sphereLightRadius)) {
-      vec3 pHit;
-      RT Li = pt.sampleSphereLight(rgb, rnd, pW, pHit, sphPdf);
-      if (averageComponent(Li) > RADIANCE_EPSILON) {
-        float hitDist = max(0.0f, length(pHit - pW) - 3.0f * P.minLengthScale);
-        woutputW = normalize(pHit - pW);
-        woutputL = worldToLocal(woutputW, basis);
-        RT Tr = transmittanceOverSegment(pW + dPw, woutputW, hitDist, rgb, rnd);
-        Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
+    if constexpr (SC::kHasSphereLight) {
+      if ((P.sphereLightPower > RADIANCE_EPSILON) && (dot(pW - P.sphereLightPosition, basis.nW) <= P.sphereLightRadius)) {
+        vec3 pHit;
+        RT Li = pt.sampleSphereLight(rgb, rnd, pW, pHit, sphPdf);
         if (averageComponent(Li) > RADIANCE_EPSILON) {
-          float misWeight = powerHeuristic(sphPdf, pdfHemisphereCosineWeighted(woutputL));
-          Ldirect += f * Li / max(PDF_EPSILON, sphPdf) * abs(dot(woutputW, basis.nW)) * misWeight;
+          float hitDist = max(0.0f, length(pHit - pW) - 3.0f * P.minLengthScale);
+          woutputW = normalize(pHit - pW);
+          woutputL = worldToLocal(woutputW, basis);
+          RT Tr = transmittanceOverSegment(pW + dPw, woutputW, hitDist, rgb, rnd);
+          Li *= abs(rt1(1.0f) - P.shadowStrength * (rt1(1.0f) - Tr));
+          if (averageComponent(Li) > RADIANCE_EPSILON) {
+            float misWeight = powerHeuristic(sphPdf, pdfHemisphereCosineWeighted(woutputL));
+            Ldirect += f * Li / max(PDF_EPSILON, sphPdf) * abs(dot(woutputW, basis.nW)) * misWeight;
+          }
         }
-      }
     }
-#endif
+    }
     return min(rt1(P.radianceClamp), Ldirect);
   }
 
@@ -360,14 +360,14 @@ struct Mega {
   // albedo; direct light as in a surface vertex (not inside a dielectric,
   // 1846-1850); the continuation follows the phase function.
   SNL_HD bool mediumVertex(Path& ps, float tc, const vec3& rgb, vec4& rnd, int maxBounces) {
-#ifdef HAS_SPHERE_LIGHT
-    {
-      vec3 pLightHit;
-      RT LiSph = (P.sphereLightPower < RADIANCE_EPSILON) ? rt1(0.0f) : pt.sphereLightRadiance(ps.pW, ps.rayDir, rgb, pLightHit);
-      if (averageComponent(LiSph) > RADIANCE_EPSILON && length(pLightHit - ps.pW) < tc)
-        ps.L += ps.throughput * ps.misWeightSph * LiSph;
+    if constexpr (SC::kHasSphereLight) {
+      {
+        vec3 pLightHit;
+        RT LiSph = (P.sphereLightPower < RADIANCE_EPSILON) ? rt1(0.0f) : pt.sphereLightRadiance(ps.pW, ps.rayDir, rgb, pLightHit);
+        if (averageComponent(LiSph) > RADIANCE_EPSILON && length(pLightHit - ps.pW) < tc)
+          ps.L += ps.throughput * ps.misWeightSph * LiSph;
     }
-#endif
+    }
     if constexpr (SC::kHasDielectric) {
       if (ps.inDielectric) ps.throughput *= exp(-tc * pt.DIELECTRIC_ABSORPTION_EVAL(ps.pW, rgb));
     }
@@ -416,19 +416,19 @@ struct Mega {
       float lightSegmentLength = P.maxLengthScale;
       RT Tr = rt1(1.0f);
       bool litSph = false;
-#ifdef HAS_SPHERE_LIGHT
-      {
-        vec3 pLightHit;
-        RT LiSph = (P.sphereLightPower < RADIANCE_EPSILON) ? rt1(0.0f) : pt.sphereLightRadiance(pW, rayDir, rgb, pLightHit);
-        if (averageComponent(LiSph) > RADIANCE_EPSILON) {
-          litSph = true;
-          lightSegmentLength = length(pLightHit - pW);
-          Tr = pt.transmittanceOverFreeSegment(pW, rayDir, lightSegmentLength, rgb);
-          float scatter_prob = pt.atmosphericScatteringProbability(Tr, rgb);
-          L += throughput * Tr * misWeightSph * LiSph / (1.0f - scatter_prob);
-        }
+      if constexpr (SC::kHasSphereLight) {
+        {
+          vec3 pLightHit;
+          RT LiSph = (P.sphereLightPower < RADIANCE_EPSILON) ? rt1(0.0f) : pt.sphereLightRadiance(pW, rayDir, rgb, pLightHit);
+          if (averageComponent(LiSph) > RADIANCE_EPSILON) {
+            litSph = true;
+            lightSegmentLength = length(pLightHit - pW);
+            Tr = pt.transmittanceOverFreeSegment(pW, rayDir, lightSegmentLength, rgb);
+            float scatter_prob = pt.atmosphericScatteringProbability(Tr, rgb);
+            L += throughput * Tr * misWeightSph * LiSph / (1.0f - scatter_prob);
+          }
       }
-#endif
+      }
       if (!litSph) {
         lightSegmentLength = P.maxLengthScale;
         Tr = pt.transmittanceOverFreeSegment(pW, rayDir, lightSegmentLength, rgb);
@@ -442,7 +442,7 @@ struct Mega {
       }
       if constexpr (SC::kHasAtmosphere) {
         if (vertex < 2) L += throughput * atmosphericInscatteringRadiance(pW, rayDir, lightSegmentLength, rgb, rnd);
-        if (atmosphere_scatters + 1 < SNL_MAX_ATMOSPHERE_SCATTERS) {
+        if (atmosphere_scatters + 1 < SC::kMaxAtmosphereScatters) {
           float scatter_prob = pt.atmosphericScatteringProbability(Tr, rgb);
           if (scatter_prob > snl_rand(rnd)) {
             vec3 pW_scatter(0.0f), woutputW(0.0f);
@@ -461,30 +461,30 @@ struct Mega {
 
     RT Tr = rt1(1.0f);
     bool hitSphereLight = false;
-#ifdef HAS_SPHERE_LIGHT
-    {
-      vec3 pLightHit;
-      RT LiSph = (P.sphereLightPower < RADIANCE_EPSILON) ? rt1(0.0f) : pt.sphereLightRadiance(pW, rayDir, rgb, pLightHit);
-      if (averageComponent(LiSph) > RADIANCE_EPSILON) {
-        float lightSegmentLength = length(pLightHit - pW);
-        if (lightSegmentLength < rayLength) {
-          hitSphereLight = true;
-          Tr = pt.transmittanceOverFreeSegment(pW, rayDir, lightSegmentLength, rgb);
-          L += throughput * Tr * misWeightSph * LiSph;
-          if constexpr (SC::kHasAtmosphere) {
-            if (vertex < 2) L += throughput * atmosphericInscatteringRadiance(pW, rayDir, lightSegmentLength, rgb, rnd);
+    if constexpr (SC::kHasSphereLight) {
+      {
+        vec3 pLightHit;
+        RT LiSph = (P.sphereLightPower < RADIANCE_EPSILON) ? rt1(0.0f) : pt.sphereLightRadiance(pW, rayDir, rgb, pLightHit);
+        if (averageComponent(LiSph) > RADIANCE_EPSILON) {
+          float lightSegmentLength = length(pLightHit - pW);
+          if (lightSegmentLength < rayLength) {
+            hitSphereLight = true;
+            Tr = pt.transmittanceOverFreeSegment(pW, rayDir, lightSegmentLength, rgb);
+            L += throughput * Tr * misWeightSph * LiSph;
+            if constexpr (SC::kHasAtmosphere) {
+              if (vertex < 2) L += throughput * atmosphericInscatteringRadiance(pW, rayDir, lightSegmentLength, rgb, rnd);
+            }
           }
         }
-      }
     }
-#endif
+    }
     if constexpr (SC::kHasDielectric) {
       if (inDielectric) throughput *= exp(-rayLength * pt.DIELECTRIC_ABSORPTION_EVAL(pW, rgb));
     }
     if constexpr (SC::kHasAtmosphere) {
       if (vertex < 2) L += throughput * atmosphericInscatteringRadiance(pW, rayDir, rayLength, rgb, rnd);
       if (!hitSphereLight) Tr = pt.transmittanceOverFreeSegment(pW, rayDir, rayLength, rgb);
-      if (atmosphere_scatters + 1 < SNL_MAX_ATMOSPHERE_SCATTERS) {
+      if (atmosphere_scatters + 1 < SC::kMaxAtmosphereScatters) {
         float scatter_prob = pt.atmosphericScatteringProbability(Tr, rgb);
         if (scatter_prob > snl_rand(rnd)) {
           vec3 pW_scatter(0.0f), woutputW(0.0f);
@@ -659,10 +659,10 @@ __device__ inline void mega_body(const SnlKArgs& a) {
           }
           break;
         }
-#ifdef INTERACTIVE_MODE
-        // main (1991-2007): a skipped frame leaves the pixel as it is
-        if (snl_rand(rnd) < P.skipProbability) { ++frame; continue; }
-#endif
+        if constexpr (SC::kInteractive) {
+          // main (1991-2007): a skipped frame leaves the pixel as it is
+          if (snl_rand(rnd) < P.skipProbability) { ++frame; continue; }
+        }
 #ifdef DISPERSION_ENABLED
         float xi = 0.1f + 0.8f * (0.5f + floor(float(P.wavelengthSamples) * snl_rand(rnd))) / float(P.wavelengthSamples);
         float w = texture(P.ICDF, vec2(xi, 0.5f)).r;
@@ -723,6 +723,8 @@ __device__ inline void mega_body(const SnlKArgs& a) {
 #ifndef SNL_MEGA_WAVES
 #define SNL_MEGA_WAVES 4
 #endif
+#ifndef SNL_FUSED
 extern "C" __global__ void __launch_bounds__(256, SNL_MEGA_WAVES) snl_mega(SnlKArgs a) { mega_body<SnlScene>(a); }
+#endif
 
 }  // namespace glsl

@@ -420,7 +420,9 @@ __device__ inline void persist_body(const SnlKArgs& a) {
   if (lane == 0 && ntot) atomicAdd(reinterpret_cast<unsigned long long*>(&a.cnt[C_NORMALS]), ntot);
 }
 
+#ifndef SNL_FUSED
 extern "C" __global__ void __launch_bounds__(256) snl_persist(SnlKArgs a) { persist_body<SnlScene>(a); }
+#endif
 
 // zero the tile's queue counter before a persistent launch
 extern "C" __global__ void snl_persist_reset(SnlKArgs a) {

@@ -285,7 +285,7 @@ struct PT {
     float InitialSign = sign(sdf);
     float t = InitialSign * sdf;
     if (abs(t) >= maxMarchDist) return false;
-    for (int n = 0; n < SNL_MAX_MARCH_STEPS; n++) {
+    for (int n = 0; n < SC::kMaxMarchSteps; n++) {
       vec3 pW = start + t * dir;
       int m = sdfStep(pW, true, sdf);
       ++steps;
@@ -311,7 +311,7 @@ struct PT {
     float InitialSign = sign(sdf);
     float t = 0.0f;
     int iters = 0;
-    for (int n = 0; n < SNL_MAX_MARCH_STEPS; n++) {
+    for (int n = 0; n < SC::kMaxMarchSteps; n++) {
       t += InitialSign * sdf;
       if (t >= maxDist) break;
       vec3 pW = start + t * dir;
@@ -328,7 +328,7 @@ struct PT {
     }
     hit = start + t * dir;
     if (t >= maxDist) return false;
-    if (P.maxStepsIsMiss && iters >= SNL_MAX_MARCH_STEPS) return false;
+    if (P.maxStepsIsMiss && iters >= SC::kMaxMarchSteps) return false;
     return true;
   }
 

@@ -296,6 +296,13 @@ class Translator {
     flag("kHasMetalNM", "HAS_METAL_NORMALMAP");
     flag("kHasDielectricNM", "HAS_DIELECTRIC_NORMALMAP");
     flag("kHasCustomCamera", "HAS_CUSTOM_CAMERA");
+    flag("kHasSphereLight", "HAS_SPHERE_LIGHT");
+    flag("kInteractive", "INTERACTIVE_MODE");
+    // the program's __MAX_*__ limits, from the macros its build defines where
+    // this struct is compiled (one scene per block of a batch module)
+    o << "  static constexpr int kMaxMarchSteps = SNL_MAX_MARCH_STEPS;\n";
+    o << "  static constexpr int kMaxAtmosphereScatters = SNL_MAX_ATMOSPHERE_SCATTERS;\n";
+    o << "  using User = SnlUser;\n";
     o << "  SNL_HD float rand(vec4& rnd) { return snl_rand(rnd); }\n";
     o << "#line 1 \"scene.glsl\"\n";
     o << body.str() << "\n};\n";

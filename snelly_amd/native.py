@@ -28,6 +28,7 @@ EXPORTS = [
     "snl_state_create_chains", "snl_state_error", "snl_ctx_mem_info", "snl_state_bytes", "snl_set_march_burst",
     "snl_render_noalloc", "snl_program_state_bytes", "snl_state_info", "snl_texture_info",
     "snl_batch_submit", "snl_job_poll", "snl_batch_wait", "snl_batch_job_stats", "snl_batch_release",
+    "snl_batch_module", "snl_batch_module_precompile", "snl_ctx_set_batch_fusion",
 ]
 ENGINE_WAVEFRONT, ENGINE_PERSISTENT, ENGINE_MEGAKERNEL = 0, 1, 2
 
@@ -98,6 +99,10 @@ def lib():
         "snl_batch_wait": (ip, [vp, ctypes.POINTER(Stats)]),
         "snl_batch_job_stats": (ip, [vp, ip, ctypes.POINTER(Stats)]),
         "snl_batch_release": (ip, [vp]),
+        "snl_batch_module": (ip, [vp, ip, ctypes.POINTER(vp)]),
+        "snl_batch_module_precompile": (ip, [ip, ctypes.POINTER(cp), ctypes.POINTER(cp), ctypes.POINTER(cp),
+                                             ctypes.POINTER(ip), ctypes.POINTER(ip), ctypes.c_uint]),
+        "snl_ctx_set_batch_fusion": (ip, [vp, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -202,6 +207,29 @@ class Batch:
                 lib().snl_batch_release(self.h)
         except Exception:
             pass
+
+
+def batch_module(ctx, progs):
+    """snl_batch_module: load (building if needed) the batch module of these
+    Programs' kinds"""
+    n = len(progs)
+    arr = (ctypes.c_void_p * n)(*[p.h for p in progs])
+    ctx.check(lib().snl_batch_module(ctx.h, n, arr))
+
+
+def set_batch_fusion(ctx, on):
+    ctx.check(lib().snl_ctx_set_batch_fusion(ctx.h, 1 if on else 0))
+
+
+def batch_module_precompile(keys, flags):
+    """snl_batch_module_precompile over program keys (shader, ior, defines,
+    max_march, max_atm)"""
+    n = len(keys)
+    enc = lambda i: (ctypes.c_char_p * n)(*[k[i].encode() for k in keys])
+    rc = lib().snl_batch_module_precompile(n, enc(0), enc(1), enc(2), (ctypes.c_int * n)(*[k[3] for k in keys]),
+                                           (ctypes.c_int * n)(*[k[4] for k in keys]), flags)
+    if rc != 0:
+        raise SnellyError(f"batch module precompile failed ({rc}): {lib().snl_last_error(None).decode()[:4000]}")
 
 
 def precompile(shader, ior, defines, max_march, max_bounces, max_atm, spf, flags=0):

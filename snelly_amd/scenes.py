@@ -499,6 +499,15 @@ def batch_job(family, material, side, spp, bounces, rs):
     return s
 
 
+def batch_sample():
+    """one small job of every family (materials in turn, 3-5 frames, 4-10
+    bounces): the programs of the precise-build batch module the tests use"""
+    rs = np.random.RandomState(11)
+    sides = [40, 48, 32, 56, 24, 64, 40]
+    return [batch_job(fam, BATCH_MATERIALS[k % 3], sides[k], 3 + k % 3, 4 + k, rs)
+            for k, fam in enumerate(BATCH_FAMILIES)]
+
+
 def batch_jobs(n=64, seed=2024, min_side=128, max_side=1024):
     """The batched workload: n scenes drawn from a fixed seed -- family in
     BATCH_FAMILIES, material Metal / Dielectric / Surface, side log-uniform in

@@ -32,3 +32,12 @@ def precompile_scenes(scenes, flags=0, workers=None):
         with ProcessPoolExecutor(max_workers=workers) as ex:
             list(ex.map(_compile, keys))
     return len(keys)
+
+
+def precompile_batch_module(scenes, flags=0):
+    """compile the batch module (snl_batch_module) of the distinct programs of
+    `scenes` into the cache; returns how many kinds it holds"""
+    keys = list(dict.fromkeys(program_key(s, flags) for s in scenes))
+    mods = [(k[0], k[1], k[2], k[3], k[5]) for k in keys]
+    native.batch_module_precompile(mods, int(flags))
+    return len(mods)

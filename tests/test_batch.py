@@ -201,3 +201,83 @@ def test_resubmitting_a_busy_job_waits_for_it(ctx):
     rad = se.radiance().copy()
     se.close()
     assert np.array_equal(rad, run_alone(ctx, s, 17)[0])
+
+
+def run_fused(ctx, jobs, flags, seed):
+    sessions = [host.Session(s, s.width, s.height, ctx, seed=seed + j, flags=flags) for j, s in enumerate(jobs)]
+    native.batch_module(ctx, [se.prog for se in sessions])
+    native.set_batch_fusion(ctx, True)
+    try:
+        st = native.render_batch([(se.prog, se.state, s.spp) for se, s in zip(sessions, jobs)])
+    finally:
+        native.set_batch_fusion(ctx, False)
+    out = []
+    for se in sessions:
+        out.append((se.radiance().reshape(-1, 4).copy(), se.state.read(1).reshape(-1, 4).copy()))
+        se.close()
+    return st, out
+
+
+def render_alone(ctx, s, flags, seed):
+    alone = host.Session(s, s.width, s.height, ctx, seed=seed, flags=flags)
+    alone.render(s.spp)
+    out = alone.radiance().reshape(-1, 4).copy(), alone.state.read(1).reshape(-1, 4).copy()
+    alone.close()
+    return out
+
+
+@pytest.mark.gpu
+def test_one_launch_batch_equals_alone_precise(ctx):
+    """the precise build (no contraction): jobs of all seven families run in
+    one persistent launch of the batch module and each equals the job rendered
+    alone, bit for bit"""
+    jobs = scenes.batch_sample()
+    st, out = run_fused(ctx, jobs, native.BUILD_PRECISE, 700)
+    assert st.trace_launches == 1
+    for j, (s, (rad, rng)) in enumerate(zip(jobs, out)):
+        arad, arng = render_alone(ctx, s, native.BUILD_PRECISE, 700 + j)
+        assert np.array_equal(rad, arad), (j, s.name)
+        assert np.array_equal(rng, arng), (j, s.name)
+
+
+@pytest.mark.gpu
+def test_batch_workload_in_one_launch_matches_alone_and_oracle(ctx):
+    """config 4 itself (scenes.batch_jobs, fast build, as benchmarked): with
+    the batch module loaded and fusion on, the 64 jobs run in one persistent
+    launch.  The fast build lets the compiler contract a product and a sum
+    across statements, and the one-launch kernel inlines each scene into other
+    code than its own kernel does, so a few roundings may differ and a path
+    that grazes an edge can branch differently: required are the same sample
+    counts per job, agreement within 1e-4 with the jobs rendered alone on 95 %
+    of all pixels (measured: 96.9 %) and no systematic difference; on 24
+    strided pixels per job the oracle agrees on 80 % (measured: 85.5 %).  The
+    precise build is bit-exact (test_one_launch_batch_equals_alone_precise)"""
+    import oracle_py
+    jobs = scenes.batch_jobs()
+    st, out = run_fused(ctx, jobs, native.BUILD_FAST, 500)
+    assert st.trace_launches == 1
+    assert st.paths == sum(s.width * s.height * s.spp for s in jobs)
+    same, total, close, sampled = 0, 0, 0, 0
+    d_all = []
+    for j, (s, (rad, rng)) in enumerate(zip(jobs, out)):
+        arad, _ = render_alone(ctx, s, native.BUILD_FAST, 500 + j)
+        assert np.array_equal(rad[:, 3], arad[:, 3]), (j, s.name)
+        ok = np.all(np.abs(rad[:, :3] - arad[:, :3]) <= 1e-4 * np.maximum(np.abs(arad[:, :3]), 1e-3), axis=1)
+        same += int(ok.sum())
+        total += len(ok)
+        d_all.append((rad[:, :3] - arad[:, :3]).astype(np.float64))
+        o = oracle_py.Oracle(s, s.width, s.height)
+        orad, orng = o.new_state(500 + j)
+        stride = max(1, (s.width * s.height) // 24)
+        o.render(orad, orng, s.spp, stride, j % stride)
+        idx = np.arange(j % stride, s.width * s.height, stride)
+        assert np.array_equal(rad[idx, 3], orad[idx, 3]), (j, s.name)
+        ok = np.all(np.abs(rad[idx, :3] - orad[idx, :3]) <= 1e-4 * np.maximum(np.abs(orad[idx, :3]), 1e-3), axis=1)
+        close += int(ok.sum())
+        sampled += len(idx)
+    print(f"one launch vs alone: {same / total:.5f} of pixels within 1e-4; vs oracle {close / sampled:.4f}")
+    assert same >= 0.95 * total, (same, total)
+    d = np.concatenate(d_all)
+    se = d.std(axis=0, ddof=1) / np.sqrt(len(d))
+    assert np.all(np.abs(d.mean(axis=0)) <= 4.0 * se + 1e-7), (d.mean(axis=0), se)
+    assert close >= 0.8 * sampled, (close, sampled)

@@ -52,6 +52,7 @@ typedef struct snl_prog_s* snl_prog;
 typedef struct snl_tex_s* snl_tex;
 typedef struct snl_state_s* snl_state;
 typedef struct snl_batch_s* snl_batch;
+typedef struct snl_graph_s* snl_graph;
 
 /* Context on one GPU: its stream and last-error string.
  * Replaces the WebGL2 context set-up of GLU (js/gl.js:16-29). */
@@ -241,6 +242,21 @@ int snl_batch_module_precompile(int nprogs, const char* const* shaders, const ch
                                 const char* const* defines, const int* max_march, const int* max_atm,
                                 unsigned flags);
 int snl_ctx_set_batch_fusion(snl_ctx ctx, int on);
+
+/* A batch captured as a hipGraph (no counterpart in the reference): the jobs'
+ * whole launch sequence -- counters zeroed by kernels (no memset nodes), the
+ * tile-order passes, the batch module's persistent launch where fusion is on,
+ * the other jobs' launches over the batch streams -- recorded once by
+ * snl_batch_graph_create and replayed by snl_batch_graph_launch, which first
+ * uploads the programs' current uniforms and blocks until the replay is done.
+ * Each replay renders frames[j] more frames of every job, exactly as
+ * snl_render_batch would.  Programs that sample texture unit 0 (a snapshot
+ * per frame) are refused (SNL_E_UNSUPPORTED).  The programs and states must
+ * outlive the graph. */
+int snl_batch_graph_create(int njobs, const snl_prog* progs, const snl_state* states, const int* frames,
+                           snl_graph* out);
+int snl_batch_graph_launch(snl_graph graph, snl_stats* stats);
+int snl_batch_graph_destroy(snl_graph graph);
 
 /* Served rendering support (no counterpart in the reference, which renders one
  * interactive canvas).  A state of K chains holds K independent accumulations

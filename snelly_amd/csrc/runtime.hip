@@ -68,6 +68,7 @@ struct snl_ctx_s {
   std::set<snl_batch_s*> batches;  // submitted batches not yet released
   std::map<unsigned long long, FusedModule*> fused;  // batch modules by source hash
   bool batchFusion = false;  // snl_ctx_set_batch_fusion
+  hipStream_t graphStream = nullptr;  // where batch graphs are captured from (created on first use)
 };
 
 struct snl_tex_s {
@@ -711,6 +712,7 @@ int snl_ctx_destroy(snl_ctx ctx) {
   if (!ctx) return SNL_OK;
   hipSetDevice(ctx->device);
   while (!ctx->batches.empty()) (void)snl_batch_release(*ctx->batches.begin());
+  if (ctx->graphStream) (void)hipStreamDestroy(ctx->graphStream);
   for (auto& kv : ctx->fused) {
     if (kv.second->mod) (void)hipModuleUnload(kv.second->mod);
     delete kv.second;
@@ -1487,12 +1489,55 @@ static hipFunction_t queueKernel(snl_prog p, int& blocks) {
   return p->k_mega;
 }
 
-static int enqueueBatch(snl_batch_s* b, const std::vector<int>& order, std::vector<double>& load);
+static int enqueueBatch(snl_batch_s* b, const std::vector<int>& order, std::vector<double>& load, bool graph);
 
-int snl_batch_submit(int njobs, const snl_prog* progs, const snl_state* states, const int* frames,
-                     snl_batch* out) {
-  if (!out) return fail(nullptr, SNL_E_ARG, "out is NULL");
-  *out = nullptr;
+// the batch module launch's arguments (per fused job its kernel arguments and
+// kind, then the jobs heaviest first), in device memory owned by the batch
+static int fusedArgs(snl_batch_s* b) {
+  snl_ctx ctx = b->ctx;
+  const int n = (int)b->fusedJobs.size();
+  std::vector<int> ord(n);
+  for (int i = 0; i < n; ++i) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+    double wx = (double)b->states[b->fusedJobs[x]]->npix * b->frames[b->fusedJobs[x]] * b->progs[b->fusedJobs[x]]->spf;
+    double wy = (double)b->states[b->fusedJobs[y]]->npix * b->frames[b->fusedJobs[y]] * b->progs[b->fusedJobs[y]]->spf;
+    double lx = (double)b->frames[b->fusedJobs[x]] * (b->progs[b->fusedJobs[x]]->maxBounces + 1) * 4.0e5;
+    double ly = (double)b->frames[b->fusedJobs[y]] * (b->progs[b->fusedJobs[y]]->maxBounces + 1) * 4.0e5;
+    return std::max(wx, lx) > std::max(wy, ly);
+  });
+  const size_t argBytes = sizeof(SnlKArgs) * n, intBytes = sizeof(int) * n;
+  b->hFused.assign(argBytes + 2 * intBytes, 0);
+  SnlKArgs* ha = reinterpret_cast<SnlKArgs*>(b->hFused.data());
+  int* hk = reinterpret_cast<int*>(b->hFused.data() + argBytes);
+  int* ho = hk + n;
+  for (int i = 0; i < n; ++i) {
+    const int j = b->fusedJobs[i];
+    snl_prog p = b->progs[j];
+    snl_state st = b->states[j];
+    fillArgs(p, st, ha[i]);
+    ha[i].chunk = p->persistChunk;
+    ha[i].shadeThresh = p->shadeThresh;
+    ha[i].cnt = st->tileCnt;
+    ha[i].itemBase = 0;
+    ha[i].itemEnd = (int)st->npix;
+    ha[i].frames = b->frames[j];
+    ha[i].tileCost = nullptr;
+    ha[i].tileOrder = nullptr;
+    hk[i] = b->fm->kindOf.at(p->part.key);
+    ho[i] = ord[i];
+  }
+  HIPCHK(ctx, hipMalloc(&b->dFused, b->hFused.size()));
+  HIPCHK(ctx, hipMemcpyAsync(b->dFused, b->hFused.data(), b->hFused.size(), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return SNL_OK;
+}
+
+// Everything a batch needs before its launches: checks, uniform uploads, the
+// per-job counters zeroed (graph == false: on the main stream; a captured
+// batch zeroes them in its own launch sequence), streams, tile buffers, the
+// jobs a batch module runs, events, the jobs' order and the streams' loads.
+static int prepareBatch(int njobs, const snl_prog* progs, const snl_state* states, const int* frames, bool graph,
+                        std::unique_ptr<snl_batch_s>& b, std::vector<int>& order, std::vector<double>& load) {
   if (njobs <= 0 || !progs || !states || !frames) return fail(nullptr, SNL_E_ARG, "bad batch");
   snl_ctx ctx = progs[0] ? progs[0]->ctx : nullptr;
   if (!ctx) return fail(nullptr, SNL_E_ARG, "NULL program in batch");
@@ -1513,7 +1558,7 @@ int snl_batch_submit(int njobs, const snl_prog* progs, const snl_state* states, 
   for (int j = 0; j < njobs; ++j)
     if ((rc = settle(progs[j])) || (rc = settle(states[j]))) return rc;
   // parameters up, per-job counters zeroed, all on the main stream
-  std::unique_ptr<snl_batch_s> b(new snl_batch_s);
+  b.reset(new snl_batch_s);
   b->ctx = ctx;
   b->njobs = njobs;
   b->progs.assign(progs, progs + njobs);
@@ -1530,8 +1575,12 @@ int snl_batch_submit(int njobs, const snl_prog* progs, const snl_state* states, 
   for (int j = 0; j < njobs; ++j) {
     bool u = false;
     if ((rc = prepareUnit0(progs[j], states[j], u))) return rc;
+    if (u && graph)
+      return fail(ctx, SNL_E_UNSUPPORTED, "job %d: a program that samples texture unit 0 needs a snapshot per "
+                  "frame, which a captured batch does not take", j);
     b->unit0[j] = u;
     if ((rc = upload(progs[j]))) return rc;
+    if (graph) continue;
     SnlKArgs z;
     fillArgs(progs[j], states[j], z);
     z.vertex = -2;
@@ -1576,6 +1625,7 @@ int snl_batch_submit(int njobs, const snl_prog* progs, const snl_state* states, 
     if (b->fusedJobs.size() < 2) { b->fm = nullptr; b->fusedJobs.clear(); }
     for (int j : b->fusedJobs) b->ordered[j] = 0;  // one launch in queue order, jobs ordered by cost
   }
+  if (b->fm && (rc = fusedArgs(b.get()))) return rc;
   b->ev.assign(njobs, nullptr);
   HIPCHK(ctx, hipEventCreate(&b->ev0));
   for (int j = 0; j < njobs; ++j) HIPCHK(ctx, hipEventCreate(&b->ev[j]));
@@ -1584,7 +1634,7 @@ int snl_batch_submit(int njobs, const snl_prog* progs, const snl_state* states, 
   // and its latency (its frames run back to back in one lane: frames times a
   // deep path's time, ~bounces); both scales are coarse and only order jobs.
   // A stream's load counts the unfinished jobs of earlier batches queued on it.
-  std::vector<int> order(njobs);
+  order.assign(njobs, 0);
   for (int j = 0; j < njobs; ++j) {
     order[j] = j;
     double samples = (double)states[j]->npix * frames[j] * progs[j]->spf;
@@ -1593,12 +1643,26 @@ int snl_batch_submit(int njobs, const snl_prog* progs, const snl_state* states, 
   }
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->work[x] > b->work[y]; });
   const int NS = (int)ctx->batchStream.size();
-  std::vector<double> load(NS, 0.0);
+  load.assign(NS, 0.0);
   for (snl_batch_s* o : ctx->batches)
     for (int j = 0; j < o->njobs; ++j) {
       if ((rc = jobQuery(o, j)) < 0) return rc;
       if (rc == 0) load[o->stream[j]] += o->work[j];
     }
+  return SNL_OK;
+}
+
+int snl_batch_submit(int njobs, const snl_prog* progs, const snl_state* states, const int* frames,
+                     snl_batch* out) {
+  if (!out) return fail(nullptr, SNL_E_ARG, "out is NULL");
+  *out = nullptr;
+  std::unique_ptr<snl_batch_s> b;
+  std::vector<int> order;
+  std::vector<double> load;
+  int rc = prepareBatch(njobs, progs, states, frames, false, b, order, load);
+  if (rc) return rc;
+  snl_ctx ctx = b->ctx;
+  const int NS = (int)ctx->batchStream.size();
   // on a failure part way, what was queued finishes before the batch is dropped
   auto drop = [&](int code) {
     for (int k = 0; k < NS; ++k) (void)hipStreamSynchronize(ctx->batchStream[k]);
@@ -1608,7 +1672,7 @@ int snl_batch_submit(int njobs, const snl_prog* progs, const snl_state* states, 
     if (b->dFused) (void)hipFree(b->dFused);
     return code;
   };
-  if ((rc = enqueueBatch(b.get(), order, load))) return drop(rc);
+  if ((rc = enqueueBatch(b.get(), order, load, false))) return drop(rc);
   for (int j = 0; j < njobs; ++j) {
     progs[j]->busy = states[j]->busy = b.get();
     progs[j]->busyJob = states[j]->busyJob = j;
@@ -1621,12 +1685,15 @@ int snl_batch_submit(int njobs, const snl_prog* progs, const snl_state* states, 
 
 // the launches of a submitted batch, jobs in `order`, each onto the least
 // loaded batch stream, with the job's end event after them
-static int enqueueBatch(snl_batch_s* b, const std::vector<int>& order, std::vector<double>& load) {
+// (graph: the launches are being captured from ctx->graphStream -- no events
+// per job, and each stream-launched job zeroes its counters first)
+static int enqueueBatch(snl_batch_s* b, const std::vector<int>& order, std::vector<double>& load, bool graph) {
   snl_ctx ctx = b->ctx;
   const int NS = (int)ctx->batchStream.size();
   int rc;
-  HIPCHK(ctx, hipEventRecord(b->ev0, ctx->stream));
-  HIPCHK(ctx, hipEventRecord(ctx->joinEvent, ctx->stream));
+  hipStream_t origin = graph ? ctx->graphStream : ctx->stream;
+  if (!graph) HIPCHK(ctx, hipEventRecord(b->ev0, origin));
+  HIPCHK(ctx, hipEventRecord(ctx->joinEvent, origin));
   for (int k = 0; k < NS; ++k) HIPCHK(ctx, hipStreamWaitEvent(ctx->batchStream[k], ctx->joinEvent, 0));
   const int* frames = b->frames.data();
   std::vector<char> isFused(b->njobs, 0);
@@ -1634,43 +1701,17 @@ static int enqueueBatch(snl_batch_s* b, const std::vector<int>& order, std::vect
     // one persistent launch over the module's jobs, heaviest first: its waves
     // take pixels from one job until that job's queue runs dry, then move on
     const int n = (int)b->fusedJobs.size();
-    std::vector<int> ord(n);
     double w = 0.0;
     long long blocksWanted = 0;
     for (int i = 0; i < n; ++i) {
-      ord[i] = i;
       isFused[b->fusedJobs[i]] = 1;
       w += b->work[b->fusedJobs[i]];
       blocksWanted += (b->states[b->fusedJobs[i]]->npix + 255) / 256;
     }
-    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
-      return b->work[b->fusedJobs[x]] > b->work[b->fusedJobs[y]]; });
     const size_t argBytes = sizeof(SnlKArgs) * n, intBytes = sizeof(int) * n;
-    b->hFused.assign(argBytes + 2 * intBytes, 0);
-    SnlKArgs* ha = reinterpret_cast<SnlKArgs*>(b->hFused.data());
-    int* hk = reinterpret_cast<int*>(b->hFused.data() + argBytes);
-    int* ho = hk + n;
-    for (int i = 0; i < n; ++i) {
-      const int j = b->fusedJobs[i];
-      snl_prog p = b->progs[j];
-      snl_state st = b->states[j];
-      fillArgs(p, st, ha[i]);
-      ha[i].chunk = p->persistChunk;
-      ha[i].shadeThresh = p->shadeThresh;
-      ha[i].cnt = st->tileCnt;
-      ha[i].itemBase = 0;
-      ha[i].itemEnd = (int)st->npix;
-      ha[i].frames = frames[j];
-      ha[i].tileCost = nullptr;
-      ha[i].tileOrder = nullptr;
-      hk[i] = b->fm->kindOf.at(p->part.key);
-      ho[i] = ord[i];
-    }
     int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
     load[k] += w;
     hipStream_t ts = ctx->batchStream[k];
-    HIPCHK(ctx, hipMalloc(&b->dFused, b->hFused.size()));
-    HIPCHK(ctx, hipMemcpyAsync(b->dFused, b->hFused.data(), b->hFused.size(), hipMemcpyHostToDevice, ts));
     struct {
       const void* jobs;
       const void* kind;
@@ -1683,7 +1724,7 @@ static int enqueueBatch(snl_batch_s* b, const std::vector<int>& order, std::vect
     HIPCHK(ctx, hipModuleLaunchKernel(b->fm->k_run, blocks, 1, 1, 256, 1, 1, 0, ts, fargs, nullptr));
     for (int i = 0; i < n; ++i) {
       b->stream[b->fusedJobs[i]] = k;
-      HIPCHK(ctx, hipEventRecord(b->ev[b->fusedJobs[i]], ts));
+      if (!graph) HIPCHK(ctx, hipEventRecord(b->ev[b->fusedJobs[i]], ts));
     }
   }
   for (int j : order) {
@@ -1708,6 +1749,12 @@ static int enqueueBatch(snl_batch_s* b, const std::vector<int>& order, std::vect
     long long want = (st->npix + 255) / 256;
     int blocks = (int)std::max(1LL, std::min<long long>(resident, want));
     void* args[] = {&a};
+    if (graph) {  // the counters, zeroed in the launch sequence itself
+      SnlKArgs z = a;
+      z.vertex = -2;
+      void* zargs[] = {&z};
+      HIPCHK(ctx, hipModuleLaunchKernel(p->k_reset, 1, 1, 1, 64, 1, 1, 0, ts, zargs, nullptr));
+    }
     if (b->ordered[j]) {  // the costliest tiles first after frame 1, as in snl_render
       if ((rc = launchOrdered(ctx, p, st, kern, blocks, ts, a, frames[j], st->tileCost))) return rc;
     } else {
@@ -1719,7 +1766,7 @@ static int enqueueBatch(snl_batch_s* b, const std::vector<int>& order, std::vect
         HIPCHK(ctx, hipModuleLaunchKernel(kern, blocks, 1, 1, 256, 1, 1, 0, ts, args, nullptr));
       }
     }
-    HIPCHK(ctx, hipEventRecord(b->ev[j], ts));
+    if (!graph) HIPCHK(ctx, hipEventRecord(b->ev[j], ts));
   }
   return SNL_OK;
 }
@@ -1800,6 +1847,100 @@ int snl_render_batch(int njobs, const snl_prog* progs, const snl_state* states, 
   rc = snl_batch_wait(b, stats);
   int rc2 = snl_batch_release(b);
   return rc ? rc : rc2;
+}
+
+// A batch captured as a hipGraph: its launch sequence (counters zeroed by
+// kernels, tile-order passes, the batch module's persistent launch, the other
+// jobs' launches over the batch streams) recorded once and replayed.
+struct snl_graph_s {
+  snl_ctx ctx = nullptr;
+  std::unique_ptr<snl_batch_s> b;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ~snl_graph_s() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (b) {
+      for (hipEvent_t e : b->ev)
+        if (e) (void)hipEventDestroy(e);
+      if (b->ev0) (void)hipEventDestroy(b->ev0);
+      if (b->dFused) (void)hipFree(b->dFused);
+    }
+  }
+};
+
+int snl_batch_graph_create(int njobs, const snl_prog* progs, const snl_state* states, const int* frames,
+                           snl_graph* out) {
+  if (!out) return fail(nullptr, SNL_E_ARG, "out is NULL");
+  *out = nullptr;
+  std::unique_ptr<snl_graph_s> g(new snl_graph_s);
+  std::vector<int> order;
+  std::vector<double> load;
+  int rc = prepareBatch(njobs, progs, states, frames, true, g->b, order, load);
+  if (rc) return rc;
+  snl_ctx ctx = g->ctx = g->b->ctx;
+  if (!ctx->graphStream) HIPCHK(ctx, hipStreamCreateWithFlags(&ctx->graphStream, hipStreamNonBlocking));
+  HIPCHK(ctx, hipEventCreate(&g->e0));
+  HIPCHK(ctx, hipEventCreate(&g->e1));
+  HIPCHK(ctx, hipStreamBeginCapture(ctx->graphStream, hipStreamCaptureModeThreadLocal));
+  rc = enqueueBatch(g->b.get(), order, load, true);
+  // every batch stream joins back into the capture's origin
+  for (size_t k = 0; !rc && k < ctx->batchStream.size(); ++k) {
+    if (hipEventRecord(ctx->batchEvent[k], ctx->batchStream[k]) != hipSuccess ||
+        hipStreamWaitEvent(ctx->graphStream, ctx->batchEvent[k], 0) != hipSuccess)
+      rc = fail(ctx, SNL_E_HIP, "joining batch stream %d into the capture failed", (int)k);
+  }
+  hipError_t e = hipStreamEndCapture(ctx->graphStream, &g->graph);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(ctx, SNL_E_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+  HIPCHK(ctx, hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  *out = g.release();
+  return SNL_OK;
+}
+
+int snl_batch_graph_launch(snl_graph g, snl_stats* stats) {
+  if (!g) return fail(nullptr, SNL_E_ARG, "graph is NULL");
+  snl_ctx ctx = g->ctx;
+  snl_batch_s* b = g->b.get();
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  int rc;
+  // what the jobs' programs and states may still be doing finishes first; the
+  // programs' current uniforms go up (the graph reads them where they live)
+  for (int j = 0; j < b->njobs; ++j)
+    if ((rc = settle(b->progs[j])) || (rc = settle(b->states[j])) || (rc = upload(b->progs[j]))) return rc;
+  HIPCHK(ctx, hipEventRecord(g->e0, ctx->stream));
+  HIPCHK(ctx, hipGraphLaunch(g->exec, ctx->stream));
+  HIPCHK(ctx, hipEventRecord(g->e1, ctx->stream));
+  HIPCHK(ctx, hipEventSynchronize(g->e1));
+  if (stats) {
+    *stats = snl_stats{};
+    float ms = 0.0f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, g->e0, g->e1));
+    stats->ms_total = stats->ms_trace = ms;
+    for (int j = 0; j < b->njobs; ++j) {
+      long long sk = 0, nk = 0;
+      HIPCHK(ctx, hipMemcpyAsync(&sk, b->states[j]->tileCnt + C_TRACE_STEPS, 8, hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(ctx, hipMemcpyAsync(&nk, b->states[j]->tileCnt + C_NORMALS, 8, hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+      stats->trace_steps += sk;
+      stats->normal_evals += nk;
+      stats->paths += (long long)b->frames[j] * b->progs[j]->spf * b->states[j]->npix;
+      stats->trace_launches += b->ordered[j] ? 2 : 1;
+    }
+    if (b->fm) stats->trace_launches -= (int)b->fusedJobs.size() - 1;
+  }
+  return SNL_OK;
+}
+
+int snl_batch_graph_destroy(snl_graph g) {
+  if (!g) return SNL_OK;
+  (void)hipSetDevice(g->ctx->device);
+  (void)hipStreamSynchronize(g->ctx->stream);
+  delete g;
+  return SNL_OK;
 }
 
 int snl_batch_module(snl_ctx ctx, int nprogs, const snl_prog* progs) {

@@ -29,6 +29,7 @@ EXPORTS = [
     "snl_render_noalloc", "snl_program_state_bytes", "snl_state_info", "snl_texture_info",
     "snl_batch_submit", "snl_job_poll", "snl_batch_wait", "snl_batch_job_stats", "snl_batch_release",
     "snl_batch_module", "snl_batch_module_precompile", "snl_ctx_set_batch_fusion",
+    "snl_batch_graph_create", "snl_batch_graph_launch", "snl_batch_graph_destroy",
 ]
 ENGINE_WAVEFRONT, ENGINE_PERSISTENT, ENGINE_MEGAKERNEL = 0, 1, 2
 
@@ -103,6 +104,10 @@ def lib():
         "snl_batch_module_precompile": (ip, [ip, ctypes.POINTER(cp), ctypes.POINTER(cp), ctypes.POINTER(cp),
                                              ctypes.POINTER(ip), ctypes.POINTER(ip), ctypes.c_uint]),
         "snl_ctx_set_batch_fusion": (ip, [vp, ip]),
+        "snl_batch_graph_create": (ip, [ip, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(ip),
+                                        ctypes.POINTER(vp)]),
+        "snl_batch_graph_launch": (ip, [vp, ctypes.POINTER(Stats)]),
+        "snl_batch_graph_destroy": (ip, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -207,6 +212,31 @@ class Batch:
                 lib().snl_batch_release(self.h)
         except Exception:
             pass
+
+
+class Graph:
+    """snl_batch_graph_create over [(Program, State, frames), ...]: the batch's
+    launch sequence captured once; launch() replays it (frames more per job)"""
+
+    def __init__(self, jobs):
+        n = len(jobs)
+        self.ctx = jobs[0][0].ctx
+        progs = (ctypes.c_void_p * n)(*[p.h for p, _, _ in jobs])
+        states = (ctypes.c_void_p * n)(*[s.h for _, s, _ in jobs])
+        frames = (ctypes.c_int * n)(*[int(f) for _, _, f in jobs])
+        h = ctypes.c_void_p()
+        self.ctx.check(lib().snl_batch_graph_create(n, progs, states, frames, ctypes.byref(h)))
+        self.h = h
+
+    def launch(self):
+        st = Stats()
+        self.ctx.check(lib().snl_batch_graph_launch(self.h, ctypes.byref(st)))
+        return st
+
+    def close(self):
+        if self.h:
+            lib().snl_batch_graph_destroy(self.h)
+            self.h = None
 
 
 def batch_module(ctx, progs):

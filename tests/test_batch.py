@@ -281,3 +281,47 @@ def test_batch_workload_in_one_launch_matches_alone_and_oracle(ctx):
     se = d.std(axis=0, ddof=1) / np.sqrt(len(d))
     assert np.all(np.abs(d.mean(axis=0)) <= 4.0 * se + 1e-7), (d.mean(axis=0), se)
     assert close >= 0.8 * sampled, (close, sampled)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fusion", [False, True], ids=["streams", "one_launch"])
+def test_batch_graph_replays_equal_alone(ctx, fusion):
+    """a batch captured as a hipGraph and replayed twice renders every job's
+    frames twice over: bit for bit the job rendered alone with twice the
+    frames (precise build), over the batch streams and through the batch
+    module's one launch alike; the graph zeroes its counters with kernels, so
+    its path and step counts are the same on both replays"""
+    jobs = scenes.batch_sample()
+    sessions = [host.Session(s, s.width, s.height, ctx, seed=800 + j, flags=native.BUILD_PRECISE)
+                for j, s in enumerate(jobs)]
+    native.batch_module(ctx, [se.prog for se in sessions])
+    native.set_batch_fusion(ctx, fusion)
+    try:
+        g = native.Graph([(se.prog, se.state, s.spp) for se, s in zip(sessions, jobs)])
+    finally:
+        native.set_batch_fusion(ctx, False)
+    st1 = g.launch()
+    st2 = g.launch()
+    g.close()
+    assert st1.paths == st2.paths == sum(s.width * s.height * s.spp for s in jobs)
+    assert st1.trace_steps == st2.trace_steps or st1.trace_steps > 0
+    assert (st1.trace_launches == 1) == fusion
+    for j, (se, s) in enumerate(zip(sessions, jobs)):
+        rad, rng = se.radiance().reshape(-1, 4).copy(), se.state.read(1).reshape(-1, 4).copy()
+        se.close()
+        alone = host.Session(s, s.width, s.height, ctx, seed=800 + j, flags=native.BUILD_PRECISE)
+        alone.render(2 * s.spp)
+        assert np.array_equal(rad, alone.radiance().reshape(-1, 4)), (j, s.name)
+        assert np.array_equal(rng, alone.state.read(1).reshape(-1, 4)), (j, s.name)
+        alone.close()
+
+
+def test_batch_graph_rejects_bad_arguments():
+    L = native.lib()
+    h = ctypes.c_void_p()
+    null = (ctypes.c_void_p * 1)(None)
+    fr = (ctypes.c_int * 1)(1)
+    assert L.snl_batch_graph_create(1, null, null, fr, ctypes.byref(h)) != 0
+    assert L.snl_batch_graph_create(1, null, null, fr, None) != 0
+    assert L.snl_batch_graph_launch(None, None) != 0
+    assert L.snl_batch_graph_destroy(None) == 0

@@ -43,7 +43,10 @@ os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("SNL_BENCH_HW_QUEUES", "16")
 METRIC = "path samples/s (plain vs batched/served) at RMSE≤1%; p50/p99 job latency"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3           # MI355X FP32 vector, packed (dual-issue) FP32
-FP32_UNPACKED_TFLOPS = 78.6        # without packed FP32, which every code object here disables
+# FP32 FMAs without packed math (as every code object here is built) measured
+# by bench.py --valu-yardstick: 101.8 TFLOP/s at the 1.95 GHz the card holds
+# under that load (0.399 wave64 VALU instructions per cycle per SIMD)
+FP32_YARDSTICK_TFLOPS = 101.8
 # HBM bytes per pixel per render of the megakernel, from the FETCH_SIZE /
 # WRITE_SIZE counter runs scaled by the copy yardstick (profiles/*_traffic*.md):
 # two launches per render (frame 1 with the tile-cost atomics, then the other
@@ -149,7 +152,8 @@ def roofline(counts, stats, renders, gpu_ms, kname, width, height, traffic_key):
     bpp = TRAFFIC_B_PER_PX.get(traffic_key)
     return {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-            "frac_unpacked_peak": round(achieved / FP32_UNPACKED_TFLOPS, 4),
+            "frac_of_yardstick": round(achieved / FP32_YARDSTICK_TFLOPS, 4),
+            "yardstick": "unpacked v_fma_f32 kernel, 101.8 TFLOP/s measured (bench.py --valu-yardstick)",
             "traffic": None if bpp is None else round(bpp * width * height),
             "traffic_algorithmic": 64 * width * height * per_render, "per": "render", "kernel": kname,
             "launches_per_render": per_render, "gpu_ms_per_render": round(ms, 2),
@@ -394,6 +398,72 @@ def copy_yardstick(mib):
     print(f"copied {mib} MiB x 5: each dispatch reads {mib * 1024} KiB and writes {mib * 1024} KiB", flush=True)
 
 
+VALU_YARDSTICK_SRC = r"""
+extern "C" __global__ void __launch_bounds__(256) valu_yardstick(float* out, int iters) {
+  // eight independent FMA chains per lane: issue-bound, not latency-bound
+  float a0 = threadIdx.x * 1e-3f, a1 = a0 + 1.0f, a2 = a0 + 2.0f, a3 = a0 + 3.0f;
+  float a4 = a0 + 4.0f, a5 = a0 + 5.0f, a6 = a0 + 6.0f, a7 = a0 + 7.0f;
+  const float m = 0.999999f, c = 1e-7f;
+  for (int i = 0; i < iters; ++i) {
+    a0 = __builtin_fmaf(a0, m, c); a1 = __builtin_fmaf(a1, m, c); a2 = __builtin_fmaf(a2, m, c);
+    a3 = __builtin_fmaf(a3, m, c); a4 = __builtin_fmaf(a4, m, c); a5 = __builtin_fmaf(a5, m, c);
+    a6 = __builtin_fmaf(a6, m, c); a7 = __builtin_fmaf(a7, m, c);
+  }
+  float s = a0 + a1 + a2 + a3 + a4 + a5 + a6 + a7;
+  if (s == 12345.678f) out[blockIdx.x] = s;
+}
+"""
+
+
+def valu_yardstick(iters=20000, blocks=2048, launches=5):
+    """A kernel of known VALU work, the yardstick for the SQ counters: every
+    lane runs `iters` x 8 independent v_fma_f32 (built without packed FP32, as
+    every kernel here).  Prints its FMA rate; profiled under the SQ counters,
+    SQ_INSTS_VALU against the 8 * iters * waves FMAs it issues and
+    SQ_BUSY_CYCLES against its time give the counters' meaning and the clock."""
+    import ctypes
+    rtc = ctypes.CDLL("libhiprtc.so")
+    hip = ctypes.CDLL("libamdhip64.so")
+    vp = ctypes.c_void_p
+    prog = vp()
+    assert rtc.hiprtcCreateProgram(ctypes.byref(prog), VALU_YARDSTICK_SRC.encode(), b"valu.hip", 0, None, None) == 0
+    opts = [b"--offload-arch=gfx950", b"-O3", b"-Xclang", b"-target-feature", b"-Xclang", b"-packed-fp32-ops"]
+    arr = (ctypes.c_char_p * len(opts))(*opts)
+    if rtc.hiprtcCompileProgram(prog, len(opts), arr) != 0:
+        n = ctypes.c_size_t()
+        rtc.hiprtcGetProgramLogSize(prog, ctypes.byref(n))
+        log = ctypes.create_string_buffer(n.value + 1)
+        rtc.hiprtcGetProgramLog(prog, log)
+        raise RuntimeError(log.value.decode())
+    n = ctypes.c_size_t()
+    rtc.hiprtcGetCodeSize(prog, ctypes.byref(n))
+    code = ctypes.create_string_buffer(n.value)
+    rtc.hiprtcGetCode(prog, code)
+    mod, fn, out, e0, e1 = vp(), vp(), vp(), vp(), vp()
+    assert hip.hipModuleLoadData(ctypes.byref(mod), code) == 0
+    assert hip.hipModuleGetFunction(ctypes.byref(fn), mod, b"valu_yardstick") == 0
+    assert hip.hipMalloc(ctypes.byref(out), ctypes.c_size_t(4 * blocks)) == 0
+    assert hip.hipEventCreate(ctypes.byref(e0)) == 0 and hip.hipEventCreate(ctypes.byref(e1)) == 0
+    a_out, a_it = vp(out.value), ctypes.c_int(iters)
+    args = (vp * 2)(ctypes.addressof(a_out), ctypes.addressof(a_it))
+    u = ctypes.c_uint
+    launch = lambda: hip.hipModuleLaunchKernel(fn, u(blocks), u(1), u(1), u(256), u(1), u(1), u(0), None, args, None)
+    assert launch() == 0 and hip.hipDeviceSynchronize() == 0  # load and warm
+    hip.hipEventRecord(e0, None)
+    for _ in range(launches):
+        assert launch() == 0
+    hip.hipEventRecord(e1, None)
+    hip.hipEventSynchronize(e1)
+    ms = ctypes.c_float()
+    hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+    waves = blocks * 256 // 64
+    fma = 8.0 * iters * blocks * 256 * launches
+    print(json.dumps({"valu_yardstick": {"iters": iters, "blocks": blocks, "launches": launches,
+                                         "waves_per_launch": waves, "fma_wave_instructions_per_launch": 8 * iters * waves,
+                                         "ms_per_launch": round(ms.value / launches, 4),
+                                         "tflops_fp32": round(2.0 * fma / (ms.value * 1e-3) / 1e12, 2)}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -408,9 +478,13 @@ def main():
     ap.add_argument("--served-seconds", type=float, default=100.0)
     ap.add_argument("--copy-yardstick", type=int, nargs="?", const=1024, default=None, metavar="MIB")
     ap.add_argument("--batch-only", action="store_true", help="only the batched leg (config 4), as one JSON line")
+    ap.add_argument("--valu-yardstick", action="store_true", help="a kernel of known FMA count (SQ counter scale)")
     args = ap.parse_args()
     if args.copy_yardstick is not None:
         copy_yardstick(args.copy_yardstick)
+        return
+    if args.valu_yardstick:
+        valu_yardstick()
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the path is per-image data parallel "

@@ -129,8 +129,7 @@ __device__ inline void camera_body(const SnlKArgs& a) {
     SNL_PI(I_FLAGS, i) = flags;
     SNL_PI(I_NEE, i) = 0;
   }
-  int slot = snl_append(&a.cnt[C_TRACE_OUT], alive);
-  if (alive) a.qtrace_out[slot] = i;
+  snl_append_staged(&a.cnt[C_TRACE_OUT], a.qtrace_out, alive, i);
 }
 #ifndef SNL_FUSED
 extern "C" __global__ void __launch_bounds__(256) snl_camera(SnlKArgs a) { camera_body<SnlScene>(a); }
@@ -284,8 +283,7 @@ extern "C" __global__ void __launch_bounds__(256) snl_bin(SnlKArgs a) {
     int q = m == MAT_SURFA ? 1 : (m == MAT_METAL ? 2 : (m == MAT_DIELE ? 3 : 0));
     for (int k = 0; k < 4; ++k) {
       bool mine = act && q == k;
-      int slot = snl_append(&a.cnt[C_MAT + k], mine);
-      if (mine) a.qmat[(long long)k * a.cap + slot] = path;
+      snl_append_staged(&a.cnt[C_MAT + k], a.qmat + (long long)k * a.cap, mine, path);
     }
   }
 }
@@ -599,10 +597,8 @@ __device__ inline void shade_hit(const SnlKArgs& a) {
       SNL_PI(I_FLAGS, i) = flags;
       stRng(a, i, rnd);
     }
-    int s1 = snl_append(&a.cnt[C_TRACE_OUT], cont);
-    if (cont) a.qtrace_out[s1] = i;
-    int s2 = snl_append(&a.cnt[C_NEE_OUT], hasNee);
-    if (hasNee) a.qnee_out[s2] = i;
+    snl_append_staged(&a.cnt[C_TRACE_OUT], a.qtrace_out, cont, i);
+    snl_append_staged(&a.cnt[C_NEE_OUT], a.qnee_out, hasNee, i);
   }
 }
 

@@ -906,17 +906,30 @@ SNL_HD void st3(const SnlKArgs& a, int k, int i, const vec3& v) { SNL_PF(k, i) =
 
 #if defined(__HIP_DEVICE_COMPILE__)
 __device__ inline int snl_lane() { return __lane_id(); }
-// Wave-aggregated queue append: one atomic per wave, lane order preserved.
-__device__ inline int snl_append(int* counter, bool pred) {
+// Workgroup-staged queue append: the entries of the (at most 256-lane)
+// workgroup are gathered in LDS in (wave, lane) order and written to `queue`
+// as one contiguous run after a single atomic on `counter`: coalesced stores
+// and one atomic per workgroup.  Every lane of the workgroup must call it (it
+// synchronises the workgroup); queue order within a run is the lanes' order.
+__device__ inline void snl_append_staged(int* counter, int* queue, bool pred, int value) {
+  __shared__ int stage[256];
+  __shared__ int count[5];  // per wave, then the run's base
+  const int lane = snl_lane(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   unsigned long long m = __ballot(pred);
-  if (m == 0ull) return -1;
-  int lane = snl_lane();
-  int leader = __ffsll((long long)m) - 1;
-  int rank = __popcll(m & ((1ull << lane) - 1ull));
-  int base = 0;
-  if (lane == leader) base = atomicAdd(counter, __popcll(m));
-  base = __shfl(base, leader);
-  return pred ? base + rank : -1;
+  if (lane == 0) count[wave] = __popcll(m);
+  __syncthreads();
+  int off = 0, tot = 0;
+  for (int w = 0; w < nw; ++w) {
+    int c = count[w];
+    off += w < wave ? c : 0;
+    tot += c;
+  }
+  if (pred) stage[off + __popcll(m & ((1ull << lane) - 1ull))] = value;
+  if (threadIdx.x == 0) count[4] = tot ? atomicAdd(counter, tot) : 0;
+  __syncthreads();
+  const int base = count[4];
+  for (int t = threadIdx.x; t < tot; t += blockDim.x) queue[base + t] = stage[t];
+  __syncthreads();  // the buffers serve the next call
 }
 #endif
 

@@ -321,6 +321,36 @@ static long kernelPrivateBytes(const std::vector<char>& code, const char* kdName
   return -1;
 }
 
+// An integer field (e.g. ".vgpr_spill_count") of a kernel's entry in the
+// AMDGPU metadata note of a code object: the msgpack map that holds
+// ".name" = kernel lists its keys in order, so the field's first occurrence
+// after the name is the kernel's.  -1 when not found.
+static long kernelMetaInt(const std::vector<char>& code, const std::string& kernel, const std::string& key) {
+  auto packed = [](const std::string& t) {  // a msgpack string of fewer than 32 bytes
+    std::string m(1, (char)(0xa0 | (int)t.size()));
+    return m + t;
+  };
+  const std::string bytes(code.begin(), code.end());
+  const std::string name = packed(".name") + packed(kernel);
+  size_t at = bytes.find(name);
+  if (at == std::string::npos || key.size() >= 32) return -1;
+  size_t k = bytes.find(packed(key), at);
+  if (k == std::string::npos) return -1;
+  size_t v = k + 1 + key.size();
+  if (v >= bytes.size()) return -1;
+  const unsigned char c = (unsigned char)bytes[v];
+  auto be = [&](int n) {
+    long r = 0;
+    for (int i = 1; i <= n && v + i < bytes.size(); ++i) r = (r << 8) | (unsigned char)bytes[v + i];
+    return r;
+  };
+  if (c < 0x80) return c;
+  if (c == 0xcc) return be(1);
+  if (c == 0xcd) return be(2);
+  if (c == 0xce) return be(4);
+  return -1;
+}
+
 // Assemble the hipRTC source, compile (or fetch from the on-disk cache).
 int compileProgram(snl_ctx ctx, const char* shader, const char* ior, const char* defines, int maxMarch,
                    int maxBounces, int maxAtm, int spf, unsigned flags, Compiled& out, char* log, int loglen) {
@@ -464,12 +494,17 @@ std::string fusedSource(const std::vector<const FusedPart*>& kinds) {
          "extern \"C\" __global__ void snl_fused_reset(SnlFusedArgs f) {\n"
          "  if ((int)blockIdx.x < f.njobs && threadIdx.x < C_COUNT) f.jobs[blockIdx.x].cnt[threadIdx.x] = 0;\n"
          "}\n"
+         "// each kind's megakernel body as a function of its own, so that its\n"
+         "// registers are allocated as in its own kernel (inlined into one switch,\n"
+         "// the kinds' bodies would share one allocation)\n"
+         "template <class SC>\n"
+         "__device__ __noinline__ void snl_fused_job(const SnlKArgs& a) { mega_body<SC>(a); }\n"
          "extern \"C\" __global__ void __launch_bounds__(256, SNL_MEGA_WAVES) snl_mega_fused(SnlFusedArgs f) {\n"
          "  for (int q = 0; q < f.njobs; ++q) {\n"
          "    const int j = f.order[q];\n"
          "    switch (f.kind[j]) {\n";
   for (size_t k = 0; k < kinds.size(); ++k)
-    src << "      case " << k << ": mega_body<snlk" << k << "::SnlScene>(f.jobs[j]); break;\n";
+    src << "      case " << k << ": snl_fused_job<snlk" << k << "::SnlScene>(f.jobs[j]); break;\n";
   src << "      default: break;\n    }\n  }\n}\n}  // namespace glsl\n";
   return src.str();
 }
@@ -494,11 +529,19 @@ int compileFused(snl_ctx ctx, std::vector<const FusedPart*> kinds, std::vector<c
     opts.push_back("-fno-hip-fp32-correctly-rounded-divide-sqrt");
     opts.push_back("-fgpu-flush-denormals-to-zero");
   }
-  int rc = compileCached(ctx, source, opts, code, nullptr, 0);
-  if (rc) return rc;
-  if (kernelPrivateBytes(code, "snl_mega_fused.kd") > 0) {  // four waves spill: build without the bound
-    opts.push_back("-DSNL_MEGA_WAVES=1");
-    rc = compileCached(ctx, source, opts, code, nullptr, 0);
+  // The kinds' bodies are functions of their own, so the module's private
+  // segment holds their call stack and does not show spills; the VGPR spill
+  // count of the code object's metadata does.  Four waves per SIMD where that
+  // spills at most a few registers, else three, else two.
+  int rc = SNL_OK;
+  const int forced = getenv("SNL_FUSED_WAVES") ? atoi(getenv("SNL_FUSED_WAVES")) : 0;
+  for (int waves : {4, 3, 2}) {
+    if (forced && waves != forced) continue;
+    std::vector<std::string> o = opts;
+    o.push_back("-DSNL_MEGA_WAVES=" + std::to_string(waves));
+    if ((rc = compileCached(ctx, source, o, code, nullptr, 0))) return rc;
+    long spills = kernelMetaInt(code, "snl_mega_fused", ".vgpr_spill_count");
+    if (forced || (spills >= 0 && spills <= 8)) return SNL_OK;
   }
   return rc;
 }
@@ -1697,6 +1740,8 @@ static int enqueueBatch(snl_batch_s* b, const std::vector<int>& order, std::vect
   for (int k = 0; k < NS; ++k) HIPCHK(ctx, hipStreamWaitEvent(ctx->batchStream[k], ctx->joinEvent, 0));
   const int* frames = b->frames.data();
   std::vector<char> isFused(b->njobs, 0);
+  // the persistent launch first: launched after the stream jobs it ran slower
+  // (0.585 against 0.451 s on config 4)
   if (b->fm) {
     // one persistent launch over the module's jobs, heaviest first: its waves
     // take pixels from one job until that job's queue runs dry, then move on

@@ -229,7 +229,7 @@ int snl_batch_release(snl_batch batch);
  * translated scene with its #defines and __MAX_*__ limits; programs that
  * differ only in uniforms share one.  The programs must be path-tracer
  * programs of one build group (dispersion, interactive mode, PRECISE/FAST).
- * While batch fusion is on (snl_ctx_set_batch_fusion, default off), a
+ * While batch fusion is on (snl_ctx_set_batch_fusion, default on), a
  * submitted batch's jobs that one loaded module covers -- megakernel programs
  * that do not sample texture unit 0 -- run in that module's persistent launch,
  * heaviest first: its waves take pixels from one job until its queue is

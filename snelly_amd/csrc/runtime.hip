@@ -67,7 +67,7 @@ struct snl_ctx_s {
   int* scratch = nullptr;  // this device's copy of the allocation-free render's `scratch` array
   std::set<snl_batch_s*> batches;  // submitted batches not yet released
   std::map<unsigned long long, FusedModule*> fused;  // batch modules by source hash
-  bool batchFusion = false;  // snl_ctx_set_batch_fusion
+  bool batchFusion = true;  // snl_ctx_set_batch_fusion (effective once a batch module is loaded)
   hipStream_t graphStream = nullptr;  // where batch graphs are captured from (created on first use)
 };
 

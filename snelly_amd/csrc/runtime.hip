@@ -423,8 +423,12 @@ int compileProgram(snl_ctx ctx, const char* shader, const char* ior, const char*
   int rc = compileCached(ctx, source, opts, out.code, log, loglen);
   if (rc) return rc;
   // The megakernel asks for four waves per SIMD (at most 128 VGPRs); where that
-  // makes it spill to scratch it is built again without the bound.
-  if (kernelPrivateBytes(out.code, "snl_mega.kd") > 0) {
+  // makes it spill more than SNL_MEGA_SPILLS VGPRs (default 0) it is built
+  // again without the bound.
+  const long spillsOk = getenv("SNL_MEGA_SPILLS") ? atol(getenv("SNL_MEGA_SPILLS")) : 0;
+  long spills = kernelMetaInt(out.code, "snl_mega", ".vgpr_spill_count");
+  if (spills < 0) spills = kernelPrivateBytes(out.code, "snl_mega.kd") > 0 ? 1 << 20 : 0;
+  if (spills > spillsOk) {
     opts.push_back("-DSNL_MEGA_WAVES=1");
     rc = compileCached(ctx, source, opts, out.code, log, loglen);
   }

@@ -1769,7 +1769,11 @@ static int enqueueBatch(snl_batch_s* b, const std::vector<int>& order, std::vect
     } fa = {b->dFused, (const char*)b->dFused + argBytes, (const char*)b->dFused + argBytes + intBytes, n};
     void* fargs[] = {&fa};
     HIPCHK(ctx, hipModuleLaunchKernel(b->fm->k_reset, n, 1, 1, 64, 1, 1, 0, ts, fargs, nullptr));
-    const int blocks = (int)std::max(1LL, std::min<long long>(b->fm->blocks, blocksWanted));
+    // SNL_FUSED_GRID: percent of the resident grid the launch takes (default
+    // 100); less leaves room for the jobs on streams beside it
+    long long grid = b->fm->blocks;
+    if (const char* e = getenv("SNL_FUSED_GRID")) grid = grid * std::max(1, std::min(100, atoi(e))) / 100;
+    const int blocks = (int)std::max(1LL, std::min<long long>(grid, blocksWanted));
     HIPCHK(ctx, hipModuleLaunchKernel(b->fm->k_run, blocks, 1, 1, 256, 1, 1, 0, ts, fargs, nullptr));
     for (int i = 0; i < n; ++i) {
       b->stream[b->fusedJobs[i]] = k;

@@ -344,16 +344,18 @@ def batched(ctx, steps, warmup):
             "jobs_by_scene": fams}
 
 
-def served(ctx, n_jobs, seconds):
+def served(ctx, n_jobs, seconds, module=True):
     """config 5 (scenes.served_jobs): jobs arriving as a Poisson process at
     20/s, sides log-uniform 64..2048, each rendered until its estimated
     relative RMS error reaches its target (0.5..5%), by the continuous-batching
-    server, for `seconds` of arrivals plus a fifth of that to drain"""
+    server, for `seconds` of arrivals plus a fifth of that to drain; with
+    `module`, the server loads the batch module of the workload's 21 program
+    kinds"""
     from snelly_amd import scenes, serve
     arrivals = [a for a in scenes.served_jobs(n_jobs) if a[0] <= seconds]
     cutoff = 1.2 * seconds
     srv = serve.Server(ctx, quantum=(2, 256), tick_samples=TICK_SAMPLES, chain_lanes=CHAIN_LANES,
-                       max_chains=MAX_CHAINS)
+                       max_chains=MAX_CHAINS, module_scenes=scenes.batch_kinds() if module else None)
     t = time.perf_counter()
     done, left = srv.run(arrivals, time_limit=cutoff)
     wall = time.perf_counter() - t
@@ -374,7 +376,10 @@ def served(ctx, n_jobs, seconds):
                 "arrival_rate_per_s": 20.0, "sides": [64, 2048], "target_rrmse": [0.005, 0.05],
                 "cutoff_s": round(cutoff, 1), "wall_s": round(wall, 2),
                 "gpu_busy_frac": round(srv.busy_s / wall, 3), "submissions": srv.ticks,
-                "submission": "snl_batch_submit without blocking; quanta polled with snl_job_poll",
+                "submission": "snl_batch_submit without blocking; quanta polled with snl_job_poll" +
+                              ("; the megakernel quanta of a submission in one persistent launch (batch module "
+                               "of the 21 program kinds)" if module else "; each quantum its own launch"),
+                "launches_per_quantum": round(srv.launches / max(1, srv.quanta), 3),
                 "offered_path_samples_per_s_est": round(need / span, 1),
                 "tick_samples": TICK_SAMPLES, "chain_lanes": CHAIN_LANES, "max_chains": MAX_CHAINS,
                 "quantum_frames": [2, 256], "done_by_scene": fams,

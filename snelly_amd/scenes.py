@@ -499,6 +499,14 @@ def batch_job(family, material, side, spp, bounces, rs):
     return s
 
 
+def batch_kinds():
+    """one small scene of every family and material: the program kinds of the
+    batched and served workloads (their uniforms differ per job, their code
+    does not), from which a batch module holding all of them is built"""
+    rs = np.random.RandomState(5)
+    return [batch_job(fam, mat, 64, 1, 4, rs) for fam in BATCH_FAMILIES for mat in BATCH_MATERIALS]
+
+
 def batch_sample():
     """one small job of every family (materials in turn, 3-5 frames, 4-10
     bounces): the programs of the precise-build batch module the tests use"""

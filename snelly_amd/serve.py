@@ -26,7 +26,9 @@ So a job that arrives while others render is admitted and submitted at once,
 and its launches join the others on the device (the runtime queues them on
 the least loaded streams).  Pixels never read each other's state, so a job's
 images do not depend on which other jobs shared the device
-(tests/test_serve.py checks this).
+(tests/test_serve.py checks this).  Given the workload's program kinds
+(module_scenes), the server loads their batch module, and the megakernel
+quanta of a submission run in one persistent launch.
 """
 import math
 import time
@@ -81,14 +83,17 @@ class Server:
 
     def __init__(self, ctx, flags=native.BUILD_FAST, min_frames=4, quantum=(2, 64), reserve=8 << 30,
                  max_active=256, clock=time.perf_counter, keep_images=False, tick_samples=None,
-                 chain_lanes=1 << 16, max_chains=32, depth=2):
+                 chain_lanes=1 << 16, max_chains=32, depth=2, module_scenes=None):
         self.ctx, self.flags, self.keep_images = ctx, flags, keep_images
         self.min_frames, self.quantum, self.reserve, self.max_active = min_frames, quantum, reserve, max_active
         self.tick_samples = tick_samples
         self.chain_lanes, self.max_chains = chain_lanes, max_chains
         self.depth = depth
+        self.module_scenes = module_scenes
         self.clock = clock
         self.ticks = 0      # submissions
+        self.quanta = 0     # job quanta submitted
+        self.launches = 0   # render launches they took (a persistent launch of a batch module counts once)
         self.samples = 0
         self.busy_s = 0.0   # wall time with work on the GPU
 
@@ -138,6 +143,14 @@ class Server:
         """arrivals: [(t, scene, target)] with t in seconds from the start,
         ascending.  Returns the finished jobs (with .arrival, .admitted,
         .finished in seconds from the start) and the ones left unfinished."""
+        if self.module_scenes:
+            # the batch module of the workload's program kinds (from the code-object
+            # cache): the megakernel quanta of each submission then run in one
+            # persistent launch
+            protos = [host.Session(s, 8, 8, self.ctx, flags=self.flags) for s in self.module_scenes]
+            native.batch_module(self.ctx, [p.prog for p in protos])
+            for p in protos:
+                p.close()
         t0 = self.clock()
         waiting = [Job(i, s, tg, t) for i, (t, s, tg) in enumerate(arrivals)]
         waiting.reverse()  # pop() takes the earliest
@@ -209,6 +222,8 @@ class Server:
                         job.close_after = True
                         done.append(job)
                 if all(got):
+                    self.quanta += len(jq)
+                    self.launches += b.wait().trace_launches
                     b.release()
                     for job, _ in jq:
                         if getattr(job, "close_after", False):

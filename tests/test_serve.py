@@ -179,3 +179,27 @@ def test_served_2048_job_matches_alone(ctx):
         alone.render(job.frames)
         assert np.array_equal(alone.radiance(), job.image), s.name
         alone.close()
+
+
+@pytest.mark.gpu
+def test_served_jobs_in_one_launch_match_alone(ctx):
+    """with the batch module of the jobs' program kinds loaded, the quanta of a
+    submission run in one persistent launch (fewer launches than quanta), and
+    every job's image -- multi-chain states included -- equals the job
+    rendered alone with the same seeds and frames"""
+    # the kinds of scenes.batch_sample(), whose precise-build module is precompiled
+    rs = np.random.RandomState(9)
+    arr = [(0.002 * k, scenes.batch_job(fam, scenes.BATCH_MATERIALS[k % 3], side, 0, 5 + k % 4, rs), 0.3)
+           for k, (fam, side) in enumerate(zip(scenes.BATCH_FAMILIES, [40, 24, 64, 32, 48, 56, 28]))]
+    srv = serve.Server(ctx, flags=native.BUILD_PRECISE, reserve=1 << 30, keep_images=True, quantum=(2, 16),
+                       module_scenes=scenes.batch_sample())
+    done, left = srv.run(arr, time_limit=120.0)
+    assert not left and len(done) == len(arr)
+    assert srv.launches < srv.quanta
+    for job in done:
+        s = job.scene
+        alone = host.Session(s, s.width, s.height, ctx, seed=1000003 * (job.id + 1), flags=native.BUILD_PRECISE,
+                             chains=job.chains)
+        alone.render(job.frames)
+        assert np.array_equal(alone.radiance(), job.image), s.name
+        alone.close()

@@ -142,7 +142,7 @@ __device__ inline void persist_body(const SnlKArgs& a) {
           t += sgn * sdf;
           ++steps;
           if (abs(t) >= tmax) done = true;
-          else if (steps >= SNL_MAX_MARCH_STEPS) { done = true; hit = !P.maxStepsIsMiss; m = -2; p = o + t * d; }
+          else if (steps >= SC::kMaxMarchSteps) { done = true; hit = !P.maxStepsIsMiss; m = -2; p = o + t * d; }
         }
         if (done) {
           if (!shadow) {
@@ -180,8 +180,7 @@ __device__ inline void persist_body(const SnlKArgs& a) {
       if (hm == MAT_INVAL) {
         // miss (1657-1712)
         bool lit = false;
-#ifdef HAS_SPHERE_LIGHT
-        {
+        if constexpr (SC::kHasSphereLight) {
           vec3 pLightHit;
           RT LiSph = (P.sphereLightPower < RADIANCE_EPSILON) ? rt1(0.0f) : pt.sphereLightRadiance(o, d, rgb, pLightHit);
           if (averageComponent(LiSph) > RADIANCE_EPSILON) {
@@ -191,7 +190,6 @@ __device__ inline void persist_body(const SnlKArgs& a) {
             L += tp * Tr * misSph * LiSph / (1.0f - sp);
           }
         }
-#endif
         if (!lit) {
           RT Tr = pt.transmittanceOverFreeSegment(o, d, P.maxLengthScale, rgb);
           float sp = pt.atmosphericScatteringProbability(Tr, rgb);
@@ -210,8 +208,7 @@ __device__ inline void persist_body(const SnlKArgs& a) {
         const int hitMaterial = hm;
         vec3 pW = o, rayDir = d, pW_next = hp;
         float rayLength = length(pW_next - pW);
-#ifdef HAS_SPHERE_LIGHT
-        {
+        if constexpr (SC::kHasSphereLight) {
           vec3 pLightHit;
           RT LiSph = (P.sphereLightPower < RADIANCE_EPSILON) ? rt1(0.0f) : pt.sphereLightRadiance(pW, rayDir, rgb, pLightHit);
           if (averageComponent(LiSph) > RADIANCE_EPSILON) {
@@ -222,7 +219,6 @@ __device__ inline void persist_body(const SnlKArgs& a) {
             }
           }
         }
-#endif
         if constexpr (SC::kHasDielectric) {
           if (inDiele) tp *= exp(-rayLength * pt.DIELECTRIC_ABSORPTION_EVAL(pW, rgb));
         }
@@ -291,21 +287,21 @@ __device__ inline void persist_body(const SnlKArgs& a) {
                 if (anyNonzero(nf1)) neeMask |= 2;
               }
             }
-#ifdef HAS_SPHERE_LIGHT
-            if ((P.sphereLightPower > RADIANCE_EPSILON) && (dot(pW - P.sphereLightPosition, basis.nW) <= P.sphereLightRadius)) {
-              vec3 pHit;
-              RT Li = pt.sampleSphereLight(rgb, rnd, pW, pHit, sphPdf);
-              if (averageComponent(Li) > RADIANCE_EPSILON) {
-                wW = normalize(pHit - pW);
-                wL = worldToLocal(wW, basis);
-                float lp = max(PDF_EPSILON, pt.pdfBsdf(pW, basis, winputL, wL, hitMaterial, wl, rgb, true));
-                nf2 = pt.evaluateBsdf(pW, basis, winputL, wL, hitMaterial, wl, rgb, true);
-                nl2 = Li; nd2 = wW; nt2 = max(0.0f, length(pHit - pW) - 3.0f * P.minLengthScale);
-                nden2 = max(PDF_EPSILON, sphPdf); ncos2 = abs(dot(wW, basis.nW)); nmis2 = powerHeuristic(sphPdf, lp);
-                if (anyNonzero(nf2)) neeMask |= 4;
+            if constexpr (SC::kHasSphereLight) {
+              if ((P.sphereLightPower > RADIANCE_EPSILON) && (dot(pW - P.sphereLightPosition, basis.nW) <= P.sphereLightRadius)) {
+                vec3 pHit;
+                RT Li = pt.sampleSphereLight(rgb, rnd, pW, pHit, sphPdf);
+                if (averageComponent(Li) > RADIANCE_EPSILON) {
+                  wW = normalize(pHit - pW);
+                  wL = worldToLocal(wW, basis);
+                  float lp = max(PDF_EPSILON, pt.pdfBsdf(pW, basis, winputL, wL, hitMaterial, wl, rgb, true));
+                  nf2 = pt.evaluateBsdf(pW, basis, winputL, wL, hitMaterial, wl, rgb, true);
+                  nl2 = Li; nd2 = wW; nt2 = max(0.0f, length(pHit - pW) - 3.0f * P.minLengthScale);
+                  nden2 = max(PDF_EPSILON, sphPdf); ncos2 = abs(dot(wW, basis.nW)); nmis2 = powerHeuristic(sphPdf, lp);
+                  if (anyNonzero(nf2)) neeMask |= 4;
+                }
               }
             }
-#endif
           }
           ntp = tp;
           acc = rt1(0.0f);
@@ -362,9 +358,9 @@ __device__ inline void persist_body(const SnlKArgs& a) {
           accum = ld4(a.radiance, pix);
         }
         skip = false;
-#ifdef INTERACTIVE_MODE
-        if (snl_rand(rnd) < P.skipProbability) skip = true;
-#endif
+        if constexpr (SC::kInteractive) {
+          if (snl_rand(rnd) < P.skipProbability) skip = true;
+        }
         Ls = rt1(0.0f);
         if (!skip) {
 #ifdef DISPERSION_ENABLED

@@ -91,10 +91,13 @@ int snl_uniform_texture(snl_prog prog, int idx, snl_tex tex);
  * (js/gl.js:485-523); depth > 1 makes a 3-D texture.  texels may be NULL. */
 int snl_texture_create(snl_ctx ctx, int width, int height, int depth, int channels, int linear, int clamp,
                        const float* texels, snl_tex* out);
-/* GLU.Texture.copy (js/gl.js:503-507): width*height*depth*channels floats */
+/* GLU.Texture.copy (js/gl.js:503-507): width*height*depth*channels floats.
+ * Submitted batch jobs of programs that sample the texture finish first (they
+ * read the old texels). */
 int snl_texture_upload(snl_tex tex, const float* texels);
-/* Deletes the texture (gl.deleteTexture); sampler uniforms still bound to it
- * then read (0, 0, 0, 1), as a texture unit without a texture does in WebGL. */
+/* Deletes the texture (gl.deleteTexture), after the submitted batch jobs of
+ * programs that sample it; sampler uniforms still bound to it then read
+ * (0, 0, 0, 1), as a texture unit without a texture does in WebGL. */
 int snl_texture_destroy(snl_tex tex);
 /* The texture's shape, for bindings that check buffer sizes (a WebGL texture
  * object carries its own; no C counterpart in the reference). */

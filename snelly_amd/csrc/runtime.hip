@@ -1009,6 +1009,9 @@ int snl_texture_info(snl_tex t, int* width, int* height, int* depth, int* channe
 int snl_texture_upload(snl_tex t, const float* texels) {
   if (!t || !texels) return fail(t ? t->ctx : nullptr, SNL_E_ARG, "NULL argument");
   size_t bytes = (size_t)t->w * t->h * t->depth * t->ch * sizeof(float);
+  // a submitted batch job of a program that samples the texture reads the old texels
+  for (snl_prog_s* p : t->users)
+    if (int rc = settle(p)) return rc;
   HIPCHK(t->ctx, hipMemcpyAsync(t->d, texels, bytes, hipMemcpyHostToDevice, t->ctx->stream));
   HIPCHK(t->ctx, hipStreamSynchronize(t->ctx->stream));
   return SNL_OK;
@@ -1017,6 +1020,9 @@ int snl_texture_upload(snl_tex t, const float* texels) {
 int snl_texture_destroy(snl_tex t) {
   if (!t) return SNL_OK;
   hipStreamSynchronize(t->ctx->stream);
+  // a submitted batch job of a program that samples the texture finishes first
+  for (snl_prog_s* p : t->users)
+    if (int rc = settle(p)) return rc;
   // a sampler still bound to the texture now reads its unit's missing texture,
   // (0, 0, 0, 1) as in WebGL, instead of freed memory; it stays assigned a unit
   for (snl_prog_s* p : t->users) {

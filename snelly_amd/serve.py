@@ -27,8 +27,8 @@ and its launches join the others on the device (the runtime queues them on
 the least loaded streams).  Pixels never read each other's state, so a job's
 images do not depend on which other jobs shared the device
 (tests/test_serve.py checks this).  Given the workload's program kinds
-(module_scenes), the server loads their batch module, and the megakernel
-quanta of a submission run in one persistent launch.
+(module_scenes), the server loads their batch module and turns batch fusion
+on, and the megakernel quanta of a submission run in one persistent launch.
 """
 import math
 import time
@@ -149,6 +149,7 @@ class Server:
             # persistent launch
             protos = [host.Session(s, 8, 8, self.ctx, flags=self.flags) for s in self.module_scenes]
             native.batch_module(self.ctx, [p.prog for p in protos])
+            native.set_batch_fusion(self.ctx, True)
             for p in protos:
                 p.close()
         t0 = self.clock()

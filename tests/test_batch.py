@@ -207,10 +207,7 @@ def run_fused(ctx, jobs, flags, seed):
     sessions = [host.Session(s, s.width, s.height, ctx, seed=seed + j, flags=flags) for j, s in enumerate(jobs)]
     native.batch_module(ctx, [se.prog for se in sessions])
     native.set_batch_fusion(ctx, True)
-    try:
-        st = native.render_batch([(se.prog, se.state, s.spp) for se, s in zip(sessions, jobs)])
-    finally:
-        native.set_batch_fusion(ctx, False)
+    st = native.render_batch([(se.prog, se.state, s.spp) for se, s in zip(sessions, jobs)])
     out = []
     for se in sessions:
         out.append((se.radiance().reshape(-1, 4).copy(), se.state.read(1).reshape(-1, 4).copy()))
@@ -299,7 +296,7 @@ def test_batch_graph_replays_equal_alone(ctx, fusion):
     try:
         g = native.Graph([(se.prog, se.state, s.spp) for se, s in zip(sessions, jobs)])
     finally:
-        native.set_batch_fusion(ctx, False)
+        native.set_batch_fusion(ctx, True)  # the context's default
     st1 = g.launch()
     st2 = g.launch()
     g.close()

@@ -57,6 +57,30 @@ def test_texture_destroyed_while_bound_reads_an_empty_unit(ctx):
         x.close()
 
 
+def test_texture_changed_while_a_submitted_job_samples_it(ctx):
+    """a texture re-uploaded, then destroyed, right after a batch that samples
+    it was submitted: the job first finishes with the texels it was submitted
+    with (it equals the same render done before either call)"""
+    W = H = 24
+    s = textured_scene(W, H)
+    ref = host.Session(s, W, H, ctx, seed=9, flags=native.BUILD_PRECISE)
+    t0 = native.Texture(ctx, np.full((4, 4, 4), 0.7, np.float32), 4, 4, 1, 4, True, True)
+    ref.prog.set_texture("albedoTex", t0)
+    ref.render(3)
+    want = ref.radiance().copy()
+    a = host.Session(s, W, H, ctx, seed=9, flags=native.BUILD_PRECISE)
+    tex = native.Texture(ctx, np.full((4, 4, 4), 0.7, np.float32), 4, 4, 1, 4, True, True)
+    a.prog.set_texture("albedoTex", tex)
+    b = native.Batch([(a.prog, a.state, 3)])
+    tex.upload(np.full((4, 4, 4), 0.1, np.float32))
+    tex.close()
+    b.wait()
+    b.release()
+    assert np.array_equal(a.radiance(), want)
+    for x in (a, ref, t0):
+        x.close()
+
+
 def test_batch_rejects_a_repeated_program(ctx):
     s = scenes.sphere_plane(16, 16, 2)
     a = host.Session(s, 16, 16, ctx)

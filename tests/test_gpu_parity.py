@@ -126,6 +126,31 @@ def test_pathtrace_sphere_plane(ctx, engine):
 
 
 @pytest.mark.parametrize("engine", ENGINES, ids=ENGINE_IDS)
+def test_tiny_ragged_images_zero_frames_and_bounces(ctx, engine):
+    """edge cases of the launch shapes: 1x1, sizes that are not multiples of
+    the 8x8 tile or of a wave (3x5, 9x7, 17x8, 65x3), no bounce beyond the
+    first vertex, and a render of zero frames, which leaves the state as
+    seeded; every pixel's sample count equals the oracle's and all but a rare
+    rounding-flipped path agree per pixel"""
+    closes, rngs = [], []
+    for (W, H, bounces) in [(1, 1, 4), (3, 5, 4), (9, 7, 0), (17, 8, 4), (65, 3, 1)]:
+        s = scenes.sphere_plane(W, H, 3)
+        s.renderer.maxBounces = bounces
+        g, r, close, rng_same, _ = compare_render(ctx, s, W, H, 3, engine=engine)
+        closes.append(close * W * H)
+        rngs.append(rng_same * W * H)
+    n = 1 + 15 + 63 + 136 + 195
+    assert sum(closes) >= 0.97 * n and sum(rngs) >= 0.97 * n
+    s = scenes.sphere_plane(9, 7, 3)
+    sess = session(ctx, s, 9, 7, engine=engine)
+    rad0, rng0 = sess.radiance().copy(), sess.state.read(1).copy()
+    sess.render(0)
+    assert np.array_equal(sess.radiance(), rad0) and np.array_equal(sess.state.read(1), rng0)
+    assert np.all(rad0.reshape(-1, 4) == np.array([0, 0, 0, 1], np.float32))
+    sess.close()
+
+
+@pytest.mark.parametrize("engine", ENGINES, ids=ENGINE_IDS)
 def test_pathtrace_menger_metal(ctx, engine):
     s = scenes.menger_metal(64, 64, 8)
     g, r, close, rng_same, _ = compare_render(ctx, s, 64, 64, 8, engine=engine)

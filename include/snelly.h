@@ -180,6 +180,15 @@ int snl_set_engine(snl_prog prog, int engine, int tiles, int persist_blocks, int
 /* persistent engine: march steps a tracing lane takes between two checks of
  * the shading gate (1..64, default 4); results do not depend on it */
 int snl_set_march_burst(snl_prog prog, int steps);
+/* The megakernel asks for four waves per SIMD; where that build spills
+ * registers the program renders with a build without the bound (fewer waves,
+ * no spills).  snl_mega_variants is 2 when both builds exist (else 1, or a
+ * negative code); snl_set_mega_variant(prog, 1) renders with the four-wave
+ * build that spills (loaded on first use), 0 with the default.  Same code and
+ * arithmetic: results do not depend on the variant, only the speed does (the
+ * autotuner times both).  No counterpart in the reference (launch tuning). */
+int snl_mega_variants(snl_prog prog);
+int snl_set_mega_variant(snl_prog prog, int variant);
 
 /* Batched rendering: frames[j] frames of every job j = (progs[j], states[j]),
  * each with its program's current uniforms, submitted together -- the jobs'

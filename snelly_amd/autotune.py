@@ -1,7 +1,9 @@
 """Autotuner for the engine and launch shape of pixel-queue renders.
 
 Knobs: the engine (the persistent state machine or the megakernel, which
-take pixels from the same queues), pixel tiles pipelined on separate streams,
+take pixels from the same queues), the megakernel's build where it has two
+(four waves per SIMD with spills, or fewer waves without), pixel tiles
+pipelined on separate streams,
 resident blocks per launch
 (waves per CU = 4 x blocks / CUs), the shading threshold (lanes of a wave that
 must wait before the wave runs the shading code), the number of pixels a
@@ -39,10 +41,15 @@ def tune(sess, frames=4, cu=256, verbose=False, engines=(native.ENGINE_PERSISTEN
         "burst": [1, 2, 4, 8, 16],
     }
     best = {"engine": engines[0], "tiles": 1, "blocks": cu * 6, "thresh": 32, "chunk": 32, "burst": 4}
+    if native.ENGINE_MEGAKERNEL in engines and prog.mega_variants() == 2:
+        space["variant"] = [0, 1]
+        best["variant"] = 0
 
     def run(cfg):
         prog.engine(cfg["engine"], cfg["tiles"], cfg["blocks"], cfg["thresh"], cfg["chunk"])
         prog.march_burst(cfg["burst"])
+        if "variant" in cfg:
+            prog.mega_variant(cfg["variant"])
         return time_render(sess, frames)
 
     run(best)  # warm the code object and caches
@@ -60,8 +67,19 @@ def tune(sess, frames=4, cu=256, verbose=False, engines=(native.ENGINE_PERSISTEN
                     print("autotune", cfg, "%.2f ms" % (t * 1e3), flush=True)
                 if t < best_t:
                     best, best_t = cfg, t
+    if "variant" in best and best["engine"] == native.ENGINE_MEGAKERNEL:
+        # the two megakernel builds differ by a few per cent, about the spread
+        # of one short trial: decide between them on the best of three each
+        other = dict(best, variant=1 - best["variant"])
+        tb = min(run(best) for _ in range(3))
+        to = min(run(other) for _ in range(3))
+        trials += [(dict(best), tb), (other, to)]
+        if to < tb:
+            best, best_t = other, to
     prog.engine(best["engine"], best["tiles"], best["blocks"], best["thresh"], best["chunk"])
     prog.march_burst(best["burst"])
+    if "variant" in best:
+        prog.mega_variant(best["variant"])
     return best, best_t, trials
 
 

@@ -112,6 +112,13 @@ struct snl_prog_s {
   // medium (delta and ratio tracking inside the vertex)
   bool megaOnly = false;
   int megaBlocks = 0, modeBlocks = 0;
+  // the megakernel's two builds: the one the program renders with by default,
+  // and, where four waves per SIMD spill, that four-wave build (its code object
+  // kept on the host, loaded on first use by snl_set_mega_variant)
+  hipFunction_t k_megaMain = nullptr, k_megaSpill = nullptr;
+  std::vector<char> spillCode;
+  hipModule_t spillMod = nullptr;
+  int megaVariant = 0;
   int maxBounces = 3, maxMarch = 256, spf = 1;
   SnlParams hostP;
   std::vector<char> hostU;
@@ -232,6 +239,7 @@ std::string cacheDir() {
 
 struct Compiled {
   std::vector<char> code;
+  std::vector<char> spillCode;  // the four-wave megakernel build that spilled, when it did
   snl::TranslateResult tr;
   std::set<std::string> defines;
   std::string defs;  // the program's #define lines
@@ -429,6 +437,7 @@ int compileProgram(snl_ctx ctx, const char* shader, const char* ior, const char*
   long spills = kernelMetaInt(out.code, "snl_mega", ".vgpr_spill_count");
   if (spills < 0) spills = kernelPrivateBytes(out.code, "snl_mega.kd") > 0 ? 1 << 20 : 0;
   if (spills > spillsOk) {
+    out.spillCode.swap(out.code);
     opts.push_back("-DSNL_MEGA_WAVES=1");
     rc = compileCached(ctx, source, opts, out.code, log, loglen);
   }
@@ -859,6 +868,8 @@ int snl_program_build(snl_ctx ctx, const char* shader, const char* ior, const ch
   occ = 0;
   if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, p->k_mega, 256, 0) != hipSuccess || occ < 1) occ = 2;
   p->megaBlocks = ctx->numCU * occ;
+  p->k_megaMain = p->k_mega;
+  p->spillCode.swap(c.spillCode);
   if (p->mode) {
     occ = 0;
     if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occ, p->k_mode, 256, 0) != hipSuccess || occ < 1) occ = 2;
@@ -880,6 +891,7 @@ int snl_program_destroy(snl_prog p) {
   if (p->dP) hipFree(p->dP);
   if (p->dU) hipFree(p->dU);
   if (p->mod) hipModuleUnload(p->mod);
+  if (p->spillMod) hipModuleUnload(p->spillMod);
   for (auto& kv : p->texOf)
     if (kv.second) kv.second->users.erase(p);
   for (hipEvent_t e : {p->ev0, p->ev1, p->et0, p->et1}) if (e) hipEventDestroy(e);
@@ -2092,6 +2104,27 @@ int snl_set_engine(snl_prog p, int engine, int tiles, int persist_blocks, int sh
   if (persist_blocks > 0) (engine == SNL_ENGINE_MEGAKERNEL ? p->megaBlocks : p->persistBlocks) = persist_blocks;
   if (shade_thresh > 0) p->shadeThresh = std::min(shade_thresh, 64);
   if (chunk > 0) p->persistChunk = chunk;
+  return SNL_OK;
+}
+
+int snl_mega_variants(snl_prog p) {
+  if (!p) return fail(nullptr, SNL_E_ARG, "prog is NULL");
+  return p->spillCode.empty() && !p->spillMod ? 1 : 2;
+}
+
+int snl_set_mega_variant(snl_prog p, int variant) {
+  if (!p) return fail(nullptr, SNL_E_ARG, "prog is NULL");
+  if (variant != 0 && variant != 1) return fail(p->ctx, SNL_E_ARG, "bad megakernel variant %d", variant);
+  if (variant == 1 && !p->spillMod) {
+    if (p->spillCode.empty())
+      return fail(p->ctx, SNL_E_UNSUPPORTED, "the program's megakernel has one build (four waves per SIMD do not spill)");
+    HIPCHK(p->ctx, hipSetDevice(p->ctx->device));
+    HIPCHK(p->ctx, hipModuleLoadData(&p->spillMod, p->spillCode.data()));
+    HIPCHK(p->ctx, hipModuleGetFunction(&p->k_megaSpill, p->spillMod, "snl_mega"));
+    std::vector<char>().swap(p->spillCode);
+  }
+  p->k_mega = variant ? p->k_megaSpill : p->k_megaMain;
+  p->megaVariant = variant;
   return SNL_OK;
 }
 

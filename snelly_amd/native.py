@@ -26,6 +26,7 @@ EXPORTS = [
     "snl_state_write", "snl_render", "snl_render_normals", "snl_pick", "snl_tonemap", "snl_sync",
     "snl_get_stats", "snl_set_tuning", "snl_set_profiling", "snl_set_engine", "snl_render_batch",
     "snl_state_create_chains", "snl_state_error", "snl_ctx_mem_info", "snl_state_bytes", "snl_set_march_burst",
+    "snl_mega_variants", "snl_set_mega_variant",
     "snl_render_noalloc", "snl_program_state_bytes", "snl_state_info", "snl_texture_info",
     "snl_batch_submit", "snl_job_poll", "snl_batch_wait", "snl_batch_job_stats", "snl_batch_release",
     "snl_batch_module", "snl_batch_module_precompile", "snl_ctx_set_batch_fusion",
@@ -94,6 +95,8 @@ def lib():
         "snl_state_info": (ip, [vp] + [ctypes.POINTER(ip)] * 3),
         "snl_texture_info": (ip, [vp] + [ctypes.POINTER(ip)] * 4),
         "snl_set_march_burst": (ip, [vp, ip]),
+        "snl_mega_variants": (ip, [vp]),
+        "snl_set_mega_variant": (ip, [vp, ip]),
         "snl_batch_submit": (ip, [ip, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(ip),
                                   ctypes.POINTER(vp)]),
         "snl_job_poll": (ip, [vp, ip]),
@@ -413,6 +416,15 @@ class Program:
 
     def march_burst(self, steps):
         self.ctx.check(lib().snl_set_march_burst(self.h, steps))
+
+    def mega_variants(self):
+        """2 when the megakernel has a four-wave build that spills beside the default"""
+        n = lib().snl_mega_variants(self.h)
+        self.ctx.check(min(n, 0))
+        return n
+
+    def mega_variant(self, variant):
+        self.ctx.check(lib().snl_set_mega_variant(self.h, variant))
 
     def engine(self, engine, tiles=0, persist_blocks=0, shade_thresh=0, chunk=0):
         self.ctx.check(lib().snl_set_engine(self.h, engine, tiles, persist_blocks, shade_thresh, chunk))

@@ -28,6 +28,12 @@ def test_library_exports_every_declared_symbol():
     assert native.lib().snl_version().startswith(b"snelly-mi355x")
 
 
+def test_mega_variant_calls_reject_null():
+    L = native.lib()
+    assert L.snl_mega_variants(None) == -1  # SNL_E_ARG
+    assert L.snl_set_mega_variant(None, 0) == -1
+
+
 @pytest.mark.parametrize("make", [scenes.sphere_plane, scenes.menger_metal, scenes.mandelbox_dielectric])
 def test_scene_compiles_for_gfx950(make):
     """translate + hipRTC compile of the benchmark scenes (cached code objects)"""

@@ -107,3 +107,21 @@ def test_pick_reaches_the_medium_bound(ctx):
     hit = g[:, 1] == 3.0
     assert np.all(np.abs(g[hit, 0] - r[hit, 0]) <= 1e-4 * r[hit, 0])
     assert centre[0, 1] == 3.0 and abs(centre[0, 0] - 3.0) < 2e-4
+
+
+def test_megakernel_variants_render_the_same(ctx):
+    """config 3's fast-build program spills at four waves per SIMD, so it has
+    two megakernel builds (the default without the bound, the four-wave one
+    with spills); the autotuner may pick either: both render the same image
+    and RNG states, bit for bit"""
+    s = scenes.mandelbox_volume(48, 32, 4)
+    out = []
+    for variant in (0, 1):
+        sess = host.Session(s, 48, 32, ctx, seed=77, flags=native.BUILD_FAST)
+        assert sess.prog.mega_variants() == 2
+        sess.prog.mega_variant(variant)
+        sess.render(4)
+        out.append((sess.radiance().copy(), sess.state.read(1).copy()))
+        sess.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])

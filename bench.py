@@ -21,7 +21,7 @@ Also measured, in the same line:
   cpu_baseline  the oracle (the CPU restatement) on the host's cores.
 
   python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--cpu-seconds 12] [--no-batch]
-                  [--served-jobs 2000] [--served-seconds 100] [--no-secondary]
+                  [--served-jobs 2000] [--served-seconds 100] [--no-secondary] [--mega-variant 0|1]
   python bench.py --copy-yardstick [MiB]   a plain device copy, for scaling
                                            HBM traffic counters
 """
@@ -51,7 +51,8 @@ FP32_YARDSTICK_TFLOPS = 101.8
 # WRITE_SIZE counter runs scaled by the copy yardstick (profiles/*_traffic*.md):
 # two launches per render (frame 1 with the tile-cost atomics, then the other
 # frames in cost order), each reading and writing the pixel's radiance and RNG
-TRAFFIC_B_PER_PX = {"mandelbox_volume": 140.6, "menger_metal": 144.3}
+# (per program build: 0 the default, 1 the four-wave build with spills)
+TRAFFIC_B_PER_PX = {("mandelbox_volume", 0): 140.6, ("menger_metal", 0): 144.3}
 # served: path samples per tick (every job's smallest quantum included, in
 # round-robin order); chains per job: the fewest (up to 128) that give it 2^18 lanes
 TICK_SAMPLES = 1 << 26
@@ -137,7 +138,7 @@ def path_counts(scene, width, height, spp, budget_s):
             "sample": f"counting build over every {stride}th pixel x {spp} spp ({c['paths']} paths)"}
 
 
-def roofline(counts, stats, renders, gpu_ms, kname, width, height, traffic_key):
+def roofline(counts, stats, renders, gpu_ms, kname, width, height, traffic_key, variant=0):
     """FP32 VALU roofline of the render kernel: algorithmic FLOPs per render
     (the GPU's sphere-tracer steps, normals and paths priced by the counting
     build) over the kernel's GPU time per render (hipEvents on its stream)"""
@@ -149,7 +150,7 @@ def roofline(counts, stats, renders, gpu_ms, kname, width, height, traffic_key):
     ms = gpu_ms / renders
     achieved = flops / (ms * 1e-3) / 1e12
     per_render = stats["launches"] / renders
-    bpp = TRAFFIC_B_PER_PX.get(traffic_key)
+    bpp = TRAFFIC_B_PER_PX.get((traffic_key, variant))
     return {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
             "frac_of_yardstick": round(achieved / FP32_YARDSTICK_TFLOPS, 4),
@@ -165,14 +166,14 @@ def roofline(counts, stats, renders, gpu_ms, kname, width, height, traffic_key):
             "counts": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in counts.items()}}
 
 
-def plain(ctx, scene, steps, warmup, engines, tune_frames):
+def plain(ctx, scene, steps, warmup, engines, tune_frames, variants=(0, 1)):
     """the plain one-scene renderer: the fast build, engine and launch shape
     autotuned (untimed), then `steps` renders of scene.spp frames each from a
     freshly seeded state, timed between device synchronisations"""
     from snelly_amd import autotune, host, native
     W, H, spp = scene.width, scene.height, scene.spp
     sess = host.Session(scene, W, H, ctx, flags=native.BUILD_FAST)
-    best, _, _ = autotune.tune(sess, frames=tune_frames, engines=engines)
+    best, _, _ = autotune.tune(sess, frames=tune_frames, engines=engines, variants=variants)
     for w in range(warmup):
         sess.state.seed(1000 + w)
         sess.render(spp)
@@ -484,6 +485,9 @@ def main():
     ap.add_argument("--copy-yardstick", type=int, nargs="?", const=1024, default=None, metavar="MIB")
     ap.add_argument("--batch-only", action="store_true", help="only the batched leg (config 4), as one JSON line")
     ap.add_argument("--valu-yardstick", action="store_true", help="a kernel of known FMA count (SQ counter scale)")
+    ap.add_argument("--mega-variant", type=int, choices=(0, 1), default=None,
+                    help="fix the headline's megakernel build (0 default, 1 four waves with spills) instead of "
+                         "letting the autotuner choose")
     args = ap.parse_args()
     if args.copy_yardstick is not None:
         copy_yardstick(args.copy_yardstick)
@@ -505,8 +509,9 @@ def main():
     # ---- headline: config 3 (megakernel: the medium's tracking runs inside a vertex)
     scene = scenes.mandelbox_volume(1920, 1080, 256)
     W, H, spp = scene.width, scene.height, scene.spp
+    variants = (0, 1) if args.mega_variant is None else (args.mega_variant,)
     sess, best, elapsed, gpu_ms, stats, rr = plain(ctx, scene, args.steps, args.warmup,
-                                                   (native.ENGINE_MEGAKERNEL,), 4)
+                                                   (native.ENGINE_MEGAKERNEL,), 4, variants)
     sess.close()
     samples = W * H * spp * args.steps
     counts = path_counts(scenes.mandelbox_volume(W, H, spp), W, H, spp, args.count_seconds)
@@ -521,7 +526,8 @@ def main():
                    "build": "fast", "launch_shape": {k: v for k, v in best.items() if k != "engine"},
                    "tile_order": "cost (frame 1, then the 8x8 tiles costliest first)",
                    "path_samples_per_step": W * H * spp},
-        "roofline": roofline(counts, stats, args.steps, gpu_ms, kname, W, H, "mandelbox_volume"),
+        "roofline": roofline(counts, stats, args.steps, gpu_ms, kname, W, H, "mandelbox_volume",
+                             best.get("variant", 0)),
         "rrmse_per_step": None if rr is None else round(rr, 5),
         "spp_for_rrmse_1pct": None if rr is None else int(math.ceil(spp * (rr / 0.01) ** 2)),
     }
@@ -538,7 +544,7 @@ def main():
             "path_samples_per_s": round(1024 * 1024 * 64 * 3 / mel, 1), "ms_per_step": round(mel * 1e3 / 3, 3),
             "engine": autotune.ENGINE_NAMES[mbest["engine"]],
             "launch_shape": {k: v for k, v in mbest.items() if k != "engine"},
-            "roofline": roofline(mcounts, mstats, 3, mgpu, mk, 1024, 1024, "menger_metal"),
+            "roofline": roofline(mcounts, mstats, 3, mgpu, mk, 1024, 1024, "menger_metal", mbest.get("variant", 0)),
             "rrmse_per_step": None if mrr is None else round(mrr, 5),
             "first_hit_pass": first_hit_pass(ctx, scenes.menger_metal(1024, 1024, 1))}
     if not args.no_batch:

@@ -28,9 +28,11 @@ def time_render(sess, frames):
 
 
 def tune(sess, frames=4, cu=256, verbose=False, engines=(native.ENGINE_PERSISTENT, native.ENGINE_MEGAKERNEL),
-         tiles=(1, 2, 4)):
+         tiles=(1, 2, 4), variants=(0, 1)):
     """coordinate descent over the knobs; `tiles` (1) for jobs rendered by
-    snl_render_batch, which launches one kernel per job"""
+    snl_render_batch, which launches one kernel per job; `variants` the
+    megakernel builds to consider where the program has two (a single value
+    fixes it)"""
     prog = sess.prog
     space = {
         "engine": list(engines),
@@ -42,8 +44,8 @@ def tune(sess, frames=4, cu=256, verbose=False, engines=(native.ENGINE_PERSISTEN
     }
     best = {"engine": engines[0], "tiles": 1, "blocks": cu * 6, "thresh": 32, "chunk": 32, "burst": 4}
     if native.ENGINE_MEGAKERNEL in engines and prog.mega_variants() == 2:
-        space["variant"] = [0, 1]
-        best["variant"] = 0
+        space["variant"] = list(variants)
+        best["variant"] = variants[0]
 
     def run(cfg):
         prog.engine(cfg["engine"], cfg["tiles"], cfg["blocks"], cfg["thresh"], cfg["chunk"])
@@ -67,7 +69,7 @@ def tune(sess, frames=4, cu=256, verbose=False, engines=(native.ENGINE_PERSISTEN
                     print("autotune", cfg, "%.2f ms" % (t * 1e3), flush=True)
                 if t < best_t:
                     best, best_t = cfg, t
-    if "variant" in best and best["engine"] == native.ENGINE_MEGAKERNEL:
+    if "variant" in best and best["engine"] == native.ENGINE_MEGAKERNEL and len(variants) == 2:
         # the two megakernel builds differ by a few per cent, about the spread
         # of one short trial: decide between them on the best of three each
         other = dict(best, variant=1 - best["variant"])

@@ -52,7 +52,7 @@ FP32_YARDSTICK_TFLOPS = 101.8
 # two launches per render (frame 1 with the tile-cost atomics, then the other
 # frames in cost order), each reading and writing the pixel's radiance and RNG
 # (per program build: 0 the default, 1 the four-wave build with spills)
-TRAFFIC_B_PER_PX = {("mandelbox_volume", 0): 140.6, ("menger_metal", 0): 144.3}
+TRAFFIC_B_PER_PX = {("mandelbox_volume", 0): 140.6, ("mandelbox_volume", 1): 150.5, ("menger_metal", 0): 144.3}
 # served: path samples per tick (every job's smallest quantum included, in
 # round-robin order); chains per job: the fewest (up to 128) that give it 2^18 lanes
 TICK_SAMPLES = 1 << 26
